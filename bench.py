@@ -1,0 +1,170 @@
+"""Benchmark of the MI355X NTT multiply engine on its headline workload.
+
+Workload: one 2^28-bit x 2^28-bit multiply (9,586,981 x 9,586,981 limbs, NTT length 2^25) of
+SplitMix64 operands (A seed 0x9e3779b97f4a7c15, B seed+1), operands and product resident in HBM.
+A step is one such multiply through the C ABI (bi_hip_mul_device).  `value` is input bits per
+second of the whole timed region (Gbit/s).  Each timed product is checked exactly (residues
+mod six primes) outside the timed region.
+
+Prints one JSON line.  With --gpus N > 1 it exits: the engine runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "2^28-bit mul ms + batched product-tree Gbit/s, learned vs plain, bit-exact"
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_info() -> dict:
+    info = {"nproc": os.cpu_count(), "model": platform.processor()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            v = v.strip()
+            if k == "Model name":
+                info["model"] = v
+            elif k == "Socket(s)":
+                info["sockets"] = int(v)
+            elif k == "Core(s) per socket":
+                info["cores_per_socket"] = int(v)
+            elif k == "Thread(s) per core":
+                info["threads_per_core"] = int(v)
+    except OSError:
+        pass
+    return info
+
+
+def cpu_baseline(a: np.ndarray, b: np.ndarray, bits: int, max_s: float = 25.0) -> dict:
+    """The threaded AVX2 restatement of the reference's NTT path (oracle/libcpu_mt.so) on the
+    host cores, timed around mul_digits with steady-clock semantics; bounded sample."""
+    import ctypes
+    from oracle import oracle
+    info = cpu_info()
+    phys = info.get("sockets", 1) * info.get("cores_per_socket", os.cpu_count() or 1)
+    threads = max(1, min(phys, 16))
+    M = oracle.mt_lib()
+    M.cpu_init_thread_pool(threads)
+    out = np.zeros(a.size + b.size + 4, np.uint32)
+    n = ctypes.c_size_t()
+    pa, pb, po = (x.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) for x in (a, b, out))
+    t0 = time.perf_counter()
+    M.cpu_mul_digits(pa, a.size, pb, b.size, 0, po, ctypes.byref(n))  # warm-up: pool + pages
+    warm = time.perf_counter() - t0
+    times = []
+    while not times or (sum(times) + warm < max_s and len(times) < 3):
+        t0 = time.perf_counter()
+        M.cpu_mul_digits(pa, a.size, pb, b.size, 0, po, ctypes.byref(n))
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return {"value": 2 * bits / med / 1e9, "unit": "Gbit/s", "cores": int(M.cpu_pool_threads()),
+            "kind": "port", "ms": med * 1e3,
+            "sample": f"{len(times)} timed 2^{bits.bit_length() - 1}x2^{bits.bit_length() - 1}-bit "
+                      f"multiplies after 1 warm-up, median; threaded AVX2 restatement of the reference "
+                      f"NTT path; host {info.get('model')}, {info.get('sockets')} socket(s) x "
+                      f"{info.get('cores_per_socket')} cores x {info.get('threads_per_core')} threads"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--bits", type=int, default=28, help="log2 of the operand size in bits")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+    from bigint_amd import engine
+    from bigint_amd import limbs as L
+
+    bits = 1 << args.bits
+    engine.init(0)
+    a = L.random_operand(bits, L.SEED)
+    b = L.random_operand(bits, L.SEED + 1)
+    dev = torch.device("cuda:0")
+    da = torch.from_numpy(a.view(np.int32)).to(dev)
+    db = torch.from_numpy(b.view(np.int32)).to(dev)
+    dout = torch.empty(a.size + b.size, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def step():
+        engine.mul_device(da.data_ptr(), a.size, db.data_ptr(), b.size, False, dout.data_ptr(),
+                          dout.numel(), stream.cuda_stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    engine.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    prof = engine.profile_read()
+    engine.profile(False)
+
+    # exactness of the timed product (identical every step), outside the timed region
+    p = dout.cpu().numpy().view(np.uint32)
+    exact = all(L.residue_mod(p, q) == L.residue_mod(a, q) * L.residue_mod(b, q) % q
+                for q in L.RESIDUE_PRIMES)
+
+    ms = wall / args.steps * 1e3
+    value = args.steps * 2 * bits / wall / 1e9
+    dom = max((k for k in prof if prof[k]["launches"]), key=lambda k: prof[k]["ms"])
+    d = prof[dom]
+    avg_ms = d["ms"] / d["launches"]
+    bytes_per_launch = d["bytes"] / d["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    n = 1 << engine.transform_log2(a.size, b.size)
+    kernels = {k: {"avg_ms": v["ms"] / v["launches"], "launches_per_step": v["launches"] / args.steps,
+                   "alg_GBs": v["bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] else None}
+               for k, v in prof.items() if v["launches"]}
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gbit/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"2^{args.bits}-bit x 2^{args.bits}-bit multiply, one product per step, "
+                               f"device-resident", "limbs": [int(a.size), int(b.size)],
+                   "ntt_length": n, "exact": bool(exact)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "alg_bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_ms, 4)},
+        "kernels": kernels,
+        "mul_ms": round(ms, 4),
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(a, b, bits)
+        line["speedup_vs_cpu"] = round(value / line["cpu_baseline"]["value"], 2)
+    print(json.dumps(line), flush=True)
+    return 0 if exact else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,467 @@
+// Host side of the NTT multiply engine: root tables, per-length plans, workspace and the C ABI
+// declared in include/bigint_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "../../include/bigint_hip.h"
+#include "ntt_plan.h"
+
+namespace bi {
+namespace {
+
+constexpr uint32_t kPh[3] = {469762049u, 2013265921u, 2281701377u};
+constexpr uint32_t kGh[3] = {3u, 31u, 3u};
+constexpr int kTileH = 4096;
+constexpr uint32_t kMaxLog = 26;
+
+uint32_t mulmod(uint64_t a, uint64_t b, uint32_t p) { return (uint32_t)((a * b) % p); }
+uint32_t powmod(uint32_t b, uint64_t e, uint32_t p) {
+    uint64_t r = 1, x = b % p;
+    while (e) {
+        if (e & 1) r = r * x % p;
+        x = x * x % p;
+        e >>= 1;
+    }
+    return (uint32_t)r;
+}
+uint32_t invmod(uint32_t a, uint32_t p) { return powmod(a, p - 2, p); }  // p prime
+uint32_t to_mont(uint32_t x, uint32_t p) { return (uint32_t)(((uint64_t)x << 32) % p); }
+uint32_t bitrev(uint32_t x, uint32_t bits) {
+    uint32_t r = 0;
+    for (uint32_t i = 0; i < bits; ++i) r |= ((x >> i) & 1u) << (bits - 1 - i);
+    return r;
+}
+// primitive 2^k-th root of unity mod p (inverse root if inv)
+uint32_t root(int I, uint32_t k, bool inv) {
+    const uint32_t p = kPh[I];
+    uint32_t w = powmod(kGh[I], (uint64_t)(p - 1) >> k, p);
+    return inv ? invmod(w, p) : w;
+}
+
+struct OuterPass {
+    uint32_t log_r, log_c, log_s;
+    uint32_t nblk;
+    uint32_t* beta = nullptr;      // device [3][nblk] (null when nblk == 1)
+    uint32_t* beta_inv = nullptr;
+};
+
+struct Plan {
+    uint32_t log_n = 0;
+    std::vector<OuterPass> outer;  // forward order
+    uint32_t mid_log_r = 0, mid_nblk = 1;
+    uint32_t* mid_beta = nullptr;
+    uint32_t* mid_beta_inv = nullptr;
+    uint32_t kscale[3] = {0, 0, 0};
+};
+
+hipError_t upload_beta(uint32_t log_n, uint32_t log_r, uint32_t log_s, uint32_t nblk,
+                       uint32_t** fwd, uint32_t** inv) {
+    *fwd = *inv = nullptr;
+    if (nblk <= 1) return hipSuccess;
+    std::vector<uint32_t> hf(3 * (size_t)nblk), hi(3 * (size_t)nblk);
+    const uint32_t lb = 31u - (uint32_t)__builtin_clz(nblk);
+    (void)log_r;
+    for (int I = 0; I < 3; ++I) {
+        const uint32_t p = kPh[I];
+        // beta_blk = w_{n/S}^{bitrev_{log nblk}(blk)}
+        const uint32_t wf = root(I, log_n - log_s, false), wi = root(I, log_n - log_s, true);
+        for (uint32_t b = 0; b < nblk; ++b) {
+            const uint32_t e = bitrev(b, lb);
+            hf[I * (size_t)nblk + b] = to_mont(powmod(wf, e, p), p);
+            hi[I * (size_t)nblk + b] = to_mont(powmod(wi, e, p), p);
+        }
+    }
+    hipError_t e = hipMalloc((void**)fwd, hf.size() * 4);
+    if (e != hipSuccess) return e;
+    e = hipMalloc((void**)inv, hi.size() * 4);
+    if (e != hipSuccess) return e;
+    e = hipMemcpy(*fwd, hf.data(), hf.size() * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    return hipMemcpy(*inv, hi.data(), hi.size() * 4, hipMemcpyHostToDevice);
+}
+
+struct ProfRec {
+    int kind;
+    hipEvent_t e0, e1;
+    double bytes;
+};
+
+class Engine {
+public:
+    std::mutex mtx;
+    bool profiling = false;
+    std::vector<ProfRec> prof;
+    std::vector<hipEvent_t> event_pool;
+
+    hipEvent_t get_event() {
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    // Bracket one launch with events when profiling.
+    template <class F>
+    hipError_t timed(int kind, double bytes, hipStream_t st, F&& launch) {
+        if (!profiling) return launch();
+        ProfRec r{kind, get_event(), get_event(), bytes};
+        if (!r.e0 || !r.e1) return hipErrorOutOfMemory;
+        hipEventRecord(r.e0, st);
+        hipError_t e = launch();
+        hipEventRecord(r.e1, st);
+        prof.push_back(r);
+        return e;
+    }
+    void prof_clear() {
+        for (auto& r : prof) {
+            event_pool.push_back(r.e0);
+            event_pool.push_back(r.e1);
+        }
+        prof.clear();
+    }
+    bool ready = false;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    uint32_t* wtab = nullptr;  // [2][3][4096]
+    GarnerConsts gk{};
+    std::map<uint32_t, std::unique_ptr<Plan>> plans;
+    // workspace (grow-only)
+    uint32_t* ws = nullptr;
+    size_t ws_words = 0;
+    // staging for host-pointer calls
+    uint32_t* io = nullptr;
+    size_t io_words = 0;
+
+    int init(int dev) {
+        if (ready) return BI_HIP_OK;
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return BI_HIP_ENODEV;
+        if (dev < 0 || dev >= count) return BI_HIP_EINVAL;
+        if (hipSetDevice(dev) != hipSuccess) return BI_HIP_EDEVICE;
+        device = dev;
+        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
+            return BI_HIP_EDEVICE;
+        // root table W[m + i] = w_{2m}^{bitrev_{log m}(i)}, Montgomery form
+        std::vector<uint32_t> h(2 * 3 * (size_t)kTileH);
+        for (int dir = 0; dir < 2; ++dir)
+            for (int I = 0; I < 3; ++I) {
+                uint32_t* t = h.data() + ((size_t)dir * 3 + I) * kTileH;
+                t[0] = to_mont(1, kPh[I]);
+                for (uint32_t lm = 0; (1u << lm) < (uint32_t)kTileH; ++lm) {
+                    const uint32_t m = 1u << lm;
+                    const uint32_t w = root(I, lm + 1, dir == 1);
+                    for (uint32_t i = 0; i < m; ++i)
+                        t[m + i] = to_mont(powmod(w, bitrev(i, lm), kPh[I]), kPh[I]);
+                }
+            }
+        if (hipMalloc((void**)&wtab, h.size() * 4) != hipSuccess) return BI_HIP_ENOMEM;
+        if (hipMemcpy(wtab, h.data(), h.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return BI_HIP_EDEVICE;
+        const uint32_t p0 = kPh[0], p1 = kPh[1], p2 = kPh[2];
+        gk.inv01 = to_mont(invmod(p0 % p1, p1), p1);
+        gk.inv02 = to_mont(invmod(p0 % p2, p2), p2);
+        gk.inv12 = to_mont(invmod(p1 % p2, p2), p2);
+        ready = true;
+        return BI_HIP_OK;
+    }
+
+    Plan* plan(uint32_t log_n) {
+        auto it = plans.find(log_n);
+        if (it != plans.end()) return it->second.get();
+        auto pl = std::make_unique<Plan>();
+        pl->log_n = log_n;
+        const uint32_t n = 1u << log_n;
+        if (log_n <= 12) {
+            pl->mid_log_r = log_n;
+            pl->mid_nblk = 1;
+        } else {
+            const uint32_t q = log_n - 12;
+            std::vector<uint32_t> rs;
+            if (q <= 8) rs = {q};
+            else rs = {(q + 1) / 2, q / 2};
+            uint32_t done = 0;
+            for (uint32_t r : rs) {
+                OuterPass op;
+                op.log_r = r;
+                op.log_c = 12 - r;
+                op.log_s = log_n - done - r;
+                op.nblk = 1u << done;
+                if (upload_beta(log_n, r, op.log_s, op.nblk, &op.beta, &op.beta_inv) != hipSuccess)
+                    return nullptr;
+                pl->outer.push_back(op);
+                done += r;
+            }
+            pl->mid_log_r = 12;
+            pl->mid_nblk = n >> 12;
+            if (upload_beta(log_n, 12, 0, pl->mid_nblk, &pl->mid_beta, &pl->mid_beta_inv) !=
+                hipSuccess)
+                return nullptr;
+        }
+        for (int I = 0; I < 3; ++I) {
+            const uint32_t p = kPh[I];
+            const uint32_t r2 = mulmod(to_mont(1, p), to_mont(1, p), p);  // R^2 mod p
+            pl->kscale[I] = mulmod(invmod(n % p, p), r2, p);
+        }
+        Plan* raw = pl.get();
+        plans[log_n] = std::move(pl);
+        return raw;
+    }
+
+    // device workspace of at least `words` u32
+    uint32_t* workspace(size_t words) {
+        if (words > ws_words) {
+            if (ws) hipFree(ws);
+            ws = nullptr;
+            ws_words = 0;
+            if (hipMalloc((void**)&ws, words * 4) != hipSuccess) return nullptr;
+            ws_words = words;
+        }
+        return ws;
+    }
+
+    int run(const uint32_t* d_a, uint32_t la, const uint32_t* d_b, uint32_t lb, bool square,
+            uint32_t* d_out, hipStream_t st) {
+        const int lg = bi_hip_transform_log2(la, lb);
+        if (lg < 0) return BI_HIP_ETOOBIG;
+        Plan* pl = plan((uint32_t)lg);
+        if (!pl) return BI_HIP_ENOMEM;
+        const uint32_t n = 1u << lg;
+        const uint32_t out_len = la + lb;
+        const uint32_t ntiles = (out_len + 4095u) / 4096u;
+        const size_t need = (size_t)3 * n * (square ? 1 : 2) + ntiles + 64;
+        uint32_t* w = workspace(need);
+        if (!w) return BI_HIP_ENOMEM;
+        uint32_t* res_a = w;
+        uint32_t* res_b = square ? nullptr : w + (size_t)3 * n;
+        uint32_t* status = w + (size_t)3 * n * (square ? 1 : 2);
+        uint32_t* counter = status + ntiles;
+
+        if (timed(BI_HIP_K_FILL, 4.0 * (ntiles + 1), st,
+                  [&] { return launch_fill(status, 0u, ntiles + 1, st); }) != hipSuccess)
+            return BI_HIP_EDEVICE;
+        const double nops_d = square ? 1.0 : 2.0;
+        PassArgs a{};
+        a.job = PassJob{d_a, square ? d_a : d_b, la, square ? la : lb, res_a, res_b};
+        a.jobs = nullptr;
+        a.log_n = (uint32_t)lg;
+        a.wtab = wtab;
+        const uint32_t sq = square ? (uint32_t)kSquare : 0u;
+        const uint32_t nops = square ? 1u : 2u;
+        for (size_t k = 0; k < pl->outer.size(); ++k) {
+            const OuterPass& op = pl->outer[k];
+            a.log_r = op.log_r;
+            a.log_c = op.log_c;
+            a.log_s = op.log_s;
+            a.flags = sq | (k == 0 ? (uint32_t)kFromLimbs : 0u);
+            a.beta = op.beta;
+            a.beta_inv = nullptr;
+            a.beta_stride = op.nblk;
+            // first pass reads the limbs once; every pass writes 3 planes per operand
+            const double bytes = k == 0 ? 4.0 * ((double)la + (square ? 0.0 : (double)lb)) + 12.0 * n * nops_d
+                                        : 24.0 * n * nops_d;
+            if (timed(BI_HIP_K_OUTER_FWD, bytes, st,
+                      [&] { return launch_outer_pass(a, false, 1, nops, st); }) != hipSuccess)
+                return BI_HIP_EDEVICE;
+        }
+        a.log_r = pl->mid_log_r;
+        a.log_c = 0;
+        a.log_s = 0;
+        a.flags = sq | (pl->outer.empty() ? (uint32_t)kFromLimbs : 0u);
+        a.beta = pl->mid_beta;
+        a.beta_inv = pl->mid_beta_inv;
+        a.beta_stride = pl->mid_nblk;
+        {
+            const double in_bytes = pl->outer.empty()
+                                        ? 4.0 * ((double)la + (square ? 0.0 : (double)lb))
+                                        : 12.0 * n * nops_d;
+            if (timed(BI_HIP_K_MID, in_bytes + 12.0 * n, st,
+                      [&] { return launch_mid_pass(a, 1, st); }) != hipSuccess)
+                return BI_HIP_EDEVICE;
+        }
+        for (size_t k = pl->outer.size(); k-- > 0;) {
+            const OuterPass& op = pl->outer[k];
+            a.log_r = op.log_r;
+            a.log_c = op.log_c;
+            a.log_s = op.log_s;
+            a.flags = 0;
+            a.beta = op.beta_inv;
+            a.beta_inv = nullptr;
+            a.beta_stride = op.nblk;
+            if (timed(BI_HIP_K_OUTER_INV, 24.0 * n, st,
+                      [&] { return launch_outer_pass(a, true, 1, 1, st); }) != hipSuccess)
+                return BI_HIP_EDEVICE;
+        }
+        GarnerArgs g{};
+        g.job.y = res_a;
+        g.job.out = d_out;
+        g.job.n = n;
+        g.job.out_len = out_len;
+        for (int I = 0; I < 3; ++I) g.job.kscale[I] = pl->kscale[I];
+        g.job.status = status;
+        g.job.counter = counter;
+        g.jobs = nullptr;
+        g.k = gk;
+        if (timed(BI_HIP_K_GARNER, 12.0 * n + 4.0 * out_len, st,
+                  [&] { return launch_garner_carry(g, out_len, 1, st); }) != hipSuccess)
+            return BI_HIP_EDEVICE;
+        return BI_HIP_OK;
+    }
+};
+
+Engine& engine() {
+    static Engine e;
+    return e;
+}
+
+int check_operand(const uint32_t* a, size_t la) {
+    if (!a || la == 0) return BI_HIP_EINVAL;
+    if (la > 0xFFFFFFFFull) return BI_HIP_ETOOBIG;
+    return BI_HIP_OK;
+}
+
+}  // namespace
+}  // namespace bi
+
+using bi::engine;
+
+extern "C" {
+
+int bi_hip_init(int device) {
+    std::lock_guard<std::mutex> lk(engine().mtx);
+    return engine().init(device);
+}
+
+int bi_hip_transform_log2(size_t la, size_t lb) {
+    if (la == 0 || lb == 0) return -1;
+    const size_t need = la + lb - 1;
+    if (need > ((size_t)1 << bi::kMaxLog)) return -1;
+    int lg = 4;  // at least one 16-point tile
+    while (((size_t)1 << lg) < need) ++lg;
+    return lg;
+}
+
+int bi_hip_mul_device(const uint32_t* d_a, size_t la, const uint32_t* d_b, size_t lb,
+                      int is_square, uint32_t* d_out, size_t out_cap, void* stream) {
+    if (bi::check_operand(d_a, la) || (!is_square && bi::check_operand(d_b, lb)) || !d_out)
+        return BI_HIP_EINVAL;
+    if (is_square) lb = la;
+    if (out_cap < la + lb) return BI_HIP_ECAP;
+    std::lock_guard<std::mutex> lk(engine().mtx);
+    int rc = engine().init(0);
+    if (rc) return rc;
+    hipStream_t st = stream ? (hipStream_t)stream : engine().stream;
+    return engine().run(d_a, (uint32_t)la, is_square ? d_a : d_b, (uint32_t)lb, is_square != 0,
+                        d_out, st);
+}
+
+int bi_hip_mul(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, int is_square,
+               uint32_t* out, size_t out_cap, size_t* out_len) {
+    if (bi::check_operand(a, la) || (!is_square && bi::check_operand(b, lb)) || !out || !out_len)
+        return BI_HIP_EINVAL;
+    if (is_square) {
+        b = a;
+        lb = la;
+    }
+    if (a[la - 1] == 0 || b[lb - 1] == 0) return BI_HIP_EINVAL;
+    if (out_cap < la + lb) return BI_HIP_ECAP;
+    if (bi_hip_transform_log2(la, lb) < 0) return BI_HIP_ETOOBIG;
+    std::lock_guard<std::mutex> lk(engine().mtx);
+    bi::Engine& e = engine();
+    int rc = e.init(0);
+    if (rc) return rc;
+    const size_t words = la + (is_square ? 0 : lb) + la + lb;
+    if (words > e.io_words) {
+        if (e.io) hipFree(e.io);
+        e.io = nullptr;
+        e.io_words = 0;
+        if (hipMalloc((void**)&e.io, words * 4) != hipSuccess) return BI_HIP_ENOMEM;
+        e.io_words = words;
+    }
+    uint32_t* da = e.io;
+    uint32_t* db = is_square ? da : da + la;
+    uint32_t* dout = da + la + (is_square ? 0 : lb);
+    if (hipMemcpyAsync(da, a, la * 4, hipMemcpyHostToDevice, e.stream) != hipSuccess)
+        return BI_HIP_EDEVICE;
+    if (!is_square &&
+        hipMemcpyAsync(db, b, lb * 4, hipMemcpyHostToDevice, e.stream) != hipSuccess)
+        return BI_HIP_EDEVICE;
+    rc = e.run(da, (uint32_t)la, db, (uint32_t)lb, is_square != 0, dout, e.stream);
+    if (rc) return rc;
+    if (hipMemcpyAsync(out, dout, (la + lb) * 4, hipMemcpyDeviceToHost, e.stream) != hipSuccess)
+        return BI_HIP_EDEVICE;
+    if (hipStreamSynchronize(e.stream) != hipSuccess) return BI_HIP_EDEVICE;
+    size_t n = la + lb;
+    while (n > 1 && out[n - 1] == 0) --n;
+    *out_len = n;
+    return BI_HIP_OK;
+}
+
+int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode) {
+    (void)plan_mode;
+    int first = BI_HIP_OK;
+    for (size_t i = 0; i < count; ++i) {
+        bi_hip_pair& p = pairs[i];
+        const bool sq = p.a == p.b && p.la == p.lb;
+        p.status = bi_hip_mul(p.a, p.la, p.b, p.lb, sq, p.out, p.out_cap, &p.out_len);
+        if (p.status && !first) first = p.status;
+    }
+    return first;
+}
+
+int bi_hip_profile(int enable) {
+    std::lock_guard<std::mutex> lk(engine().mtx);
+    bi::Engine& e = engine();
+    if (enable) {
+        if (e.ready) hipStreamSynchronize(e.stream);
+        e.prof_clear();
+    }
+    e.profiling = enable != 0;
+    return BI_HIP_OK;
+}
+
+int bi_hip_profile_read(bi_hip_kstat* stats, size_t cap) {
+    if (!stats || cap < BI_HIP_K_COUNT) return BI_HIP_EINVAL;
+    std::lock_guard<std::mutex> lk(engine().mtx);
+    bi::Engine& e = engine();
+    for (size_t i = 0; i < BI_HIP_K_COUNT; ++i) stats[i] = bi_hip_kstat{0.0, 0.0, 0};
+    for (auto& r : e.prof) {
+        if (hipEventSynchronize(r.e1) != hipSuccess) return BI_HIP_EDEVICE;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.e0, r.e1) != hipSuccess) return BI_HIP_EDEVICE;
+        stats[r.kind].ms += ms;
+        stats[r.kind].bytes += r.bytes;
+        stats[r.kind].launches += 1;
+    }
+    return BI_HIP_OK;
+}
+
+int bi_hip_synchronize(void) {
+    std::lock_guard<std::mutex> lk(engine().mtx);
+    if (!engine().ready) return BI_HIP_OK;
+    return hipStreamSynchronize(engine().stream) == hipSuccess ? BI_HIP_OK : BI_HIP_EDEVICE;
+}
+
+const char* bi_hip_strerror(int code) {
+    switch (code) {
+        case BI_HIP_OK: return "ok";
+        case BI_HIP_EINVAL: return "invalid argument";
+        case BI_HIP_ETOOBIG: return "product needs a transform longer than 2^26 points";
+        case BI_HIP_ENOMEM: return "out of memory";
+        case BI_HIP_EDEVICE: return "HIP runtime error";
+        case BI_HIP_ENODEV: return "no gfx950 device";
+        case BI_HIP_ECAP: return "output capacity too small";
+        default: return "unknown status";
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,67 @@
+// Device-side modular arithmetic for the three NTT primes (gfx950 / CDNA4).
+//
+// Primes, generators and the 28-bit limb width are the reference's
+// (include/bigint/ntt_base.h:14-20); Montgomery reduction uses R = 2^32 as in
+// include/bigint/ntt_base.h:40-71.  The prime index is a template parameter so every modulus and
+// its -P^-1 mod 2^32 are immediates in the generated code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bi {
+
+constexpr int kNumPrimes = 3;
+constexpr uint32_t kP[kNumPrimes] = {469762049u, 2013265921u, 2281701377u};
+constexpr uint32_t kG[kNumPrimes] = {3u, 31u, 3u};
+// -P^-1 mod 2^32.  Each P = k*2^m + 1 with 2m >= 32, so P*(P-2) = (P-1)^2 - 1 == -1 mod 2^32.
+constexpr uint32_t kNegInvP[kNumPrimes] = {kP[0] - 2u, kP[1] - 2u, kP[2] - 2u};
+constexpr uint32_t kDigitBits = 28;
+constexpr uint32_t kDigitMask = (1u << kDigitBits) - 1u;
+constexpr int kMaxLogN = 26;  // smallest 2-adic valuation of P-1 (ntt_base.h:22-27)
+
+static_assert(kP[0] * (kP[0] - 2u) == 0xFFFFFFFFu, "P0 Montgomery constant");
+static_assert(kP[1] * (kP[1] - 2u) == 0xFFFFFFFFu, "P1 Montgomery constant");
+static_assert(kP[2] * (kP[2] - 2u) == 0xFFFFFFFFu, "P2 Montgomery constant");
+
+// a*b*2^-32 mod P for a, b < P.  t + u*P < P*(P + 2^32) < 2^64 for all three primes, and the
+// reduced value is below 1.54*P < 2^32, so one conditional subtraction finishes it.
+template <int I>
+__device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t b) {
+    constexpr uint32_t p = kP[I];
+    const uint64_t t = (uint64_t)a * b;
+    const uint32_t u = (uint32_t)t * kNegInvP[I];
+    const uint64_t s = t + (uint64_t)u * p;
+    const uint32_t r = (uint32_t)(s >> 32);
+    const uint32_t r2 = r - p;
+    return r2 < r ? r2 : r;  // unsigned min: r - p wraps above r exactly when r < p
+}
+
+// (a + b) mod P for a, b < P.
+template <int I>
+__device__ __forceinline__ uint32_t add_mod(uint32_t a, uint32_t b) {
+    constexpr uint32_t p = kP[I];
+    if constexpr (p < 0x80000000u) {
+        const uint32_t s = a + b;  // < 2p < 2^32
+        const uint32_t s2 = s - p;
+        return s2 < s ? s2 : s;
+    } else {
+        const uint32_t d = a - (p - b);  // a + b - p, wraps when a + b < p
+        return a >= p - b ? d : d + p;
+    }
+}
+
+// (a - b) mod P for a, b < P.
+template <int I>
+__device__ __forceinline__ uint32_t sub_mod(uint32_t a, uint32_t b) {
+    constexpr uint32_t p = kP[I];
+    const uint32_t d = a - b;
+    if constexpr (p < 0x80000000u) {
+        const uint32_t d2 = d + p;  // < 2p < 2^32 when no borrow
+        return d2 < d ? d2 : d;
+    } else {
+        return a >= b ? d : d + p;
+    }
+}
+
+}  // namespace bi

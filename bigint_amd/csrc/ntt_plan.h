@@ -1,0 +1,62 @@
+// Argument blocks shared by the kernels and the host engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bi {
+
+enum PassFlags : uint32_t {
+    kFromLimbs = 1u,  // the pass reads 28-bit limbs (zero past la/lb) instead of residues
+    kSquare = 2u,     // B is A: one forward transform, pointwise square
+};
+
+// One product inside a batched launch.  Residue buffers are planar: 3 planes of n u32.
+struct PassJob {
+    const uint32_t* limbs_a;
+    const uint32_t* limbs_b;
+    uint32_t la, lb;
+    uint32_t* res_a;
+    uint32_t* res_b;
+};
+
+struct PassArgs {
+    PassJob job;            // used when jobs == nullptr
+    const PassJob* jobs;    // device array, indexed by blockIdx.y
+    uint32_t log_n;         // transform length n = 2^log_n
+    uint32_t log_r, log_c;  // tile: R rows x C columns
+    uint32_t log_s;         // row stride S
+    uint32_t flags;
+    const uint32_t* wtab;      // root table: [fwd/inv][prime][4096], Montgomery form
+    const uint32_t* beta;      // per-block roots for this pass, [prime][beta_stride], or null
+    const uint32_t* beta_inv;  // inverse roots (block pass only)
+    uint32_t beta_stride;
+};
+
+struct GarnerConsts {
+    uint32_t inv01, inv02, inv12;  // P0^-1 mod P1, P0^-1 mod P2, P1^-1 mod P2 (Montgomery form)
+};
+
+struct GarnerJob {
+    const uint32_t* y;   // 3 planes of n: inverse transform output (times n * R^-1)
+    uint32_t* out;       // out_len limbs
+    uint32_t n, out_len;
+    uint32_t kscale[3];  // n^-1 * R^2 mod P_i, so mont_mul(y, kscale) = y * n^-1 * R
+    uint32_t* status;    // look-back status words, one per 4096-limb tile, zeroed before launch
+    uint32_t* counter;   // dynamic tile counter, zeroed before launch
+};
+
+struct GarnerArgs {
+    GarnerJob job;
+    const GarnerJob* jobs;
+    GarnerConsts k;
+};
+
+hipError_t launch_outer_pass(const PassArgs& a, bool inv, uint32_t njobs, uint32_t nops,
+                             hipStream_t st);
+hipError_t launch_mid_pass(const PassArgs& a, uint32_t njobs, hipStream_t st);
+hipError_t launch_garner_carry(const GarnerArgs& a, uint32_t max_out_len, uint32_t njobs,
+                               hipStream_t st);
+hipError_t launch_fill(uint32_t* p, uint32_t v, uint32_t count, hipStream_t st);
+
+}  // namespace bi

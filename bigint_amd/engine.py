@@ -1,0 +1,143 @@
+"""ctypes binding of the C ABI in include/bigint_hip.h (libbigint_hip.so, built in-tree).
+
+The shared library must exist; nothing here falls back to a CPU path.  Tests and bench.py call
+the engine through this module, i.e. through the same C ABI a C/C++ caller uses.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbigint_hip.so")
+
+OK, EINVAL, ETOOBIG, ENOMEM, EDEVICE, ENODEV, ECAP = range(7)
+PLAN_FIXED, PLAN_TABLE, PLAN_LEARNED = 0, 1, 2
+
+EXPORTS = ["bi_hip_init", "bi_hip_mul", "bi_hip_mul_device", "bi_hip_mul_batch",
+           "bi_hip_synchronize", "bi_hip_transform_log2", "bi_hip_strerror", "bi_hip_profile",
+           "bi_hip_profile_read"]
+KERNEL_NAMES = ["fill", "outer_fwd", "mid", "outer_inv", "garner_carry"]
+
+u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+class Pair(ctypes.Structure):
+    _fields_ = [("a", u32p), ("la", ctypes.c_size_t), ("b", u32p), ("lb", ctypes.c_size_t),
+                ("out", u32p), ("out_cap", ctypes.c_size_t), ("out_len", ctypes.c_size_t),
+                ("status", ctypes.c_int)]
+
+
+class KStat(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double), ("bytes", ctypes.c_double), ("launches", ctypes.c_uint64)]
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code: int, what: str = ""):
+        self.code = code
+        super().__init__(f"{what}: {strerror(code)} ({code})" if what else strerror(code))
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libbigint_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.bi_hip_init.argtypes = [ctypes.c_int]
+        L.bi_hip_mul.argtypes = [u32p, ctypes.c_size_t, u32p, ctypes.c_size_t, ctypes.c_int, u32p,
+                                 ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+        L.bi_hip_mul_device.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                                        ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.c_size_t, ctypes.c_void_p]
+        L.bi_hip_mul_batch.argtypes = [ctypes.POINTER(Pair), ctypes.c_size_t, ctypes.c_int]
+        L.bi_hip_transform_log2.argtypes = [ctypes.c_size_t, ctypes.c_size_t]
+        L.bi_hip_profile.argtypes = [ctypes.c_int]
+        L.bi_hip_profile_read.argtypes = [ctypes.POINTER(KStat), ctypes.c_size_t]
+        L.bi_hip_strerror.restype = ctypes.c_char_p
+        L.bi_hip_strerror.argtypes = [ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def strerror(code: int) -> str:
+    return lib().bi_hip_strerror(code).decode()
+
+
+def init(device: int = 0) -> None:
+    rc = lib().bi_hip_init(device)
+    if rc:
+        raise EngineError(rc, "bi_hip_init")
+
+
+def transform_log2(la: int, lb: int) -> int:
+    return int(lib().bi_hip_transform_log2(la, lb))
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(u32p)
+
+
+def mul(a: np.ndarray, b: np.ndarray | None = None) -> np.ndarray:
+    """Product of two 28-bit limb arrays (host memory); b=None squares a."""
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    sq = b is None
+    bb = a if sq else np.ascontiguousarray(b, dtype=np.uint32)
+    out = np.zeros(a.size + bb.size, dtype=np.uint32)
+    n = ctypes.c_size_t()
+    rc = lib().bi_hip_mul(_p(a), a.size, _p(bb), bb.size, int(sq), _p(out), out.size, ctypes.byref(n))
+    if rc:
+        raise EngineError(rc, "bi_hip_mul")
+    return out[: n.value]
+
+
+def mul_device(d_a: int, la: int, d_b: int, lb: int, square: bool, d_out: int, out_cap: int,
+               stream: int = 0) -> None:
+    """Device-resident product; pointers are raw device addresses (e.g. tensor.data_ptr())."""
+    rc = lib().bi_hip_mul_device(d_a, la, d_b, lb, int(square), d_out, out_cap, stream or None)
+    if rc:
+        raise EngineError(rc, "bi_hip_mul_device")
+
+
+def mul_batch(pairs: list[tuple[np.ndarray, np.ndarray | None]], plan_mode: int = PLAN_FIXED):
+    """Batch of products on host arrays; returns the list of products."""
+    arr = (Pair * len(pairs))()
+    keep, outs = [], []
+    for i, (a, b) in enumerate(pairs):
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        bb = a if b is None else np.ascontiguousarray(b, dtype=np.uint32)
+        out = np.zeros(a.size + bb.size, dtype=np.uint32)
+        keep += [a, bb]
+        outs.append(out)
+        arr[i] = Pair(_p(a), a.size, _p(bb), bb.size, _p(out), out.size, 0, 0)
+    rc = lib().bi_hip_mul_batch(arr, len(pairs), plan_mode)
+    if rc:
+        raise EngineError(rc, "bi_hip_mul_batch")
+    return [outs[i][: arr[i].out_len] for i in range(len(pairs))]
+
+
+def synchronize() -> None:
+    rc = lib().bi_hip_synchronize()
+    if rc:
+        raise EngineError(rc, "bi_hip_synchronize")
+
+
+def profile(enable: bool) -> None:
+    lib().bi_hip_profile(int(enable))
+
+
+def profile_read() -> dict:
+    """Per kernel class: {'ms': total event ms, 'bytes': algorithmic bytes, 'launches': count}."""
+    arr = (KStat * len(KERNEL_NAMES))()
+    rc = lib().bi_hip_profile_read(arr, len(KERNEL_NAMES))
+    if rc:
+        raise EngineError(rc, "bi_hip_profile_read")
+    return {name: {"ms": arr[i].ms, "bytes": arr[i].bytes, "launches": int(arr[i].launches)}
+            for i, name in enumerate(KERNEL_NAMES)}

@@ -1,0 +1,120 @@
+/*
+ * bigint_hip.h — C ABI of the MI355X (gfx950) NTT multiply engine.
+ *
+ * Plain C: pointers, sizes and status codes only.  Limbs are the reference's Digits layout:
+ * uint32_t, little-endian, each below 2^28 (include/bigint/bigint_base.h:21-26).
+ *
+ * The engine replaces, one for one, the reference's multiply seam below BigInt/BigFloat:
+ *   bi_hip_mul          <- bigint::mul::ntt::mul(span<const u64>, span<const u64>)
+ *                          (include/bigint/mul.h:194, src/mul/ntt_multithread.cpp:444-530), taking
+ *                          28-bit limbs directly instead of the u64 repack of mul.h:245-300
+ *   bi_hip_mul_device   <- the same product with device-resident operands (no reference
+ *                          counterpart: the reference has no device memory)
+ *   bi_hip_mul_batch    <- a loop of `operator*` calls at one product-tree level
+ *                          (examples/pi.cpp:45-58, examples/binary_split.cpp:23-75), done as one
+ *                          batched launch sequence
+ *   bi_hip_init         <- first-use initialisation done lazily by get_pool()
+ *                          (src/mul/ntt_multithread.cpp:287-290)
+ * Preconditions follow the reference: operands are nonzero with a nonzero top limb
+ * (mul.h:222, 232).  Where the reference hits unreachable() (bigint_base.h:40-46) this ABI
+ * returns BI_HIP_EINVAL instead.  A product needing a transform longer than 2^26 points
+ * (ntt_base.h:22-27; the reference sends it to SSA, mul.h:199-219) returns BI_HIP_ETOOBIG.
+ *
+ * Thread safety: every entry point may be called from any thread; calls are serialised by an
+ * internal mutex (the reference serialises pool phases the same way,
+ * src/mul/ntt_multithread.cpp:259-260).
+ */
+#ifndef BIGINT_HIP_H
+#define BIGINT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    BI_HIP_OK = 0,
+    BI_HIP_EINVAL = 1,   /* bad argument: null pointer, zero length, top limb zero, limb >= 2^28 */
+    BI_HIP_ETOOBIG = 2,  /* la + lb - 1 > 2^26 (beyond the three primes' transform length) */
+    BI_HIP_ENOMEM = 3,   /* device or host allocation failed */
+    BI_HIP_EDEVICE = 4,  /* HIP runtime error */
+    BI_HIP_ENODEV = 5,   /* no gfx950 device */
+    BI_HIP_ECAP = 6      /* output capacity too small */
+};
+
+/* Plan modes for bi_hip_mul_batch. */
+enum {
+    BI_HIP_PLAN_FIXED = 0,   /* one launch sequence per product, default transform plan */
+    BI_HIP_PLAN_TABLE = 1,   /* autotuned plan table */
+    BI_HIP_PLAN_LEARNED = 2  /* learned cost model, table as fallback */
+};
+
+/* One product of a batch.  out must hold la + lb limbs; out_len and status are written. */
+typedef struct bi_hip_pair {
+    const uint32_t* a;
+    size_t la;
+    const uint32_t* b; /* may equal a (with lb == la) for a square */
+    size_t lb;
+    uint32_t* out;
+    size_t out_cap;
+    size_t out_len;
+    int status;
+} bi_hip_pair;
+
+/* Select the device and build the root tables.  Idempotent; later calls with another device
+ * are ignored (first call wins, like init_thread_pool, include/bigint/bigint.h:35-38). */
+int bi_hip_init(int device);
+
+/* out = a * b for host limb arrays.  is_square != 0 means b is a (the reference detects this by
+ * address, include/bigint/mul.h:225).  *out_len receives the product length without leading zero
+ * limbs (la + lb or la + lb - 1).  out_cap must be >= la + lb. */
+int bi_hip_mul(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, int is_square,
+               uint32_t* out, size_t out_cap, size_t* out_len);
+
+/* Device-resident variant: a, b, out are device pointers; exactly la + lb limbs are written
+ * (the top one may be zero).  Asynchronous on `stream` (a hipStream_t; NULL = the engine's
+ * stream). */
+int bi_hip_mul_device(const uint32_t* d_a, size_t la, const uint32_t* d_b, size_t lb,
+                      int is_square, uint32_t* d_out, size_t out_cap, void* stream);
+
+/* Batch of independent products (host pointers).  Each pair gets its own status; the call
+ * returns the first nonzero status or BI_HIP_OK. */
+int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode);
+
+/* Wait for all work queued by the engine. */
+int bi_hip_synchronize(void);
+
+/* Transform length (log2) the engine uses for an la x lb product, or -1 if too large. */
+int bi_hip_transform_log2(size_t la, size_t lb);
+
+/* Per-kernel-class timing, for benchmarks: while enabled, every engine launch is bracketed by
+ * HIP events on the stream it runs on, and its algorithmic HBM bytes are accumulated. */
+enum {
+    BI_HIP_K_FILL = 0,      /* status/counter clearing */
+    BI_HIP_K_OUTER_FWD = 1, /* strided forward passes (first one reads limbs) */
+    BI_HIP_K_MID = 2,       /* contiguous block pass: forward + pointwise + inverse */
+    BI_HIP_K_OUTER_INV = 3, /* strided inverse passes */
+    BI_HIP_K_GARNER = 4,    /* Garner CRT + carry look-back scan */
+    BI_HIP_K_COUNT = 5
+};
+typedef struct bi_hip_kstat {
+    double ms;       /* summed event time */
+    double bytes;    /* summed algorithmic HBM bytes */
+    uint64_t launches;
+} bi_hip_kstat;
+
+/* enable != 0 clears the counters and starts recording; 0 stops. */
+int bi_hip_profile(int enable);
+/* Waits for the recorded launches and copies BI_HIP_K_COUNT entries into stats. */
+int bi_hip_profile_read(bi_hip_kstat* stats, size_t cap);
+
+/* Human-readable text for a status code. */
+const char* bi_hip_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BIGINT_HIP_H */

@@ -1,0 +1,41 @@
+"""The C-ABI library loads and exports every symbol include/bigint_hip.h declares (CPU only:
+no compute calls are made without a GPU)."""
+import os
+import re
+
+import pytest
+
+from bigint_amd import engine
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    with open(os.path.join(ROOT, "include", "bigint_hip.h")) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(bi_hip_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_matches_binding_list():
+    assert declared_symbols() == sorted(engine.EXPORTS)
+
+
+def test_library_exports_every_symbol():
+    if not os.path.exists(engine.LIB_PATH):
+        pytest.skip("libbigint_hip.so not built")
+    L = engine.lib()
+    for s in declared_symbols():
+        assert hasattr(L, s), s
+
+
+def test_host_only_entry_points():
+    if not os.path.exists(engine.LIB_PATH):
+        pytest.skip("libbigint_hip.so not built")
+    assert engine.strerror(0) == "ok"
+    assert engine.transform_log2(1, 1) == 4
+    assert engine.transform_log2(4096, 1) == 12
+    assert engine.transform_log2(4096, 2) == 13
+    assert engine.transform_log2(1 << 25, 1 << 25) == 26
+    assert engine.transform_log2(1 << 25, (1 << 25) + 2) == -1
+    # the 2^28 x 2^28 workload: 9,586,981 limbs each -> n = 2^25
+    assert engine.transform_log2(9586981, 9586981) == 25

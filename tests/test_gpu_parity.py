@@ -1,0 +1,145 @@
+"""Parity of the HIP engine (through the C ABI) with the oracle and the golden products.
+
+Bar: bit-exact limbs.  Sizes the oracle finishes in seconds are compared limb for limb with the
+oracle or with the committed CPython-int fixtures; the full 2^26 and 2^28 workloads are checked
+through size-independent properties: residues modulo six primes below 2^31
+((a mod q)(b mod q) == (a*b) mod q), commutativity and square == general product.
+"""
+import hashlib
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+from bigint_amd import engine
+from bigint_amd import limbs as L
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "products.json")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    engine.init(0)
+
+
+def sha(limbs):
+    return hashlib.sha256(np.ascontiguousarray(limbs, dtype="<u4").tobytes()).hexdigest()
+
+
+def check_residues(a, b, p):
+    for q in L.RESIDUE_PRIMES:
+        ra = L.residue_mod(a, q)
+        rb = ra if b is None else L.residue_mod(b, q)
+        assert L.residue_mod(p, q) == ra * rb % q, q
+
+
+with open(GOLDEN) as _f:
+    _CASES = json.load(_f)["cases"]
+
+
+@pytest.mark.parametrize("case", _CASES, ids=lambda c: c["name"])
+def test_golden(case):
+    a = L.random_operand(case["bits_a"], case["seed_a"])
+    b = None if case["bits_b"] is None else L.random_operand(case["bits_b"], case["seed_b"])
+    p = engine.mul(a, b)
+    assert p.size == case["n_limbs"]
+    assert sha(p) == case["sha256"]
+
+
+def test_every_small_length_against_oracle():
+    """All la, lb in 1..40 plus lengths around every transform-length boundary up to 2^13."""
+    rng = np.random.default_rng(3)
+    sizes = [(la, lb) for la in range(1, 41) for lb in range(1, 41, 3)]
+    for k in range(4, 14):
+        for d in (-1, 0, 1):
+            tot = (1 << k) + 1 + d
+            sizes += [(tot // 2, tot - tot // 2), (1, tot - 1), (tot - 3, 3)]
+    for la, lb in sizes:
+        if la < 1 or lb < 1:
+            continue
+        a = rng.integers(0, 1 << 28, la, dtype=np.uint32)
+        b = rng.integers(0, 1 << 28, lb, dtype=np.uint32)
+        a[-1] |= 1
+        b[-1] |= 1
+        got = engine.mul(a, b)
+        exp = oracle.mul_limbs(a, b)
+        assert np.array_equal(got, exp), (la, lb)
+
+
+@pytest.mark.parametrize("bits", [28, 29, 1000, 28 * 2048, 28 * 2049, 28 * 4096, 123457, 1 << 20])
+def test_squares_against_oracle(bits):
+    a = L.random_operand(bits, 100 + bits)
+    assert np.array_equal(engine.mul(a, None), oracle.mul_limbs(a, None))
+    assert np.array_equal(engine.mul(a, None), engine.mul(a, a.copy()))
+
+
+@pytest.mark.parametrize("a_bits,b_bits", [(28, 28), (28 * 8, 28 * 8), (28 * 2048, 28 * 2048),
+                                           (28 * 4096, 28 * 4096), (28 * 70000, 28 * 70000),
+                                           (28 * 300000, 28 * 5000), (1 << 22, 1 << 22)])
+def test_all_ones_worst_case(a_bits, b_bits):
+    """(2^a - 1)(2^b - 1): every limb 2^28-1 maximises every convolution coefficient."""
+    x, y = (1 << a_bits) - 1, (1 << b_bits) - 1
+    p = engine.mul(L.int_to_limbs(x), L.int_to_limbs(y))
+    assert L.limbs_to_int(p) == (1 << (a_bits + b_bits)) - (1 << a_bits) - (1 << b_bits) + 1
+
+
+def test_random_mid_sizes_against_oracle():
+    rng = random.Random(17)
+    for _ in range(25):
+        la = int(2 ** rng.uniform(4, 17))
+        lb = int(2 ** rng.uniform(0, 17))
+        a = L.random_operand(28 * la - rng.randrange(28), rng.getrandbits(64))
+        b = L.random_operand(28 * lb - rng.randrange(28), rng.getrandbits(64))
+        assert np.array_equal(engine.mul(a, b), oracle.mul_limbs(a, b)), (la, lb)
+
+
+@pytest.mark.parametrize("a_bits,b_bits", [(1 << 26, 1 << 26), (1 << 28, 1 << 20), (1 << 28, 1 << 28)])
+def test_large_workloads_residues(a_bits, b_bits):
+    a = L.random_operand(a_bits, L.SEED)
+    b = L.random_operand(b_bits, L.SEED + 1)
+    p = engine.mul(a, b)
+    assert p.size in (a.size + b.size, a.size + b.size - 1)
+    assert p[-1] != 0
+    check_residues(a, b, p)
+    if a_bits == b_bits == 1 << 26:
+        assert np.array_equal(p, engine.mul(b, a))
+        s = engine.mul(a, None)
+        check_residues(a, None, s)
+
+
+def test_max_transform_length():
+    """la + lb - 1 = 2^26 exactly: the largest product the three primes allow."""
+    la = (1 << 25) + 1
+    lb = (1 << 25)
+    a = L.random_operand(28 * la, 5)
+    b = L.random_operand(28 * lb, 6)
+    assert engine.transform_log2(la, lb) == 26
+    p = engine.mul(a, b)
+    check_residues(a, b, p)
+
+
+def test_error_paths():
+    a = np.array([1, 2, 3], np.uint32)
+    with pytest.raises(engine.EngineError) as e:
+        engine.mul(np.array([1, 0], np.uint32), a)  # top limb zero
+    assert e.value.code == engine.EINVAL
+    big = np.ones((1 << 25) + 2, np.uint32)
+    with pytest.raises(engine.EngineError) as e:
+        engine.mul(big, big)
+    assert e.value.code == engine.ETOOBIG
+
+
+def test_batch_matches_single():
+    rng = random.Random(4)
+    pairs = []
+    for _ in range(12):
+        a = L.random_operand(rng.randrange(28, 200000), rng.getrandbits(63))
+        b = None if rng.random() < 0.3 else L.random_operand(rng.randrange(28, 200000), rng.getrandbits(63))
+        pairs.append((a, b))
+    outs = engine.mul_batch(pairs)
+    for (a, b), p in zip(pairs, outs):
+        assert np.array_equal(p, oracle.mul_limbs(a, b))
