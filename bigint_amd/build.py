@@ -20,6 +20,11 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++20", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + os.path.join(ROOT, "include"), "-Wall"]
 SOURCES = ["ntt_kernels.hip", "engine.cpp"]
+HOST_SOURCES = ["host/mul.cpp", "host/bigint.cpp"]
+HOST_LIB = os.path.join(HERE, "libbigint.so")
+CXX = os.environ.get("BIGINT_CXX", "/opt/rocm/llvm/bin/clang++")
+HOST_FLAGS = ["-O2", "-std=gnu++20", "-fPIC", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"),
+              "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -45,6 +50,24 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # host C++ library (BigInt/BigFloat API) on top of the C ABI
+    hobjs = []
+    for src in HOST_SOURCES:
+        obj = os.path.join(bdir, os.path.basename(src) + ".o")
+        cmd = [CXX] + HOST_FLAGS + ["-c", os.path.join(HERE, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        _run(cmd)
+        hobjs.append(obj)
+    tmp = HOST_LIB + ".tmp"
+    _run([CXX, "-shared", "-fPIC", "-o", tmp] + hobjs + ["-L" + HERE, "-lbigint_hip",
+                                                        "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, HOST_LIB)
+    # test driver of the host API
+    calc = os.path.join(bdir, "bigint_calc")
+    _run([CXX] + HOST_FLAGS + [os.path.join(ROOT, "tests", "cpp", "bigint_calc.cpp"), "-o", calc,
+                               "-L" + HERE, "-lbigint", "-lbigint_hip",
+                               "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/.."])
     return LIB
 
 

@@ -1,0 +1,815 @@
+// Host implementation of BigInt / BigFloat (include/bigint/bigint.h).
+//
+// Behaviour follows the reference's src/bigint.cpp function by function (cited below) so that
+// the same sequence of calls yields the same limbs, point positions and decimal text; the big
+// products underneath go to the MI355X engine through mul::mul_digits.
+#include "bigint/bigint.h"
+
+#include <algorithm>
+#include <atomic>
+#include <bit>
+#include <cmath>
+#include <cstring>
+#include <iomanip>
+#include <iostream>
+#include <mutex>
+#include <optional>
+#include <thread>
+#include <vector>
+
+namespace bigint {
+
+// ---- init_thread_pool (reference bigint.h:35-38, src/mul/ntt_multithread.cpp:287-290, 407-409) --
+namespace {
+std::once_flag g_pool_once;
+std::atomic<uint32_t> g_pool_threads{0};
+void pool_set(uint32_t n) {
+    std::call_once(g_pool_once, [n] { g_pool_threads.store(n); });
+}
+}  // namespace
+
+void init_thread_pool(uint32_t n) { pool_set(n == 0 ? 1 : n); }
+
+auto thread_pool_size() -> uint32_t {
+    pool_set(std::max(1u, std::thread::hardware_concurrency()));
+    return g_pool_threads.load();
+}
+
+std::array<size_type, 2> BigInt::DEC_STRING_BRUTE_THRESHOLDS = DEC_STRING_BRUTE_THRESHOLDS_DEFAULT;
+
+// ---- parsing (src/bigint.cpp:25-124) ------------------------------------------------------------
+BigInt::BigInt(const std::string& s, bool hex) : BigInt() {
+    if (hex) {
+        auto val = [](unsigned char c) -> int {
+            if (c >= '0' && c <= '9') return c - '0';
+            if (c >= 'a' && c <= 'f') return c - 'a' + 10;
+            if (c >= 'A' && c <= 'F') return c - 'A' + 10;
+            return -1;
+        };
+        size_type first = 0;
+        for (; first < s.size(); ++first) {
+            const unsigned char c = (unsigned char)s[first];
+            if (c == '-') is_neg_ = true;
+            else if (val(c) >= 0) break;
+        }
+        uint32_t limb = 0, bits = 0;
+        for (size_type i = s.size(); i > first; --i) {
+            const int v = val((unsigned char)s[i - 1]);
+            if (v < 0) continue;
+            limb |= (uint32_t)v << bits;
+            bits += 4;
+            if (bits == DIGIT_BITS) {
+                data_.push_back(limb);
+                limb = 0;
+                bits = 0;
+            }
+        }
+        if (limb) data_.push_back(limb);
+    } else {
+        size_type first = 0;
+        for (; first < s.size(); ++first) {
+            const unsigned char c = (unsigned char)s[first];
+            if (c >= '0' && c <= '9') break;
+            if (c == '-') is_neg_ = true;
+        }
+        data_ = std::move(convert_from_dec_string(s, first, s.size()).first.data_);
+    }
+    if (data_.empty()) {
+        data_.push_back(0);
+        is_neg_ = false;
+    } else {
+        remove_leading_zero();
+    }
+}
+
+auto BigInt::convert_from_dec_string(const std::string& s, size_type start, size_type end)
+    -> std::pair<BigInt, size_type> {
+    if (end - start < std::max<size_type>(2, DEC_STRING_BRUTE_THRESHOLDS[0])) {
+        BigInt res(0);
+        size_type ndig = 0;
+        uint64_t chunk = 0, scale = 1;
+        for (size_type i = start; i < end; ++i) {
+            const unsigned char c = (unsigned char)s[i];
+            if (c < '0' || c > '9') continue;
+            ++ndig;
+            scale *= TEN;
+            chunk = chunk * TEN + (c - '0');
+            if (scale > UINT64_MAX / TEN) {
+                res *= scale;
+                res += chunk;
+                scale = 1;
+                chunk = 0;
+            }
+        }
+        if (scale > 1) {
+            res *= scale;
+            res += chunk;
+        }
+        return {std::move(res), ndig};
+    }
+    const size_type mid = (start + end) / 2;
+    auto hi = convert_from_dec_string(s, start, mid);
+    auto lo = convert_from_dec_string(s, mid, end);
+    return {hi.first * get_pow_of_ten((uint32_t)lo.second) + lo.first, hi.second + lo.second};
+}
+
+// ---- printing (src/bigint.cpp:126-265) ------------------------------------------------------------
+auto BigInt::to_dec_string_brute() const -> std::string {
+    if (is_zero()) return "0";
+    constexpr size_type chunk_digits = 19;  // 10^19 < 2^64
+    constexpr uint64_t chunk = 10000000000000000000ull;
+    BigInt t = *this;
+    std::string r;
+    r.reserve(chunk_digits + (size_type)std::ceil((double)data_.size() * DIGIT_BITS * std::log10(2.0)));
+    while (!t.is_zero()) {
+        uint64_t rem = t.unsigned_inplace_divmod(chunk);
+        for (size_type i = 0; i < chunk_digits && (rem != 0 || !t.is_zero()); ++i) {
+            r.push_back((char)('0' + rem % TEN));
+            rem /= TEN;
+        }
+    }
+    if (is_neg_) r.push_back('-');
+    std::reverse(r.begin(), r.end());
+    return r;
+}
+
+auto BigInt::get_pow_of_ten(uint32_t exponent) -> BigInt {
+    static std::vector<BigInt> squares = {BigInt(TEN)};  // 10^(2^i)
+    static std::mutex mtx;
+    std::lock_guard<std::mutex> lk(mtx);
+    BigInt res(1);
+    if (exponent == 0) return res;
+    const uint32_t nb = (uint32_t)std::bit_width(exponent);
+    while (squares.size() < nb) squares.push_back(squares.back() * squares.back());
+    for (uint32_t i = 0; i < nb; ++i)
+        if ((exponent >> i) & 1u) res *= squares[i];
+    return res;
+}
+
+void BigInt::print(std::ostream& output, bool hex, bool direct) const {
+    std::optional<std::stringstream> buf;
+    std::ostream& out = direct ? output : buf.emplace();
+    if (hex) print_hex(out);
+    else print_dec(out);
+    if (!direct) output << buf->rdbuf();
+}
+
+void BigInt::print_dec(std::ostream& output, size_type len) const {
+    auto emit = [&](const BigInt& x, size_type width) {
+        const std::string s = x.to_dec_string_brute();
+        size_type body = s.size();
+        auto it = s.begin();
+        if (x.is_neg_) {
+            output.put(*it++);
+            --body;
+        }
+        for (size_type i = body; i < width; ++i) output.put('0');
+        for (; it != s.end(); ++it) output.put(*it);
+    };
+    const size_type thr = DEC_STRING_BRUTE_THRESHOLDS[1];
+    if (data_.size() == 1 || data_.size() < thr) {
+        emit(*this, len);
+        return;
+    }
+    std::vector<std::pair<BigInt, size_type>> stack;
+    auto split = [&](const BigInt& x, size_type width) {
+        const size_type est = (size_type)std::ceil((double)x.data_.size() * DIGIT_BITS * std::log10(2.0));
+        const size_type k = est / 2;
+        auto [q, r] = x.divmod(get_pow_of_ten((uint32_t)k), RoundMode::Truncate);
+        r.remove_sign();
+        stack.emplace_back(std::move(r), k);
+        stack.emplace_back(std::move(q), width > k ? width - k : 0);
+    };
+    split(*this, len);
+    while (!stack.empty()) {
+        auto [x, width] = std::move(stack.back());
+        stack.pop_back();
+        if (x.data_.size() == 1 || x.data_.size() < thr) emit(x, width);
+        else split(x, width);
+    }
+}
+
+void BigInt::print_hex(std::ostream& output) const {
+    if (is_zero()) {
+        output.put('0');
+        return;
+    }
+    if (is_neg_) output.put('-');
+    output << std::hex << data_.back() << std::setfill('0');
+    for (size_type i = data_.size() - 1; i-- > 0;) output << std::setw(DIGIT_BITS / 4) << data_[i];
+    output << std::dec << std::setfill(' ');
+}
+
+auto operator<<(std::ostream& output, const BigInt& a) -> std::ostream& {
+    a.print(output, (output.flags() & std::ios_base::hex) != 0, false);
+    return output;
+}
+
+auto operator>>(std::istream& input, BigInt& a) -> std::istream& {
+    std::string s;
+    input >> s;
+    a = BigInt(s, (input.flags() & std::ios_base::hex) != 0);
+    return input;
+}
+
+// ---- basic magnitude arithmetic (src/bigint.cpp:267-548) ------------------------------------------
+auto BigInt::remove_leading_zero() -> size_type {
+    size_type n = data_.size();
+    while (n > 1 && data_[n - 1] == 0) --n;
+    if (n == 1 && data_[0] == 0) {
+        reset();
+        return 1;
+    }
+    data_.resize(n);
+    return n;
+}
+
+auto BigInt::compare_abs(const BigInt& a, const BigInt& b) noexcept -> std::strong_ordering {
+    if (auto c = a.data_.size() <=> b.data_.size(); c != 0) return c;
+    for (size_type i = a.data_.size(); i-- > 0;)
+        if (auto c = a.data_[i] <=> b.data_[i]; c != 0) return c;
+    return std::strong_ordering::equal;
+}
+
+void BigInt::unsigned_self_inc_or_dec(bool is_inc) {
+    size_type i = 0;
+    const size_type n = data_.size();
+    if (is_inc) {
+        while (i < n && data_[i] == DIGIT_MASK) data_[i++] = 0;
+        if (i == n) data_.push_back(1);
+        else ++data_[i];
+    } else {
+        while (i < n && data_[i] == 0) data_[i++] = DIGIT_MASK;
+        if (i < n) --data_[i];
+    }
+}
+
+auto BigInt::operator++() -> BigInt& {
+    if (is_neg_) {
+        unsigned_self_inc_or_dec(false);
+        remove_leading_zero();
+    } else {
+        unsigned_self_inc_or_dec(true);
+    }
+    return *this;
+}
+auto BigInt::operator++(int) -> BigInt {
+    BigInt t = *this;
+    ++*this;
+    return t;
+}
+auto BigInt::operator--() -> BigInt& {
+    if (is_zero()) {
+        data_[0] = 1;
+        is_neg_ = true;
+    } else if (is_neg_) {
+        unsigned_self_inc_or_dec(true);
+    } else {
+        unsigned_self_inc_or_dec(false);
+        remove_leading_zero();
+    }
+    return *this;
+}
+auto BigInt::operator--(int) -> BigInt {
+    BigInt t = *this;
+    --*this;
+    return t;
+}
+
+namespace {
+// |x| + |y| into r (limbs), canonical
+void mag_add(Digits& r, const Digits& x, const Digits& y) {
+    const size_type n = std::max(x.size(), y.size());
+    r.resize(n);
+    uint32_t carry = 0;
+    for (size_type i = 0; i < n; ++i) {
+        const uint32_t v = (i < x.size() ? x[i] : 0u) + (i < y.size() ? y[i] : 0u) + carry;
+        r[i] = v & DIGIT_MASK;
+        carry = v >> DIGIT_BITS;
+    }
+    if (carry) r.push_back(carry);
+}
+// |x| - |y| into r with |x| >= |y|
+void mag_sub(Digits& r, const Digits& x, const Digits& y) {
+    r.resize(x.size());
+    uint32_t borrow = 0;
+    for (size_type i = 0; i < x.size(); ++i) {
+        const uint32_t v = x[i] - (i < y.size() ? y[i] : 0u) - borrow;
+        r[i] = v & DIGIT_MASK;
+        borrow = v >> 31;  // wrapped
+    }
+}
+}  // namespace
+
+void BigInt::inplace_add_or_sub(const BigInt& b, bool is_sub) {
+    if (is_neg_ == (b.is_neg_ != is_sub)) {  // same effective sign: magnitudes add
+        mag_add(data_, data_, b.data_);
+        return;
+    }
+    const auto c = compare_abs(*this, b);
+    if (c == 0) {
+        reset();
+        return;
+    }
+    if (c > 0) {
+        mag_sub(data_, data_, b.data_);
+    } else {
+        Digits t;
+        mag_sub(t, b.data_, data_);
+        data_ = std::move(t);
+        is_neg_ = !is_neg_;
+    }
+    remove_leading_zero();
+}
+
+void BigInt::inplace_add_or_sub(uint64_t b, bool is_sub) {
+    if (b == 0) return;
+    BigInt t(b, is_sub);
+    inplace_add_or_sub(t, false);
+}
+
+void BigInt::unsigned_inplace_mul(uint64_t b) {
+    if (is_zero() || b == 1) return;
+    if (b == 0) {
+        reset();
+        return;
+    }
+    if ((b & (b - 1)) == 0) {
+        *this <<= std::countr_zero(b);
+        return;
+    }
+    uint128_t carry = 0;
+    for (auto& d : data_) {
+        const uint128_t v = (uint128_t)b * d + carry;
+        d = (uint32_t)v & DIGIT_MASK;
+        carry = v >> DIGIT_BITS;
+    }
+    while (carry) {
+        data_.push_back((uint32_t)carry & DIGIT_MASK);
+        carry >>= DIGIT_BITS;
+    }
+    remove_leading_zero();
+}
+
+auto BigInt::unsigned_inplace_divmod(uint64_t b) -> uint64_t {
+    if (b == 0) throw std::domain_error("division by zero");
+    uint128_t rem = 0;
+    for (size_type i = data_.size(); i-- > 0;) {
+        rem = (rem << DIGIT_BITS) + data_[i];
+        data_[i] = (uint32_t)(rem / b);
+        rem %= b;
+    }
+    const bool neg = is_neg_;
+    remove_leading_zero();
+    if (!is_zero()) is_neg_ = neg;
+    return (uint64_t)rem;
+}
+
+void BigInt::shift_left(Digits& v, uint64_t offset) {
+    const uint64_t whole = offset / DIGIT_BITS, bits = offset % DIGIT_BITS;
+    if (bits == 0) {
+        v.insert(v.begin(), whole, 0u);
+        return;
+    }
+    const size_type n = v.size();
+    v.resize(n + whole + 1);
+    // high to low: new[i] = (old[i-whole] << bits | old[i-whole-1] >> (28-bits)) & mask
+    for (size_type i = n + whole; i > whole; --i) {
+        const uint32_t lo_src = v[i - whole - 1];
+        const uint32_t hi_src = (i - whole < n) ? v[i - whole] : 0u;
+        v[i] = ((hi_src << bits) | (lo_src >> (DIGIT_BITS - bits))) & DIGIT_MASK;
+    }
+    v[whole] = (v[0] << bits) & DIGIT_MASK;
+    std::fill(v.begin(), v.begin() + (std::ptrdiff_t)whole, 0u);
+}
+
+void BigInt::shift_right(Digits& v, uint64_t offset) {
+    const uint64_t whole = offset / DIGIT_BITS, bits = offset % DIGIT_BITS;
+    if (whole >= v.size()) {
+        v.assign(1, 0u);
+        return;
+    }
+    if (bits == 0) {
+        v.erase(v.begin(), v.begin() + (std::ptrdiff_t)whole);
+        return;
+    }
+    const size_type n = v.size() - whole;
+    for (size_type i = 0; i + 1 < n; ++i)
+        v[i] = ((v[i + whole] >> bits) | (v[i + whole + 1] << (DIGIT_BITS - bits))) & DIGIT_MASK;
+    v[n - 1] = v[v.size() - 1] >> bits;
+    v.resize(n);
+}
+
+// ---- bitwise (src/bigint.cpp:551-623) -------------------------------------------------------------
+auto BigInt::operator&=(const BigInt& b) -> BigInt& {
+    if (is_zero()) return *this;
+    if (b.is_zero()) {
+        reset();
+        return *this;
+    }
+    if (data_.size() > b.data_.size()) data_.resize(b.data_.size());
+    for (size_type i = 0; i < data_.size(); ++i) data_[i] &= b.data_[i];
+    remove_leading_zero();
+    return *this;
+}
+auto BigInt::operator|=(const BigInt& b) -> BigInt& {
+    if (is_zero()) return *this = b;
+    if (b.is_zero()) return *this;
+    const size_type n = data_.size();
+    if (n < b.data_.size()) data_.insert(data_.end(), b.data_.begin() + (std::ptrdiff_t)n, b.data_.end());
+    for (size_type i = 0; i < std::min(n, b.data_.size()); ++i) data_[i] |= b.data_[i];
+    remove_leading_zero();
+    return *this;
+}
+auto BigInt::operator^=(const BigInt& b) -> BigInt& {
+    if (is_zero()) return *this = b;
+    if (b.is_zero()) return *this;
+    const size_type n = data_.size();
+    if (n < b.data_.size()) data_.insert(data_.end(), b.data_.begin() + (std::ptrdiff_t)n, b.data_.end());
+    for (size_type i = 0; i < std::min(n, b.data_.size()); ++i) data_[i] ^= b.data_[i];
+    remove_leading_zero();
+    return *this;
+}
+auto BigInt::bitwise_not(size_type len) -> BigInt& {
+    if (len) data_.resize(len);
+    for (auto& d : data_) d = ~d & DIGIT_MASK;
+    remove_leading_zero();
+    return *this;
+}
+
+// ---- division via the reciprocal (src/bigint.cpp:625-677) -----------------------------------------
+auto BigInt::divmod(const BigInt& b, RoundMode mode, bool check) const -> std::pair<BigInt, BigInt> {
+    if (b.is_zero()) throw std::domain_error("division by zero");
+    if (is_zero()) return {BigInt(0), BigInt(0)};
+    constexpr size_type guard = 16;
+    const BigInt& a = *this;
+    const size_type prec = a.data_.size() + guard;
+    BigInt q(BigFloat::mul(BigFloat(a), BigFloat(b).reciprocal(prec), prec), mode);
+    BigInt r(a - q * b);
+    if (!check || r.is_zero()) return {std::move(q), std::move(r)};
+    bool fix = false;
+    switch (mode) {
+        case RoundMode::Floor: fix = (r.is_neg_ != b.is_neg_) || compare_abs(r, b) >= 0; break;
+        case RoundMode::Ceil: fix = (r.is_neg_ == b.is_neg_) || compare_abs(r, b) >= 0; break;
+        case RoundMode::RoundHalfUp: {
+            const auto c = compare_abs(r * 2, b);
+            fix = c > 0 || (c == 0 && r.is_neg_ == a.is_neg_);
+            break;
+        }
+        case RoundMode::Truncate:
+        default: fix = (r.is_neg_ != a.is_neg_) || compare_abs(r, b) >= 0; break;
+    }
+    if (fix) {
+        if (r.is_neg_ != b.is_neg_) {
+            r += b;
+            --q;
+        } else {
+            r -= b;
+            ++q;
+        }
+    }
+    return {std::move(q), std::move(r)};
+}
+
+// ---- BigFloat ------------------------------------------------------------------------------------
+BigFloat::BigFloat(const BigInt& x, int64_t offset) : data_(x.data_), point_pos_(0), is_neg_(x.is_neg_) {
+    *this <<= offset;
+    remove_tail_zero();
+}
+
+BigFloat::BigFloat(BigInt&& x, int64_t offset)
+    : data_(std::move(x.data_)), point_pos_(0), is_neg_(x.is_neg_) {
+    x.reset();
+    *this <<= offset;
+    remove_tail_zero();
+}
+
+// src/bigint.cpp:681-715
+BigFloat::BigFloat(double value) : point_pos_(0), is_neg_(false) {
+    if (std::isinf(value)) throw std::domain_error("invalid double to BigFloat conversion: inf");
+    if (std::isnan(value)) throw std::domain_error("invalid double to BigFloat conversion: nan");
+    if (value == 0) {
+        data_.push_back(0);
+        return;
+    }
+    const uint64_t raw = std::bit_cast<uint64_t>(value);
+    is_neg_ = (raw >> 63) != 0;
+    const int64_t ebits = (int64_t)((raw >> DOUBLE_MANTISSA_LEN) & ((1u << DOUBLE_EXPONENT_LEN) - 1));
+    uint64_t mant = raw & ((1ull << DOUBLE_MANTISSA_LEN) - 1);
+    int64_t offset = -DOUBLE_MANTISSA_LEN - DOUBLE_EXPONENT_BIAS;
+    if (ebits == 0) {
+        offset += 1;
+    } else {
+        mant |= 1ull << DOUBLE_MANTISSA_LEN;
+        offset += ebits;
+    }
+    while (mant) {
+        data_.push_back((uint32_t)mant & DIGIT_MASK);
+        mant >>= DIGIT_BITS;
+    }
+    shift(offset);
+}
+
+// src/bigint.cpp:717-766
+auto BigFloat::to_double() const -> double {
+    if (is_zero()) return 0;
+    const int64_t hw = std::bit_width(data_.back());
+    const int64_t exponent = (int64_t)DIGIT_BITS * ((int64_t)data_.size() - 1 - point_pos_) + hw - 1;
+    int64_t ebits = exponent + DOUBLE_EXPONENT_BIAS;
+    if (ebits >= (1 << DOUBLE_EXPONENT_LEN) - 1) return is_neg_ ? -HUGE_VAL : HUGE_VAL;
+    auto shl = [](uint64_t x, int64_t r) -> uint64_t {
+        if (r <= -64 || r >= 64) return 0;
+        return r < 0 ? x >> (-r) : x << r;
+    };
+    int64_t off;
+    uint64_t mbits;
+    if (ebits <= 0) {
+        constexpr int64_t min_sub = -DOUBLE_MANTISSA_LEN - DOUBLE_EXPONENT_BIAS + 1;
+        if (exponent < min_sub) return is_neg_ ? -0.0 : 0.0;
+        ebits = 0;
+        off = exponent + DOUBLE_MANTISSA_LEN + DOUBLE_EXPONENT_BIAS - hw;
+        mbits = shl(data_.back(), off);
+    } else {
+        off = DOUBLE_MANTISSA_LEN - (hw - 1);
+        mbits = shl(data_.back() & ((1u << (hw - 1)) - 1), off);
+    }
+    size_type i = data_.size() - 1;
+    while (off > 0 && i > 0) {
+        off -= DIGIT_BITS;
+        --i;
+        mbits |= shl(data_[i], off);
+    }
+    return std::bit_cast<double>(((uint64_t)is_neg_ << 63) | ((uint64_t)ebits << DOUBLE_MANTISSA_LEN) | mbits);
+}
+
+// src/bigint.cpp:769-812
+void BigFloat::shift(int64_t offset) {
+    point_pos_ -= offset / (int64_t)DIGIT_BITS;
+    offset %= (int64_t)DIGIT_BITS;
+    if (offset == 0) {
+        remove_leading_zero();
+        return;
+    }
+    if (offset > 0) {
+        offset = DIGIT_BITS - offset;
+        --point_pos_;
+    } else {
+        offset = -offset;
+    }
+    if (std::countr_zero(data_[0]) >= offset) {
+        shift_right(data_, (uint32_t)offset);
+    } else {
+        shift_left(data_, DIGIT_BITS - (uint32_t)offset);
+        ++point_pos_;
+    }
+    remove_leading_zero();
+}
+
+void BigFloat::shift_right(Digits& v, uint32_t offset) {
+    const size_type n = v.size();
+    for (size_type i = 0; i + 1 < n; ++i) v[i] = ((v[i] >> offset) | (v[i + 1] << (DIGIT_BITS - offset))) & DIGIT_MASK;
+    v[n - 1] >>= offset;
+}
+
+void BigFloat::shift_left(Digits& v, uint32_t offset) {
+    v.push_back(0);
+    for (size_type i = v.size() - 1; i > 0; --i)
+        v[i] = ((v[i] << offset) | (v[i - 1] >> (DIGIT_BITS - offset))) & DIGIT_MASK;
+    v[0] = (v[0] << offset) & DIGIT_MASK;
+}
+
+auto BigFloat::remove_leading_zero() -> size_type {
+    size_type n = data_.size();
+    while (n > 1 && data_[n - 1] == 0) --n;
+    if (n == 1 && data_[0] == 0) {
+        reset();
+        return 1;
+    }
+    data_.resize(n);
+    return n;
+}
+
+auto BigFloat::remove_tail_zero() -> size_type {
+    if (is_zero()) return 1;
+    size_type z = 0;
+    while (data_[z] == 0) ++z;
+    if (z) {
+        point_pos_ -= (int64_t)z;
+        data_.erase(data_.begin(), data_.begin() + (std::ptrdiff_t)z);
+    }
+    return data_.size();
+}
+
+// src/bigint.cpp:849-936
+auto BigFloat::add_or_sub(const BigFloat& a, const BigFloat& b, bool is_sub) -> BigFloat {
+    const int64_t la = (int64_t)a.data_.size(), lb = (int64_t)b.data_.size();
+    int64_t tza = 0, tzb = 0;
+    while (tza < la && a.data_[tza] == 0) ++tza;
+    while (tzb < lb && b.data_[tzb] == 0) ++tzb;
+    if (tza == la) return is_sub ? -b : b;
+    if (tzb == lb) return a;
+    const int64_t pp = std::max(a.point_pos_ - tza, b.point_pos_ - tzb);
+    const int64_t lr = std::max(la - a.point_pos_, lb - b.point_pos_) + pp;
+    const int64_t oa = pp - a.point_pos_, ob = pp - b.point_pos_;
+    // signed limb accumulation, then normalisation
+    std::vector<int64_t> acc((size_t)lr, 0);
+    for (int64_t i = tza; i < la; ++i) acc[(size_t)(i + oa)] = a.data_[(size_t)i];
+    const bool subtract = a.is_neg_ != (b.is_neg_ != is_sub);
+    for (int64_t i = tzb; i < lb; ++i) acc[(size_t)(i + ob)] += subtract ? -(int64_t)b.data_[(size_t)i] : (int64_t)b.data_[(size_t)i];
+    BigFloat res;
+    res.point_pos_ = pp;
+    res.is_neg_ = a.is_neg_;
+    // sign of the signed-limb value = sign of its top nonzero limb
+    int64_t top = lr - 1;
+    while (top >= 0 && acc[(size_t)top] == 0) --top;
+    if (top < 0) {
+        res.data_.assign(1, 0u);
+        res.point_pos_ = 0;
+        res.is_neg_ = false;
+        return res;
+    }
+    if (acc[(size_t)top] < 0) {
+        res.is_neg_ = !res.is_neg_;
+        for (auto& x : acc) x = -x;
+    }
+    res.data_.resize((size_t)lr + 1);
+    int64_t carry = 0;
+    for (int64_t i = 0; i < lr; ++i) {
+        int64_t v = acc[(size_t)i] + carry;
+        carry = v >> DIGIT_BITS;  // arithmetic shift: floor division
+        res.data_[(size_t)i] = (uint32_t)(v & DIGIT_MASK);
+    }
+    res.data_[(size_t)lr] = (uint32_t)carry;
+    res.remove_leading_zero();
+    res.remove_tail_zero();
+    return res;
+}
+
+void BigFloat::unsigned_inplace_mul(uint64_t b) {
+    if (is_zero() || b == 1) return;
+    if (b == 0) {
+        reset();
+        return;
+    }
+    if ((b & (b - 1)) == 0) {
+        *this <<= std::countr_zero(b);
+        return;
+    }
+    uint128_t carry = 0;
+    for (auto& d : data_) {
+        const uint128_t v = (uint128_t)b * d + carry;
+        d = (uint32_t)v & DIGIT_MASK;
+        carry = v >> DIGIT_BITS;
+    }
+    while (carry) {
+        data_.push_back((uint32_t)carry & DIGIT_MASK);
+        carry >>= DIGIT_BITS;
+    }
+    remove_leading_zero();
+}
+
+// src/bigint.cpp:972-1042
+void BigFloat::print(std::ostream& output, size_type dec_digits, bool direct) const {
+    std::optional<std::stringstream> buf;
+    std::ostream& out = direct ? output : buf.emplace();
+    BigInt ip, fp;
+    if (point_pos_ < (int64_t)data_.size()) {
+        if (point_pos_ < 0) ip.data_.assign((size_type)(-point_pos_), 0u);
+        ip.data_.insert(ip.data_.end(), data_.begin() + std::max<int64_t>(0, point_pos_), data_.end());
+    }
+    if (ip.data_.empty()) ip.data_.push_back(0);
+    if (point_pos_ > 0)
+        fp.data_.assign(data_.begin(), data_.begin() + std::min<int64_t>(point_pos_, (int64_t)data_.size()));
+    if (dec_digits == 0)
+        dec_digits = (size_type)std::floor((double)(fp.data_.size() * DIGIT_BITS) * std::log10(2.0));
+    if (dec_digits) {
+        if (fp.data_.empty()) fp.data_.push_back(0);
+        fp.remove_leading_zero();
+        const BigInt five_pow = fast_pow(BigInt(TEN / 2), (uint32_t)dec_digits);
+        fp *= five_pow;
+        const int64_t off = (int64_t)DIGIT_BITS * point_pos_ - (int64_t)dec_digits;
+        BigFloat scaled(std::move(fp), -off);
+        fp = BigInt(scaled, RoundMode::RoundHalfUp);
+        // did rounding carry into the integer part (fp == 10^dec_digits)?
+        size_type tz = 0;
+        for (auto d : fp.data_) {
+            if (d != 0) {
+                tz += (size_type)std::countr_zero(d);
+                break;
+            }
+            tz += DIGIT_BITS;
+        }
+        if (tz == dec_digits && (fp >> dec_digits) == five_pow) {
+            fp.reset();
+            ++ip;
+        }
+    }
+    if (is_neg_) out.put('-');
+    ip.print_dec(out);
+    out.put('.');
+    if (dec_digits) fp.print_dec(out, dec_digits);
+    if (!direct) output << buf->rdbuf();
+}
+
+auto operator<<(std::ostream& output, const BigFloat& a) -> std::ostream& {
+    a.print(output, 0, false);
+    return output;
+}
+
+auto operator>>(std::istream& input, BigFloat& a) -> std::istream& {
+    std::string s;
+    input >> s;
+    a = BigFloat(s, (input.flags() & std::ios_base::hex) != 0);
+    return input;
+}
+
+// src/bigint.cpp:1044-1071
+auto BigFloat::reciprocal(size_type precision) const -> BigFloat {
+    if (is_zero()) throw std::domain_error("division by zero");
+    if (precision == 0) precision = data_.size();
+    const int64_t exponent =
+        (int64_t)DIGIT_BITS * ((int64_t)data_.size() - 1 - point_pos_) + std::bit_width(data_.back());
+    BigFloat b = *this >> exponent;
+    b.remove_tail_zero();
+    BigFloat x(1 / b.to_double());
+    size_type cur = DOUBLE_MANTISSA_LEN / DIGIT_BITS;
+    const BigFloat one(1);
+    while (cur < precision) {
+        cur *= 2;
+        x += mul(x, one - mul(b, x, cur + 1), cur + 1);
+    }
+    x >>= exponent;
+    return x;
+}
+
+// src/bigint.cpp:1074-1113: drop limbs [0, idx] rounding by mode
+static void round_limbs(RoundMode mode, Digits& v, bool is_neg, int64_t idx) {
+    bool up = false;
+    switch (mode) {
+        case RoundMode::Floor:
+        case RoundMode::Ceil: {
+            const bool away = (mode == RoundMode::Floor) ? is_neg : !is_neg;
+            if (away) {
+                int64_t i = idx;
+                while (i >= 0 && v[(size_t)i] == 0) --i;
+                up = i >= 0;
+            }
+            break;
+        }
+        case RoundMode::RoundHalfUp: up = (v[(size_t)idx] >> (DIGIT_BITS - 1)) != 0; break;
+        case RoundMode::Truncate:
+        default: up = false; break;
+    }
+    v.erase(v.begin(), v.begin() + idx + 1);
+    if (!up) return;
+    size_type i = 0;
+    while (i < v.size() && v[i] == DIGIT_MASK) v[i++] = 0;
+    if (i == v.size()) v.push_back(1);
+    else ++v[i];
+}
+
+// src/bigint.cpp:1115-1140
+void BigFloat::round(RoundMode mode, int64_t precision, RoundRelativeTo relative) {
+    if (is_zero()) return;
+    const int64_t n = (int64_t)data_.size();
+    if (relative == RoundRelativeTo::Point) precision = n - point_pos_ - precision;
+    const int64_t idx = n - precision - 1;
+    if (idx >= n) {
+        reset();
+        return;
+    }
+    if (idx < 0) return;
+    point_pos_ -= idx + 1;
+    round_limbs(mode, data_, is_neg_, idx);
+    remove_leading_zero();
+}
+
+// src/bigint.cpp:1142-1172
+BigInt::BigInt(const BigFloat& x, RoundMode mode)
+    : BigInt(x.is_zero() || x.point_pos_ > (int64_t)x.data_.size()
+                 ? BigInt(0u)
+                 : convert_from_BigFloat(mode, x.data_, x.is_neg_, x.point_pos_)) {}
+
+BigInt::BigInt(BigFloat&& x, RoundMode mode)
+    : BigInt(x.is_zero() || x.point_pos_ > (int64_t)x.data_.size()
+                 ? BigInt(0u)
+                 : convert_from_BigFloat(mode, std::move(x.data_), x.is_neg_, x.point_pos_)) {
+    x.reset();
+}
+
+auto BigInt::convert_from_BigFloat(RoundMode mode, Digits data, bool is_neg, int64_t point_pos) -> BigInt {
+    if (point_pos < 0) {
+        data.insert(data.begin(), (size_type)(-point_pos), 0u);
+    } else if (point_pos > 0) {
+        if (point_pos > (int64_t)data.size()) return BigInt(0u);
+        round_limbs(mode, data, is_neg, point_pos - 1);
+    }
+    BigInt r;
+    r.data_ = std::move(data);
+    if (r.data_.empty()) r.data_.push_back(0);
+    r.is_neg_ = is_neg;
+    r.remove_leading_zero();
+    return r;
+}
+
+}  // namespace bigint

@@ -1,0 +1,114 @@
+// mul_digits: the host side of the multiply seam (reference include/bigint/mul.h:223-318).
+#include "bigint/mul.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <new>
+
+#include "bigint_hip.h"
+
+namespace bigint::mul {
+namespace {
+
+std::atomic<uint64_t> g_gpu_products{0};
+std::atomic<uint64_t> g_host_products{0};
+
+// 64-bit words of the value held in limbs (what the reference's digits_to_vec64 produces after
+// dropping leading zero words, mul.h:245-268).
+auto words64(std::span<const uint32_t> x) -> size_type {
+    size_type n = x.size();
+    while (n > 0 && x[n - 1] == 0) --n;
+    if (n == 0) return 0;
+    const size_type bits = (n - 1) * DIGIT_BITS + (size_type)std::bit_width(x[n - 1]);
+    return (bits + 63) / 64;
+}
+
+// Schoolbook on 28-bit limbs with 64-bit accumulation: each partial product is < 2^56, so up to
+// 255 of them can be summed before the accumulator needs folding; the host route only takes
+// products of <= 256 words (<= 586 limbs), and the carry is folded every row anyway.
+auto schoolbook(std::span<const uint32_t> a, std::span<const uint32_t> b) -> Digits {
+    Digits r(a.size() + b.size(), 0);
+    std::vector<uint64_t> acc(a.size() + b.size() + 1, 0);
+    for (size_type i = 0; i < a.size(); ++i) {
+        const uint64_t ai = a[i];
+        if (ai == 0) continue;
+        for (size_type j = 0; j < b.size(); ++j) acc[i + j] += ai * b[j];
+        // fold so every accumulator stays below 2^60
+        uint64_t carry = 0;
+        for (size_type k = i; k < i + b.size() + 1; ++k) {
+            const uint64_t v = acc[k] + carry;
+            acc[k] = v & DIGIT_MASK;
+            carry = v >> DIGIT_BITS;
+        }
+        size_type k = i + b.size() + 1;
+        while (carry) {
+            const uint64_t v = acc[k] + carry;
+            acc[k] = v & DIGIT_MASK;
+            carry = v >> DIGIT_BITS;
+            ++k;
+        }
+    }
+    for (size_type k = 0; k < r.size(); ++k) r[k] = (uint32_t)acc[k];
+    return r;
+}
+
+}  // namespace
+
+auto gpu_product_count() noexcept -> uint64_t { return g_gpu_products.load(); }
+auto host_product_count() noexcept -> uint64_t { return g_host_products.load(); }
+
+auto mul_limbs(std::span<const uint32_t> a, std::span<const uint32_t> b, bool square) -> Digits {
+    if (square) b = a;
+    if ((words64(a) + words64(b)) * 64 <= HOST_MAX_TOTAL_BITS) {
+        g_host_products.fetch_add(1, std::memory_order_relaxed);
+        Digits r = schoolbook(a, b);
+        while (r.size() > 1 && r.back() == 0) r.pop_back();
+        return r;
+    }
+    // strip leading zero limbs: the engine requires a nonzero top limb
+    size_type la = a.size(), lb = b.size();
+    while (la > 1 && a[la - 1] == 0) --la;
+    while (lb > 1 && b[lb - 1] == 0) --lb;
+    Digits r(la + lb);
+    size_t len = 0;
+    const int rc = bi_hip_mul(a.data(), la, b.data(), lb, square ? 1 : 0, r.data(), r.size(), &len);
+    if (rc == BI_HIP_ENOMEM) throw std::bad_alloc();
+    if (rc != BI_HIP_OK) throw engine_error(rc, bi_hip_strerror(rc));
+    g_gpu_products.fetch_add(1, std::memory_order_relaxed);
+    r.resize(len);
+    return r;
+}
+
+auto mul_digits(const Digits& a, const Digits& b, size_type output_precision,
+                int64_t* p_result_point_pos) -> Digits {
+    const bool same = &a == &b;
+    size_type off_a =
+        output_precision == 0 || a.size() <= output_precision ? 0 : a.size() - output_precision;
+    size_type off_b =
+        output_precision == 0 || b.size() <= output_precision ? 0 : b.size() - output_precision;
+    size_type tz_a = 0, tz_b = 0;
+    while (a[off_a + tz_a] == 0) ++tz_a;
+    if (same) tz_b = tz_a;
+    else
+        while (b[off_b + tz_b] == 0) ++tz_b;
+    off_a += tz_a;
+    off_b += tz_b;
+
+    const std::span<const uint32_t> sa(a.data() + off_a, a.size() - off_a);
+    const std::span<const uint32_t> sb(b.data() + off_b, b.size() - off_b);
+    Digits res = mul_limbs(sa, sb, same);
+    while (res.size() > 1 && res.back() == 0) res.pop_back();
+
+    if (output_precision) {
+        size_type off_r = res.size() <= output_precision ? 0 : res.size() - output_precision;
+        while (res[off_r] == 0) ++off_r;
+        if (off_r) res.erase(res.begin(), res.begin() + (std::ptrdiff_t)off_r);
+        if (p_result_point_pos) *p_result_point_pos -= (int64_t)(off_a + off_b + off_r);
+    } else {
+        res.insert(res.begin(), tz_a + tz_b, 0u);
+    }
+    return res;
+}
+
+}  // namespace bigint::mul

@@ -1,0 +1,44 @@
+// The multiply seam of the host library (reference include/bigint/mul.h:208-318).
+//
+// mul_digits keeps the reference's contract: both inputs nonzero, squaring detected by object
+// identity (&a == &b), optional precision truncation of operands and result with the point
+// position adjusted.  Below it, products of at most 16,384 total bits (counted in 64-bit words,
+// reference mul.h:80, 210-212) use a host schoolbook; every larger product goes to the MI355X
+// engine through the C ABI of include/bigint_hip.h.  There is no CPU fallback for those: if the
+// engine is unavailable, mul_digits throws.
+#pragma once
+
+#include <cstdint>
+#include <span>
+#include <stdexcept>
+
+#include "bigint/bigint_base.h"
+
+namespace bigint::mul {
+
+// Largest total size (bits, counted as the reference counts them) multiplied on the host.
+constexpr size_type HOST_MAX_TOTAL_BITS = 2 * 128 * 64;
+
+// Raised when the GPU engine reports an error other than out-of-memory.
+class engine_error : public std::runtime_error {
+public:
+    engine_error(int code, const char* what) : std::runtime_error(what), code_(code) {}
+    [[nodiscard]] auto code() const noexcept -> int { return code_; }
+
+private:
+    int code_;
+};
+
+// out = a * b (28-bit limbs, canonical or with leading zeros); the core product below the
+// precision/zero-stripping bookkeeping.  `square` means b is a.
+auto mul_limbs(std::span<const uint32_t> a, std::span<const uint32_t> b, bool square) -> Digits;
+
+// Reference semantics of mul::mul_digits (mul.h:223-318).  Inputs must be nonzero.
+auto mul_digits(const Digits& a, const Digits& b, size_type output_precision = 0,
+                int64_t* p_result_point_pos = nullptr) -> Digits;
+
+// Number of products routed to the GPU engine / multiplied on the host since start-up.
+auto gpu_product_count() noexcept -> uint64_t;
+auto host_product_count() noexcept -> uint64_t;
+
+}  // namespace bigint::mul

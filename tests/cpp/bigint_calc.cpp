@@ -1,0 +1,161 @@
+// Line-oriented driver of the host BigInt/BigFloat API for the test suite.
+// Reads commands from stdin, one per line, and prints one result line each.  Integers are
+// signed hex ("-1f", "0"); a BigFloat is printed as "<sign> <hex magnitude> <point_pos>".
+//
+//   mul A B | sq A | add A B | sub A B | muli A k | divi A k | shl A k | shr A k
+//   and A B | or A B | xor A B | not A len | cmp A B | inc A | dec A
+//   divmod A B mode          -> "Q R"  (mode 0 Truncate, 1 Floor, 2 Ceil, 3 RoundHalfUp)
+//   todec A | fromdec S | fromhex S
+//   fmul A pa B pb prec      -> BigFloat(A) >> 28*pa times BigFloat(B) >> 28*pb at prec limbs
+//   fadd A pa B pb | fsub A pa B pb
+//   frecip A pa prec | fprint A pa digits | ftodouble A pa | ffromdouble X (hex bits)
+//   fround A pa mode prec rel
+//   counts                   -> "<gpu products> <host products>"
+#include <bit>
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+#include <sstream>
+#include <string>
+
+#include "bigint/bigint.h"
+
+using namespace bigint;
+
+static BigInt H(const std::string& s) { return BigInt(s, true); }
+static std::string X(const BigInt& x) { return x.to_string(true); }
+static BigFloat F(const std::string& a, int64_t pa) { return BigFloat(H(a), -28 * pa); }
+static std::string XF(const BigFloat& f) {
+    // magnitude limbs -> BigInt through the public API: BigInt(f << 28*point_pos)
+    BigFloat m = f;
+    m.remove_sign();
+    m <<= 28 * m.get_point_pos();
+    BigInt mag(m, RoundMode::Truncate);
+    std::ostringstream o;
+    o << (f.sign() < 0 ? "-" : "+") << " " << X(mag) << " " << f.get_point_pos();
+    return o.str();
+}
+
+int main() {
+    std::ios::sync_with_stdio(false);
+    std::string line;
+    while (std::getline(std::cin, line)) {
+        std::istringstream in(line);
+        std::string op;
+        in >> op;
+        std::string a, b;
+        std::ostringstream out;
+        try {
+            if (op == "mul") {
+                in >> a >> b;
+                out << X(H(a) * H(b));
+            } else if (op == "sq") {
+                in >> a;
+                BigInt x = H(a);
+                x *= x;
+                out << X(x);
+            } else if (op == "add") {
+                in >> a >> b;
+                out << X(H(a) + H(b));
+            } else if (op == "sub") {
+                in >> a >> b;
+                out << X(H(a) - H(b));
+            } else if (op == "muli" || op == "divi" || op == "shl" || op == "shr") {
+                long long k;
+                in >> a >> k;
+                BigInt x = H(a);
+                if (op == "muli") x *= (int64_t)k;
+                else if (op == "divi") x /= (int64_t)k;
+                else if (op == "shl") x <<= (int64_t)k;
+                else x >>= (int64_t)k;
+                out << X(x);
+            } else if (op == "and") {
+                in >> a >> b;
+                out << X(H(a) & H(b));
+            } else if (op == "or") {
+                in >> a >> b;
+                out << X(H(a) | H(b));
+            } else if (op == "xor") {
+                in >> a >> b;
+                out << X(H(a) ^ H(b));
+            } else if (op == "not") {
+                size_t len;
+                in >> a >> len;
+                BigInt x = H(a);
+                x.bitwise_not(len);
+                out << X(x);
+            } else if (op == "cmp") {
+                in >> a >> b;
+                auto c = H(a) <=> H(b);
+                out << (c < 0 ? -1 : c > 0 ? 1 : 0);
+            } else if (op == "inc") {
+                in >> a;
+                BigInt x = H(a);
+                ++x;
+                out << X(x);
+            } else if (op == "dec") {
+                in >> a;
+                BigInt x = H(a);
+                --x;
+                out << X(x);
+            } else if (op == "divmod") {
+                int mode;
+                in >> a >> b >> mode;
+                auto [q, r] = H(a).divmod(H(b), (RoundMode)mode);
+                out << X(q) << " " << X(r);
+            } else if (op == "todec") {
+                in >> a;
+                out << H(a).to_string(false);
+            } else if (op == "fromdec") {
+                in >> a;
+                out << X(BigInt(a, false));
+            } else if (op == "fromhex") {
+                in >> a;
+                out << X(BigInt(a, true));
+            } else if (op == "fmul" || op == "fadd" || op == "fsub") {
+                long long pa, pb;
+                size_t prec = 0;
+                in >> a >> pa >> b >> pb;
+                if (op == "fmul") in >> prec;
+                BigFloat fa = F(a, pa), fb = F(b, pb);
+                if (op == "fmul") out << XF(BigFloat::mul(fa, fb, prec));
+                else if (op == "fadd") out << XF(fa + fb);
+                else out << XF(fa - fb);
+            } else if (op == "frecip") {
+                long long pa;
+                size_t prec;
+                in >> a >> pa >> prec;
+                out << XF(F(a, pa).reciprocal(prec));
+            } else if (op == "fprint") {
+                long long pa;
+                size_t digits;
+                in >> a >> pa >> digits;
+                out << F(a, pa).to_string(digits);
+            } else if (op == "ftodouble") {
+                long long pa;
+                in >> a >> pa;
+                double d = F(a, pa).to_double();
+                out << std::hex << std::bit_cast<uint64_t>(d);
+            } else if (op == "ffromdouble") {
+                uint64_t bits;
+                in >> std::hex >> bits;
+                out << XF(BigFloat(std::bit_cast<double>(bits)));
+            } else if (op == "fround") {
+                long long pa, prec;
+                int mode, rel;
+                in >> a >> pa >> mode >> prec >> rel;
+                BigFloat f = F(a, pa);
+                f.round((RoundMode)mode, prec, (RoundRelativeTo)rel);
+                out << XF(f);
+            } else if (op == "counts") {
+                out << mul::gpu_product_count() << " " << mul::host_product_count();
+            } else {
+                out << "ERR unknown op";
+            }
+        } catch (const std::exception& e) {
+            out << "EXC " << e.what();
+        }
+        std::cout << out.str() << "\n" << std::flush;
+    }
+    return 0;
+}

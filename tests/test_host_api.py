@@ -1,0 +1,226 @@
+"""The host C++ API (include/bigint/bigint.h) driven through tests/cpp/bigint_calc.cpp.
+
+CPU tests keep every product at or below 16,384 total bits, which the library multiplies on the
+host exactly as the reference's dispatcher does (include/bigint/mul.h:80, 210-212); the GPU tests
+push products through the engine.  Expected values come from CPython ints and from
+tests/bigfloat_model.py, an independent Python model of the reference's BigFloat bookkeeping.
+"""
+import os
+import random
+import subprocess
+
+import pytest
+
+import bigfloat_model as BF
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CALC = os.path.join(ROOT, "bigint_amd", "_build", "bigint_calc")
+
+
+def hx(x: int) -> str:
+    return ("-" if x < 0 else "") + format(abs(x), "x")
+
+
+def unhx(s: str) -> int:
+    return -int(s[1:], 16) if s.startswith("-") else int(s, 16)
+
+
+class Calc:
+    def __init__(self):
+        self.p = subprocess.Popen([CALC], stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True,
+                                  bufsize=1)
+
+    def __call__(self, *args) -> str:
+        self.p.stdin.write(" ".join(str(a) for a in args) + "\n")
+        self.p.stdin.flush()
+        return self.p.stdout.readline().strip()
+
+    def close(self):
+        self.p.stdin.close()
+        self.p.wait(timeout=30)
+
+
+@pytest.fixture(scope="module")
+def calc():
+    if not os.path.exists(CALC):
+        pytest.skip("bigint_calc not built")
+    c = Calc()
+    yield c
+    c.close()
+
+
+def rnd(rng, maxbits):
+    x = rng.getrandbits(rng.randrange(1, maxbits))
+    return -x if rng.random() < 0.3 else x
+
+
+def xf(f):
+    neg, D, pp = f
+    return f"{'-' if neg and D else '+'} {format(D, 'x')} {pp if D else 0}"
+
+
+def trunc_div(a, b):
+    q = abs(a) // abs(b)
+    return q if (a >= 0) == (b >= 0) else -q
+
+
+def divmod_mode(a, b, mode):
+    if mode == 0:
+        q = trunc_div(a, b)
+    elif mode == 1:
+        q = a // b
+    elif mode == 2:
+        q = -((-a) // b)
+    else:  # half away from zero
+        q0 = trunc_div(a, b)
+        r = a - q0 * b
+        q = q0 + ((1 if (a >= 0) == (b >= 0) else -1) if 2 * abs(r) >= abs(b) else 0)
+    return q, a - q * b
+
+
+def test_integer_ops_small(calc):
+    rng = random.Random(1)
+    for _ in range(300):
+        a, b = rnd(rng, 4000), rnd(rng, 4000)
+        assert unhx(calc("mul", hx(a), hx(b))) == a * b
+        assert unhx(calc("add", hx(a), hx(b))) == a + b
+        assert unhx(calc("sub", hx(a), hx(b))) == a - b
+        assert unhx(calc("sq", hx(a))) == a * a
+        k = rng.randrange(-(1 << 63), 1 << 63) or 3
+        assert unhx(calc("muli", hx(a), k)) == a * k
+        assert unhx(calc("divi", hx(a), k)) == trunc_div(a, k)
+        s = rng.randrange(0, 300)
+        sign = -1 if a < 0 else 1
+        assert unhx(calc("shl", hx(a), s)) == sign * (abs(a) << s)
+        assert unhx(calc("shr", hx(a), s)) == sign * (abs(a) >> s)
+        assert int(calc("cmp", hx(a), hx(b))) == (a > b) - (a < b)
+        assert unhx(calc("inc", hx(a))) == a + 1
+        assert unhx(calc("dec", hx(a))) == a - 1
+    assert unhx(calc("dec", "0")) == -1
+    assert unhx(calc("inc", "-1")) == 0
+
+
+def test_bitwise_small(calc):
+    rng = random.Random(2)
+    for _ in range(100):
+        a, b = rnd(rng, 500), rnd(rng, 500)
+        sa = -1 if a < 0 else 1
+        r = unhx(calc("and", hx(a), hx(b)))
+        assert r == (sa * (abs(a) & abs(b)) if a else 0)
+        r = unhx(calc("or", hx(a), hx(b)))
+        assert r == (b if a == 0 else sa * (abs(a) | abs(b)))
+        r = unhx(calc("xor", hx(a), hx(b)))
+        assert r == (b if a == 0 else a if b == 0 else sa * (abs(a) ^ abs(b)))
+
+
+def test_divmod_modes_small(calc):
+    rng = random.Random(3)
+    for _ in range(150):
+        a, b = rnd(rng, 3000), rnd(rng, 1500)
+        if b == 0:
+            continue
+        for mode in range(4):
+            q, r = calc("divmod", hx(a), hx(b), mode).split()
+            assert (unhx(q), unhx(r)) == divmod_mode(a, b, mode), (a, b, mode)
+
+
+def test_decimal_round_trip_small(calc):
+    rng = random.Random(4)
+    for _ in range(50):
+        a = rnd(rng, 3000)
+        assert calc("todec", hx(a)) == str(a)
+        assert unhx(calc("fromdec", str(a))) == a
+
+
+def test_bigfloat_mul_add_small(calc):
+    rng = random.Random(5)
+    for _ in range(200):
+        a, b = rnd(rng, 2000), rnd(rng, 2000)
+        pa, pb = rng.randrange(-5, 40), rng.randrange(-5, 40)
+        fa, fb = BF.from_int(a, -28 * pa), BF.from_int(b, -28 * pb)
+        prec = rng.choice([0, 1, 2, 5, 20, 70])
+        assert calc("fmul", hx(a), pa, hx(b), pb, prec) == xf(BF.mul(fa, fb, prec))
+        assert calc("fadd", hx(a), pa, hx(b), pb) == xf(BF.add_or_sub(fa, fb, False))
+        assert calc("fsub", hx(a), pa, hx(b), pb) == xf(BF.add_or_sub(fa, fb, True))
+
+
+def test_bigfloat_double_round_trip(calc):
+    rng = random.Random(6)
+    import struct
+    for _ in range(200):
+        x = rng.uniform(-1e30, 1e30) * 10 ** rng.randrange(-200, 200)
+        bits = struct.unpack("<Q", struct.pack("<d", x))[0]
+        f = BF.from_double(x)
+        assert calc("ffromdouble", format(bits, "x")) == xf(f)
+        neg, D, pp = f
+        got = int(calc("ftodouble", hx(-D if neg else D), pp), 16)
+        assert got == bits
+
+
+def test_reciprocal_small(calc):
+    """Limb-exact against the Python model, and within the Newton error bound."""
+    rng = random.Random(7)
+    for _ in range(40):
+        a = rng.getrandbits(rng.randrange(30, 1500)) | 1
+        pa = rng.randrange(-3, 30)
+        prec = rng.choice([1, 2, 3, 8, 17, 40])
+        f = BF.from_int(a, -28 * pa)
+        exp = BF.reciprocal(f, prec)
+        assert calc("frecip", hx(a), pa, prec) == xf(exp)
+        # |x * b - 1| small relative to the working precision
+        neg, D, pp = exp
+        err = abs(D * a - (1 << (28 * (pp + pa))))
+        assert err * (1 << (28 * max(1, prec - 2))) < (1 << (28 * (pp + pa)))
+
+
+def test_print_fraction_small(calc):
+    from fractions import Fraction
+    rng = random.Random(8)
+    for _ in range(40):
+        a = rng.getrandbits(rng.randrange(10, 600))
+        pa = rng.randrange(0, 12)
+        digits = rng.randrange(1, 60)
+        val = Fraction(a, 1 << (28 * pa))
+        s = calc("fprint", hx(a), pa, digits)
+        ip, fp = s.split(".")
+        # round half up at `digits` decimals
+        scaled = val * 10 ** digits
+        q = int(scaled)
+        if scaled - q >= Fraction(1, 2):
+            q += 1
+        assert int(ip) * 10 ** digits + int(fp) == q
+        assert len(fp) == digits
+
+
+def test_products_stay_on_host_below_16384_bits(calc):
+    g0, h0 = map(int, calc("counts").split())
+    calc("mul", "f" * 100, "e" * 100)
+    g1, h1 = map(int, calc("counts").split())
+    assert (g1, h1) == (g0, h0 + 1)
+
+
+@pytest.mark.gpu
+def test_large_ops_through_engine(calc):
+    rng = random.Random(9)
+    g0, _ = map(int, calc("counts").split())
+    for bits in [20000, 100000, 1 << 20]:
+        a, b = rng.getrandbits(bits) | 1, -(rng.getrandbits(bits - 7) | 1)
+        assert unhx(calc("mul", hx(a), hx(b))) == a * b
+        assert unhx(calc("sq", hx(b))) == b * b
+        for mode in range(4):
+            q, r = calc("divmod", hx(a * 3 + 17), hx(b), mode).split()
+            assert (unhx(q), unhx(r)) == divmod_mode(a * 3 + 17, b, mode)
+    g1, _ = map(int, calc("counts").split())
+    assert g1 > g0  # the big products went through the GPU engine
+    a = rng.getrandbits(200000) | 1
+    assert calc("todec", hx(a)) == str(a)
+    assert unhx(calc("fromdec", str(a))) == a
+
+
+@pytest.mark.gpu
+def test_large_reciprocal_matches_model(calc):
+    rng = random.Random(10)
+    for prec in [600, 3000]:
+        a = rng.getrandbits(28 * 700) | 1
+        f = BF.from_int(a, -28 * 650)
+        assert calc("frecip", hx(a), 650, prec) == xf(BF.reciprocal(f, prec))
