@@ -26,6 +26,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <array>
+#include <utility>
+
 #include "ntt_device.h"
 #include "ntt_plan.h"
 
@@ -38,20 +41,56 @@ constexpr int kLdsWords = kTile + kTile / 16; // data tile with one pad word per
 
 __device__ __forceinline__ uint32_t lds_idx(uint32_t a) { return a + (a >> 4); }
 
-// ---------------------------------------------------------------------------------------------
-// Twiddle table for one tile: tw[k], k in [1, R), for the stage with tr = R / (2 * 2^floor(log2 k))
-// rows of half-span:  tw[k] = W[k] * beta^tr  (beta = 1 for the first block of a pass).
-template <int I>
-__device__ void build_twiddles(uint32_t* __restrict__ tw, uint32_t* __restrict__ bpow,
-                               const uint32_t* __restrict__ wtab, uint32_t beta, uint32_t log_r,
-                               uint32_t nthreads) {
-    const uint32_t R = 1u << log_r;
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    [&]<int... Is>(std::integer_sequence<int, Is...>) {
+        (f(std::integral_constant<int, Is>{}), ...);
+    }(std::make_integer_sequence<int, N>{});
+}
+
+// Register-group schedule of a tile with 2^LR rows: ceil(LR/4) groups from the top rows down,
+// four radix-2 stages each except the last, which takes the remainder.  Group gi covers the
+// stages whose row half-span tr satisfies 2^trlo(gi) <= tr < 2^(trlo(gi) + stages(gi)).
+template <int LR>
+struct Sched {
+    static constexpr int NG = (LR + 3) / 4;
+    static constexpr int stages(int gi) { return (LR - 4 * gi) >= 4 ? 4 : (LR - 4 * gi); }
+    static constexpr int trlo(int gi) { return LR - 4 * gi - stages(gi); }
+};
+
+// Tile of 2^LR rows x 2^LC columns, 16 elements per thread.  In a group with EG = 2^ST elements
+// per register group at row stride 2^TL, sub-group sg of thread tid holds rows
+// base + q * 2^TL (q < EG) of column col, where s = tid + T*sg is split as
+// (rh | rl | col) = (s >> (LC + TL), (s >> LC) & (2^TL - 1), s & (2^LC - 1)) and
+// base = rh * 2^(TL + ST) + rl.
+template <int LR, int LC, int ST, int TL>
+struct Layout {
+    static constexpr int EG = 1 << ST;
+    static constexpr int NSG = kE / EG;
+    static constexpr uint32_t T = 1u << (LR + LC - 4);
+    __device__ __forceinline__ static uint32_t s_of(int sg) { return threadIdx.x + T * (uint32_t)sg; }
+    __device__ __forceinline__ static uint32_t col(int sg) { return s_of(sg) & ((1u << LC) - 1u); }
+    __device__ __forceinline__ static uint32_t rh(int sg) { return s_of(sg) >> (LC + TL); }
+    __device__ __forceinline__ static uint32_t row(int sg, int q) {
+        const uint32_t s = s_of(sg);
+        const uint32_t rl = (s >> LC) & ((1u << TL) - 1u);
+        return ((s >> (LC + TL)) << (TL + ST)) + rl + ((uint32_t)q << TL);
+    }
+};
+
+// Twiddle table of one tile: tw[k], k in [1, 2^LR), for the stage with half-span
+// tr = 2^LR / (2 * 2^floor(log2 k)) rows:  tw[k] = W[k] * beta^tr  (beta == 0 encodes beta = 1).
+template <int I, int LR>
+__device__ __forceinline__ void build_twiddles(uint32_t* __restrict__ tw, uint32_t* __restrict__ bpow,
+                                               const uint32_t* __restrict__ wtab, uint32_t beta,
+                                               uint32_t nthreads) {
+    constexpr uint32_t R = 1u << LR;
     const uint32_t tid = threadIdx.x;
-    if (beta == 0) {  // beta == 1 marker: plain copy of the root table
+    if (beta == 0) {
         for (uint32_t k = tid; k < R; k += nthreads) tw[k] = wtab[k];
         return;
     }
-    for (uint32_t e = tid; e < log_r; e += nthreads) {
+    for (uint32_t e = tid; e < (uint32_t)LR; e += nthreads) {
         uint32_t b = beta;
         for (uint32_t j = 0; j < e; ++j) b = mont_mul<I>(b, b);
         bpow[e] = b;  // beta^(2^e)
@@ -60,257 +99,118 @@ __device__ void build_twiddles(uint32_t* __restrict__ tw, uint32_t* __restrict__
     for (uint32_t k = tid; k < R; k += nthreads) {
         if (k == 0) continue;
         const uint32_t log_m = 31u - __builtin_clz(k);
-        const uint32_t log_tr = log_r - 1u - log_m;
-        tw[k] = mont_mul<I>(wtab[k], bpow[log_tr]);
+        tw[k] = mont_mul<I>(wtab[k], bpow[LR - 1 - log_m]);
     }
 }
 
-// ---------------------------------------------------------------------------------------------
-// Register groups.  A group of EG = 2^e elements at rows base + q*tr_lo (q < EG) runs e radix-2
-// stages.  Every thread holds 16 elements = 16/EG groups.  Row index of element (sg, q) is
-// base[sg] + (q << log_trlo); the twiddle index of a butterfly whose lower row is r at the stage
-// with log2(tr) = lt is (R >> (lt + 1)) + (r >> (lt + 1)).
-template <int I, int EG>
-__device__ __forceinline__ void fwd_group(uint32_t (&x)[kE], const uint32_t* __restrict__ tw,
-                                          uint32_t log_r, uint32_t log_trlo,
-                                          const uint32_t (&base)[kE / EG]) {
+// Forward (Cooley-Tukey) or inverse (Gentleman-Sande) stages of one register group.
+// Butterfly with lower row r at the stage with half-span 2^lt uses tw[2^(LR-lt-1) + (r >> (lt+1))].
+template <int I, bool INV, int LR, int LC, int ST, int TL>
+__device__ __forceinline__ void group_stages(uint32_t (&x)[kE], const uint32_t* __restrict__ tw) {
+    using Ly = Layout<LR, LC, ST, TL>;
+    constexpr int EG = Ly::EG;
+    static_for<ST>([&](auto hi) {
+        constexpr int lh = INV ? (int)hi : (ST - 1 - (int)hi);  // log2 of the span in q units
+        constexpr int h = 1 << lh;
+        constexpr int lt = TL + lh;
 #pragma unroll
-    for (int h = EG / 2; h >= 1; h >>= 1) {
-        const uint32_t lt = log_trlo + (31u - __builtin_clz((uint32_t)h));
-#pragma unroll
-        for (int sg = 0; sg < kE / EG; ++sg) {
-#pragma unroll
-            for (int q = 0; q < EG; ++q) {
-                if (q & h) continue;
-                const uint32_t r = base[sg] + ((uint32_t)q << log_trlo);
-                const uint32_t w = tw[(1u << (log_r - lt - 1u)) + (r >> (lt + 1u))];
-                const uint32_t u = x[sg * EG + q];
-                const uint32_t v = mont_mul<I>(x[sg * EG + q + h], w);
-                x[sg * EG + q] = add_mod<I>(u, v);
-                x[sg * EG + q + h] = sub_mod<I>(u, v);
-            }
-        }
-    }
-}
-
-template <int I, int EG>
-__device__ __forceinline__ void inv_group(uint32_t (&x)[kE], const uint32_t* __restrict__ tw,
-                                          uint32_t log_r, uint32_t log_trlo,
-                                          const uint32_t (&base)[kE / EG]) {
-#pragma unroll
-    for (int h = 1; h <= EG / 2; h <<= 1) {
-        const uint32_t lt = log_trlo + (31u - __builtin_clz((uint32_t)h));
-#pragma unroll
-        for (int sg = 0; sg < kE / EG; ++sg) {
+        for (int sg = 0; sg < Ly::NSG; ++sg) {
+            const uint32_t rhv = Ly::rh(sg);
 #pragma unroll
             for (int q = 0; q < EG; ++q) {
                 if (q & h) continue;
-                const uint32_t r = base[sg] + ((uint32_t)q << log_trlo);
-                const uint32_t w = tw[(1u << (log_r - lt - 1u)) + (r >> (lt + 1u))];
+                const uint32_t k = (1u << (LR - lt - 1)) + (rhv << (TL + ST - lt - 1)) +
+                                   ((uint32_t)q >> (lh + 1));
+                const uint32_t w = tw[k];
                 const uint32_t u = x[sg * EG + q];
                 const uint32_t v = x[sg * EG + q + h];
-                x[sg * EG + q] = add_mod<I>(u, v);
-                x[sg * EG + q + h] = mont_mul<I>(sub_mod<I>(u, v), w);
+                if (!INV) {
+                    const uint32_t vw = mont_mul<I>(v, w);
+                    x[sg * EG + q] = add_mod<I>(u, vw);
+                    x[sg * EG + q + h] = sub_mod<I>(u, vw);
+                } else {
+                    x[sg * EG + q] = add_mod<I>(u, v);
+                    x[sg * EG + q + h] = mont_mul<I>(sub_mod<I>(u, v), w);
+                }
             }
         }
-    }
+    });
 }
 
-// Tile geometry: R = 2^log_r rows, C = 2^log_c columns, threads = R*C/16.
-struct Geom {
-    uint32_t log_r, log_c;
-};
-
-// Row base and column of sub-group sg for a group of EG elements with row stride tr_lo.
-template <int EG>
-__device__ __forceinline__ void group_layout(const Geom& g, uint32_t log_trlo, uint32_t nthreads,
-                                             uint32_t (&base)[kE / EG], uint32_t (&col)[kE / EG]) {
-    constexpr uint32_t le = (EG == 2) ? 1 : (EG == 4) ? 2 : (EG == 8) ? 3 : 4;
+template <int LR, int LC, int ST, int TL, bool STORE>
+__device__ __forceinline__ void lds_io(uint32_t* lds, uint32_t (&x)[kE]) {
+    using Ly = Layout<LR, LC, ST, TL>;
 #pragma unroll
-    for (int sg = 0; sg < kE / EG; ++sg) {
-        const uint32_t s = threadIdx.x + nthreads * (uint32_t)sg;
-        col[sg] = s & ((1u << g.log_c) - 1u);
-        const uint32_t rest = s >> g.log_c;
-        const uint32_t rl = rest & ((1u << log_trlo) - 1u);
-        const uint32_t rh = rest >> log_trlo;
-        base[sg] = (rh << (log_trlo + le)) + rl;
-    }
-}
-
-// The group schedule of a tile: ceil(log_r / 4) groups from the top rows down; all groups have
-// 4 stages except the last, which takes the remainder.
-__device__ __forceinline__ int num_groups(uint32_t log_r) { return (int)((log_r + 3u) / 4u); }
-__device__ __forceinline__ uint32_t group_stages(uint32_t log_r, int gi) {
-    const uint32_t done = 4u * (uint32_t)gi;
-    return (log_r - done) >= 4u ? 4u : (log_r - done);
-}
-__device__ __forceinline__ uint32_t group_trlo(uint32_t log_r, int gi) {
-    return log_r - 4u * (uint32_t)gi - group_stages(log_r, gi);
-}
-
-// Element addressing callbacks: a tile element (row r, column c) <-> register slot.
-// LDS exchange: write the current layout, read the next.
-template <int EG>
-__device__ __forceinline__ void lds_store(uint32_t* lds, const uint32_t (&x)[kE], const Geom& g,
-                                          uint32_t log_trlo, uint32_t nthreads) {
-    uint32_t base[kE / EG], col[kE / EG];
-    group_layout<EG>(g, log_trlo, nthreads, base, col);
+    for (int sg = 0; sg < Ly::NSG; ++sg) {
+        const uint32_t c = Ly::col(sg);
 #pragma unroll
-    for (int sg = 0; sg < kE / EG; ++sg)
-#pragma unroll
-        for (int q = 0; q < EG; ++q) {
-            const uint32_t r = base[sg] + ((uint32_t)q << log_trlo);
-            lds[lds_idx((r << g.log_c) + col[sg])] = x[sg * EG + q];
+        for (int q = 0; q < Ly::EG; ++q) {
+            const uint32_t a = lds_idx((Ly::row(sg, q) << LC) + c);
+            if (STORE) lds[a] = x[sg * Ly::EG + q];
+            else x[sg * Ly::EG + q] = lds[a];
         }
+    }
 }
 
-template <int EG>
-__device__ __forceinline__ void lds_load(const uint32_t* lds, uint32_t (&x)[kE], const Geom& g,
-                                         uint32_t log_trlo, uint32_t nthreads) {
-    uint32_t base[kE / EG], col[kE / EG];
-    group_layout<EG>(g, log_trlo, nthreads, base, col);
+// Global memory <-> registers in a group layout.  Element (r, c) of the tile is at
+// tile_base + c + (r << log_s) of the plane; limb input reads zero at and past `len`.
+template <int LR, int LC, int ST, int TL, bool STORE>
+__device__ __forceinline__ void global_io(uint32_t* __restrict__ plane,
+                                          const uint32_t* __restrict__ limbs, uint32_t len,
+                                          uint32_t (&x)[kE], uint32_t tile_base, uint32_t log_s) {
+    using Ly = Layout<LR, LC, ST, TL>;
 #pragma unroll
-    for (int sg = 0; sg < kE / EG; ++sg)
+    for (int sg = 0; sg < Ly::NSG; ++sg) {
+        const uint32_t c = Ly::col(sg);
 #pragma unroll
-        for (int q = 0; q < EG; ++q) {
-            const uint32_t r = base[sg] + ((uint32_t)q << log_trlo);
-            x[sg * EG + q] = lds[lds_idx((r << g.log_c) + col[sg])];
+        for (int q = 0; q < Ly::EG; ++q) {
+            const uint32_t idx = tile_base + c + (Ly::row(sg, q) << log_s);
+            if (STORE) plane[idx] = x[sg * Ly::EG + q];
+            else if (limbs) x[sg * Ly::EG + q] = idx < len ? limbs[idx] : 0u;
+            else x[sg * Ly::EG + q] = plane[idx];
         }
-}
-
-template <int I, bool INV>
-__device__ __forceinline__ void run_group(int eg, uint32_t (&x)[kE], const uint32_t* tw,
-                                          const Geom& g, uint32_t log_trlo, uint32_t nthreads) {
-    switch (eg) {
-#define BI_GROUP_CASE(EGV)                                                       \
-    case EGV: {                                                                  \
-        uint32_t base[kE / EGV], col[kE / EGV];                                  \
-        group_layout<EGV>(g, log_trlo, nthreads, base, col);                     \
-        if (INV) inv_group<I, EGV>(x, tw, g.log_r, log_trlo, base);              \
-        else fwd_group<I, EGV>(x, tw, g.log_r, log_trlo, base);                  \
-        break;                                                                   \
-    }
-        BI_GROUP_CASE(2)
-        BI_GROUP_CASE(4)
-        BI_GROUP_CASE(8)
-        BI_GROUP_CASE(16)
-#undef BI_GROUP_CASE
-        default: break;
     }
 }
 
-__device__ __forceinline__ void exchange(uint32_t* lds, uint32_t (&x)[kE], const Geom& g,
-                                         int eg_from, uint32_t trlo_from, int eg_to,
-                                         uint32_t trlo_to, uint32_t nthreads) {
-    __syncthreads();  // previous readers of lds are done
-    switch (eg_from) {
-        case 2: lds_store<2>(lds, x, g, trlo_from, nthreads); break;
-        case 4: lds_store<4>(lds, x, g, trlo_from, nthreads); break;
-        case 8: lds_store<8>(lds, x, g, trlo_from, nthreads); break;
-        default: lds_store<16>(lds, x, g, trlo_from, nthreads); break;
-    }
-    __syncthreads();
-    switch (eg_to) {
-        case 2: lds_load<2>(lds, x, g, trlo_to, nthreads); break;
-        case 4: lds_load<4>(lds, x, g, trlo_to, nthreads); break;
-        case 8: lds_load<8>(lds, x, g, trlo_to, nthreads); break;
-        default: lds_load<16>(lds, x, g, trlo_to, nthreads); break;
-    }
-}
-
-// Full forward transform of the tile's R-point columns.  x arrives in the layout of group 0 and
-// leaves in the layout of the last group.
-template <int I>
-__device__ void tile_forward(uint32_t (&x)[kE], uint32_t* lds, const uint32_t* tw, const Geom& g,
-                             uint32_t nthreads) {
-    const int ng = num_groups(g.log_r);
-    for (int gi = 0; gi < ng; ++gi) {
-        const uint32_t st = group_stages(g.log_r, gi);
-        const uint32_t tl = group_trlo(g.log_r, gi);
-        if (gi > 0) {
-            const uint32_t pst = group_stages(g.log_r, gi - 1);
-            exchange(lds, x, g, 1 << pst, group_trlo(g.log_r, gi - 1), 1 << st, tl, nthreads);
+// Whole tile transform.  Forward: x arrives in group 0's layout and leaves in the last group's;
+// inverse: the reverse (and scales by 2^LR).
+template <int I, bool INV, int LR, int LC>
+__device__ __forceinline__ void tile_transform(uint32_t (&x)[kE], uint32_t* lds,
+                                               const uint32_t* tw) {
+    using S = Sched<LR>;
+    static_for<S::NG>([&](auto step) {
+        constexpr int gi = INV ? (S::NG - 1 - (int)step) : (int)step;
+        if constexpr ((int)step > 0) {
+            constexpr int pg = INV ? gi + 1 : gi - 1;
+            __syncthreads();
+            lds_io<LR, LC, S::stages(pg), S::trlo(pg), true>(lds, x);
+            __syncthreads();
+            lds_io<LR, LC, S::stages(gi), S::trlo(gi), false>(lds, x);
         }
-        run_group<I, false>(1 << st, x, tw, g, tl, nthreads);
-    }
-}
-
-// Inverse of tile_forward (times R): x arrives in the last group's layout and leaves in group 0's.
-template <int I>
-__device__ void tile_inverse(uint32_t (&x)[kE], uint32_t* lds, const uint32_t* tw, const Geom& g,
-                             uint32_t nthreads) {
-    const int ng = num_groups(g.log_r);
-    for (int gi = ng - 1; gi >= 0; --gi) {
-        const uint32_t st = group_stages(g.log_r, gi);
-        const uint32_t tl = group_trlo(g.log_r, gi);
-        if (gi < ng - 1) {
-            const uint32_t pst = group_stages(g.log_r, gi + 1);
-            exchange(lds, x, g, 1 << pst, group_trlo(g.log_r, gi + 1), 1 << st, tl, nthreads);
-        }
-        run_group<I, true>(1 << st, x, tw, g, tl, nthreads);
-    }
-}
-
-// Global memory <-> registers in a given group layout.  Element (r, c) of tile (blk, cg) is at
-// index blk * (R*S) + cg * C + c + r * S of a plane.  Limb input reads zero past `len`.
-template <int EG, bool STORE>
-__device__ __forceinline__ void global_io(uint32_t* plane, const uint32_t* limbs, uint32_t len,
-                                          uint32_t (&x)[kE], const Geom& g, uint32_t log_trlo,
-                                          uint32_t tile_base, uint32_t log_s, uint32_t nthreads) {
-    uint32_t base[kE / EG], col[kE / EG];
-    group_layout<EG>(g, log_trlo, nthreads, base, col);
-#pragma unroll
-    for (int sg = 0; sg < kE / EG; ++sg)
-#pragma unroll
-        for (int q = 0; q < EG; ++q) {
-            const uint32_t r = base[sg] + ((uint32_t)q << log_trlo);
-            const uint32_t idx = tile_base + col[sg] + (r << log_s);
-            if (STORE) {
-                plane[idx] = x[sg * EG + q];
-            } else if (limbs) {
-                x[sg * EG + q] = idx < len ? limbs[idx] : 0u;
-            } else {
-                x[sg * EG + q] = plane[idx];
-            }
-        }
-}
-
-template <bool STORE>
-__device__ __forceinline__ void global_io_eg(int eg, uint32_t* plane, const uint32_t* limbs,
-                                             uint32_t len, uint32_t (&x)[kE], const Geom& g,
-                                             uint32_t log_trlo, uint32_t tile_base, uint32_t log_s,
-                                             uint32_t nthreads) {
-    switch (eg) {
-        case 2: global_io<2, STORE>(plane, limbs, len, x, g, log_trlo, tile_base, log_s, nthreads); break;
-        case 4: global_io<4, STORE>(plane, limbs, len, x, g, log_trlo, tile_base, log_s, nthreads); break;
-        case 8: global_io<8, STORE>(plane, limbs, len, x, g, log_trlo, tile_base, log_s, nthreads); break;
-        default: global_io<16, STORE>(plane, limbs, len, x, g, log_trlo, tile_base, log_s, nthreads); break;
-    }
+        group_stages<I, INV, LR, LC, S::stages(gi), S::trlo(gi)>(x, tw);
+    });
 }
 
 // ---------------------------------------------------------------------------------------------
-// Outer (strided) pass.  grid.x = tiles, grid.y = job, grid.z = prime * nops + operand.
-template <int I, bool INV>
-__device__ void outer_pass_body(const PassArgs& a, const PassJob& job, uint32_t op) {
-    __shared__ uint32_t lds[kLdsWords];
-    __shared__ uint32_t tw[kTile];
-    __shared__ uint32_t bpow[16];
-    const Geom g{a.log_r, a.log_c};
-    const uint32_t nthreads = kThreads;
+// Outer (strided) pass: tile = 2^LR rows x 2^(12-LR) columns at row stride S = 2^log_s.
+// grid.x = tiles, grid.y = job, grid.z = prime * nops + operand.
+template <int I, bool INV, int LR>
+__device__ __forceinline__ void outer_body(const PassArgs& a, const PassJob& job, uint32_t op,
+                                           uint32_t* lds, uint32_t* tw, uint32_t* bpow) {
+    constexpr int LC = 12 - LR;
+    using S = Sched<LR>;
     const uint32_t n = 1u << a.log_n;
-    // tiles: ncg column groups per block, XCD-friendly order (consecutive tiles share lines)
     const uint32_t ntiles = gridDim.x;
     uint32_t t = blockIdx.x;
-    if ((ntiles & 7u) == 0u) t = (t & 7u) * (ntiles >> 3) + (t >> 3);
-    const uint32_t log_ncg = a.log_s - a.log_c;
+    if ((ntiles & 7u) == 0u) t = (t & 7u) * (ntiles >> 3) + (t >> 3);  // neighbours on one XCD
+    const uint32_t log_ncg = a.log_s - LC;
     const uint32_t blk = t >> log_ncg;
     const uint32_t cg = t & ((1u << log_ncg) - 1u);
-    const uint32_t tile_base = (blk << (a.log_r + a.log_s)) + (cg << a.log_c);
+    const uint32_t tile_base = (blk << (LR + a.log_s)) + (cg << LC);
 
     const uint32_t* wt = a.wtab + ((INV ? kNumPrimes : 0) + I) * kTile;
-    uint32_t beta = 0;
-    if (a.beta) beta = a.beta[I * a.beta_stride + blk];
-    build_twiddles<I>(tw, bpow, wt, blk == 0 ? 0u : beta, a.log_r, nthreads);
+    const uint32_t beta = (a.beta && blk != 0) ? a.beta[I * a.beta_stride + blk] : 0u;
+    build_twiddles<I, LR>(tw, bpow, wt, beta, kThreads);
 
     uint32_t* plane = (op == 0 ? job.res_a : job.res_b) + (size_t)I * n;
     const uint32_t* limbs = nullptr;
@@ -319,52 +219,48 @@ __device__ void outer_pass_body(const PassArgs& a, const PassJob& job, uint32_t 
         limbs = op == 0 ? job.limbs_a : job.limbs_b;
         len = op == 0 ? job.la : job.lb;
     }
+    constexpr int G0 = 0, GL = S::NG - 1;
     uint32_t x[kE];
-    const int ng = num_groups(a.log_r);
-    const int eg_first = 1 << group_stages(a.log_r, 0);
-    const int eg_last = 1 << group_stages(a.log_r, ng - 1);
-    const uint32_t tl_first = group_trlo(a.log_r, 0);
     if (!INV) {
-        global_io_eg<false>(eg_first, plane, limbs, len, x, g, tl_first, tile_base, a.log_s, nthreads);
-        __syncthreads();  // twiddles ready
-        tile_forward<I>(x, lds, tw, g, nthreads);
-        global_io_eg<true>(eg_last, plane, nullptr, 0, x, g, 0, tile_base, a.log_s, nthreads);
-    } else {
-        global_io_eg<false>(eg_last, plane, nullptr, 0, x, g, 0, tile_base, a.log_s, nthreads);
+        global_io<LR, LC, S::stages(G0), S::trlo(G0), false>(plane, limbs, len, x, tile_base, a.log_s);
         __syncthreads();
-        tile_inverse<I>(x, lds, tw, g, nthreads);
-        global_io_eg<true>(eg_first, plane, nullptr, 0, x, g, tl_first, tile_base, a.log_s, nthreads);
+        tile_transform<I, false, LR, LC>(x, lds, tw);
+        global_io<LR, LC, S::stages(GL), S::trlo(GL), true>(plane, nullptr, 0, x, tile_base, a.log_s);
+    } else {
+        global_io<LR, LC, S::stages(GL), S::trlo(GL), false>(plane, nullptr, 0, x, tile_base, a.log_s);
+        __syncthreads();
+        tile_transform<I, true, LR, LC>(x, lds, tw);
+        global_io<LR, LC, S::stages(G0), S::trlo(G0), true>(plane, nullptr, 0, x, tile_base, a.log_s);
     }
 }
 
-template <bool INV>
+template <bool INV, int LR>
 __global__ __launch_bounds__(kThreads) void k_outer_pass(PassArgs a) {
+    __shared__ uint32_t lds[kLdsWords];
+    __shared__ uint32_t tw[1 << LR];
+    __shared__ uint32_t bpow[16];
     const PassJob& job = a.jobs ? a.jobs[blockIdx.y] : a.job;
     const uint32_t nops = INV ? 1u : ((a.flags & kSquare) ? 1u : 2u);
     const uint32_t prime = blockIdx.z / nops, op = blockIdx.z % nops;
     switch (prime) {
-        case 0: outer_pass_body<0, INV>(a, job, op); break;
-        case 1: outer_pass_body<1, INV>(a, job, op); break;
-        default: outer_pass_body<2, INV>(a, job, op); break;
+        case 0: outer_body<0, INV, LR>(a, job, op, lds, tw, bpow); break;
+        case 1: outer_body<1, INV, LR>(a, job, op, lds, tw, bpow); break;
+        default: outer_body<2, INV, LR>(a, job, op, lds, tw, bpow); break;
     }
 }
 
 // ---------------------------------------------------------------------------------------------
-// Block pass with the pointwise product: contiguous blocks of B = 2^log_r points (C = S = 1).
+// Block pass with the pointwise product: contiguous blocks of 2^LR points (one column).
 // Forward-transform the block of A (and of B unless squaring), multiply pointwise, inverse
-// transform the block, store into A's plane.  grid.x = n / B, grid.y = job, grid.z = prime.
-template <int I>
-__device__ void mid_pass_body(const PassArgs& a, const PassJob& job) {
-    __shared__ uint32_t lds[kLdsWords];
-    __shared__ uint32_t twf[kTile];
-    __shared__ uint32_t twi[kTile];
-    __shared__ uint32_t bpow[2][16];
-    const Geom g{a.log_r, 0};
-    const uint32_t nthreads = blockDim.x;
+// transform, store into A's plane.  grid.x = n >> LR, grid.y = job, grid.z = prime.
+template <int I, int LR>
+__device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, uint32_t* lds,
+                                         uint32_t* twf, uint32_t* twi, uint32_t* bpow) {
+    using S = Sched<LR>;
+    constexpr uint32_t T = 1u << (LR - 4);
     const uint32_t n = 1u << a.log_n;
     const uint32_t blk = blockIdx.x;
-    const uint32_t tile_base = blk << a.log_r;
-
+    const uint32_t tile_base = blk << LR;
     const uint32_t* wf = a.wtab + I * kTile;
     const uint32_t* wi = a.wtab + (kNumPrimes + I) * kTile;
     uint32_t bf = 0, bi_ = 0;
@@ -372,43 +268,45 @@ __device__ void mid_pass_body(const PassArgs& a, const PassJob& job) {
         bf = a.beta[I * a.beta_stride + blk];
         bi_ = a.beta_inv[I * a.beta_stride + blk];
     }
-    build_twiddles<I>(twf, bpow[0], wf, bf, a.log_r, nthreads);
-    build_twiddles<I>(twi, bpow[1], wi, bi_, a.log_r, nthreads);
+    build_twiddles<I, LR>(twf, bpow, wf, bf, T);
+    build_twiddles<I, LR>(twi, bpow + 16, wi, bi_, T);
 
     const bool from_limbs = (a.flags & kFromLimbs) != 0;
     const bool square = (a.flags & kSquare) != 0;
     uint32_t* pa = job.res_a + (size_t)I * n;
-    uint32_t* pb = job.res_b ? job.res_b + (size_t)I * n : nullptr;
-    const int ng = num_groups(a.log_r);
-    const int eg_first = 1 << group_stages(a.log_r, 0);
-    const uint32_t tl_first = group_trlo(a.log_r, 0);
-
-    uint32_t xa[kE], xb[kE];
-    global_io_eg<false>(eg_first, pa, from_limbs ? job.limbs_a : nullptr, job.la, xa, g, tl_first,
-                        tile_base, 0, nthreads);
+    constexpr int G0 = 0;
+    uint32_t xa[kE];
+    global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pa, from_limbs ? job.limbs_a : nullptr,
+                                                        job.la, xa, tile_base, 0);
     __syncthreads();  // twiddles ready
-    tile_forward<I>(xa, lds, twf, g, nthreads);
+    tile_transform<I, false, LR, 0>(xa, lds, twf);
     if (!square) {
-        global_io_eg<false>(eg_first, pb, from_limbs ? job.limbs_b : nullptr, job.lb, xb, g,
-                            tl_first, tile_base, 0, nthreads);
-        tile_forward<I>(xb, lds, twf, g, nthreads);
+        uint32_t* pb = job.res_b + (size_t)I * n;
+        uint32_t xb[kE];
+        global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pb, from_limbs ? job.limbs_b : nullptr,
+                                                            job.lb, xb, tile_base, 0);
+        tile_transform<I, false, LR, 0>(xb, lds, twf);
 #pragma unroll
         for (int k = 0; k < kE; ++k) xa[k] = mont_mul<I>(xa[k], xb[k]);
     } else {
 #pragma unroll
         for (int k = 0; k < kE; ++k) xa[k] = mont_mul<I>(xa[k], xa[k]);
     }
-    (void)ng;
-    tile_inverse<I>(xa, lds, twi, g, nthreads);
-    global_io_eg<true>(eg_first, pa, nullptr, 0, xa, g, tl_first, tile_base, 0, nthreads);
+    tile_transform<I, true, LR, 0>(xa, lds, twi);
+    global_io<LR, 0, S::stages(G0), S::trlo(G0), true>(pa, nullptr, 0, xa, tile_base, 0);
 }
 
-__global__ __launch_bounds__(kThreads) void k_mid_pass(PassArgs a) {
+template <int LR>
+__global__ __launch_bounds__(1 << (LR - 4)) void k_mid_pass(PassArgs a) {
+    __shared__ uint32_t lds[(1 << LR) + (1 << LR) / 16];
+    __shared__ uint32_t twf[1 << LR];
+    __shared__ uint32_t twi[1 << LR];
+    __shared__ uint32_t bpow[32];
     const PassJob& job = a.jobs ? a.jobs[blockIdx.y] : a.job;
     switch (blockIdx.z) {
-        case 0: mid_pass_body<0>(a, job); break;
-        case 1: mid_pass_body<1>(a, job); break;
-        default: mid_pass_body<2>(a, job); break;
+        case 0: mid_body<0, LR>(a, job, lds, twf, twi, bpow); break;
+        case 1: mid_body<1, LR>(a, job, lds, twf, twi, bpow); break;
+        default: mid_body<2, LR>(a, job, lds, twf, twi, bpow); break;
     }
 }
 
@@ -560,20 +458,35 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t count) {
 // Launch wrappers (host side of this translation unit).
 namespace bi {
 
+using KernelFn = void (*)(PassArgs);
+
+template <bool INV, int... LRs>
+constexpr auto outer_table(std::integer_sequence<int, LRs...>) {
+    return std::array<KernelFn, sizeof...(LRs)>{&k_outer_pass<INV, LRs + 1>...};
+}
+template <int... LRs>
+constexpr auto mid_table(std::integer_sequence<int, LRs...>) {
+    return std::array<KernelFn, sizeof...(LRs)>{&k_mid_pass<LRs + 4>...};
+}
+
 hipError_t launch_outer_pass(const PassArgs& a, bool inv, uint32_t njobs, uint32_t nops,
                              hipStream_t st) {
+    static constexpr auto fwd = outer_table<false>(std::make_integer_sequence<int, 8>{});
+    static constexpr auto bwd = outer_table<true>(std::make_integer_sequence<int, 8>{});
+    if (a.log_r < 1 || a.log_r > 8 || a.log_r + a.log_c != 12) return hipErrorInvalidValue;
     const uint32_t ntiles = 1u << (a.log_n - a.log_r - a.log_c);
     dim3 grid(ntiles, njobs, kNumPrimes * nops);
-    if (inv) hipLaunchKernelGGL(k_outer_pass<true>, grid, dim3(kThreads), 0, st, a);
-    else hipLaunchKernelGGL(k_outer_pass<false>, grid, dim3(kThreads), 0, st, a);
+    hipLaunchKernelGGL((inv ? bwd : fwd)[a.log_r - 1], grid, dim3(kThreads), 0, st, a);
     return hipGetLastError();
 }
 
 hipError_t launch_mid_pass(const PassArgs& a, uint32_t njobs, hipStream_t st) {
+    static constexpr auto tab = mid_table(std::make_integer_sequence<int, 9>{});
+    if (a.log_r < 4 || a.log_r > 12) return hipErrorInvalidValue;
     const uint32_t nblk = 1u << (a.log_n - a.log_r);
     const uint32_t threads = (1u << a.log_r) / kE;
     dim3 grid(nblk, njobs, kNumPrimes);
-    hipLaunchKernelGGL(k_mid_pass, grid, dim3(threads), 0, st, a);
+    hipLaunchKernelGGL(tab[a.log_r - 4], grid, dim3(threads), 0, st, a);
     return hipGetLastError();
 }
 
