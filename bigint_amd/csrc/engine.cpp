@@ -229,6 +229,76 @@ public:
         return ws;
     }
 
+    // Launch the whole pass sequence of one plan for njobs products of the same transform
+    // length.  single/gsingle are used when njobs == 1 and no descriptor arrays are given.
+    int launch_products(const Plan* pl, bool square, uint32_t njobs, const PassJob& single,
+                        const PassJob* d_jobs, const GarnerJob& gsingle, const GarnerJob* d_gjobs,
+                        uint32_t max_out_len, double in_limbs, double out_limbs, uint32_t* status,
+                        uint32_t status_words, hipStream_t st) {
+        const uint32_t n = 1u << pl->log_n;
+        const double nj = (double)njobs;
+        if (timed(BI_HIP_K_FILL, 4.0 * status_words, st,
+                  [&] { return launch_fill(status, 0u, status_words, st); }) != hipSuccess)
+            return BI_HIP_EDEVICE;
+        const double nops_d = square ? 1.0 : 2.0;
+        PassArgs a{};
+        a.job = single;
+        a.jobs = d_jobs;
+        a.log_n = pl->log_n;
+        a.wtab = wtab;
+        const uint32_t sq = square ? (uint32_t)kSquare : 0u;
+        const uint32_t nops = square ? 1u : 2u;
+        for (size_t k = 0; k < pl->outer.size(); ++k) {
+            const OuterPass& op = pl->outer[k];
+            a.log_r = op.log_r;
+            a.log_c = op.log_c;
+            a.log_s = op.log_s;
+            a.flags = sq | (k == 0 ? (uint32_t)kFromLimbs : 0u);
+            a.beta = op.beta;
+            a.beta_inv = nullptr;
+            a.beta_stride = op.nblk;
+            // the first pass reads the limbs once; every pass writes 3 planes per operand
+            const double bytes = k == 0 ? 4.0 * in_limbs + 12.0 * n * nops_d * nj : 24.0 * n * nops_d * nj;
+            if (timed(BI_HIP_K_OUTER_FWD, bytes, st,
+                      [&] { return launch_outer_pass(a, false, njobs, nops, st); }) != hipSuccess)
+                return BI_HIP_EDEVICE;
+        }
+        a.log_r = pl->mid_log_r;
+        a.log_c = 0;
+        a.log_s = 0;
+        a.flags = sq | (pl->outer.empty() ? (uint32_t)kFromLimbs : 0u);
+        a.beta = pl->mid_beta;
+        a.beta_inv = pl->mid_beta_inv;
+        a.beta_stride = pl->mid_nblk;
+        {
+            const double in_bytes = pl->outer.empty() ? 4.0 * in_limbs : 12.0 * n * nops_d * nj;
+            if (timed(BI_HIP_K_MID, in_bytes + 12.0 * n * nj, st,
+                      [&] { return launch_mid_pass(a, njobs, st); }) != hipSuccess)
+                return BI_HIP_EDEVICE;
+        }
+        for (size_t k = pl->outer.size(); k-- > 0;) {
+            const OuterPass& op = pl->outer[k];
+            a.log_r = op.log_r;
+            a.log_c = op.log_c;
+            a.log_s = op.log_s;
+            a.flags = 0;
+            a.beta = op.beta_inv;
+            a.beta_inv = nullptr;
+            a.beta_stride = op.nblk;
+            if (timed(BI_HIP_K_OUTER_INV, 24.0 * n * nj, st,
+                      [&] { return launch_outer_pass(a, true, njobs, 1, st); }) != hipSuccess)
+                return BI_HIP_EDEVICE;
+        }
+        GarnerArgs g{};
+        g.job = gsingle;
+        g.jobs = d_gjobs;
+        g.k = gk;
+        if (timed(BI_HIP_K_GARNER, 12.0 * n * nj + 4.0 * out_limbs, st,
+                  [&] { return launch_garner_carry(g, max_out_len, njobs, st); }) != hipSuccess)
+            return BI_HIP_EDEVICE;
+        return BI_HIP_OK;
+    }
+
     int run(const uint32_t* d_a, uint32_t la, const uint32_t* d_b, uint32_t lb, bool square,
             uint32_t* d_out, hipStream_t st) {
         const int lg = bi_hip_transform_log2(la, lb);
@@ -244,76 +314,136 @@ public:
         uint32_t* res_a = w;
         uint32_t* res_b = square ? nullptr : w + (size_t)3 * n;
         uint32_t* status = w + (size_t)3 * n * (square ? 1 : 2);
-        uint32_t* counter = status + ntiles;
+        const PassJob job{d_a, square ? d_a : d_b, la, square ? la : lb, res_a, res_b};
+        GarnerJob gj{};
+        gj.y = res_a;
+        gj.out = d_out;
+        gj.n = n;
+        gj.out_len = out_len;
+        for (int I = 0; I < 3; ++I) gj.kscale[I] = pl->kscale[I];
+        gj.status = status;
+        gj.counter = status + ntiles;
+        return launch_products(pl, square, 1, job, nullptr, gj, nullptr, out_len,
+                               (double)la + (square ? 0.0 : (double)lb), (double)out_len, status,
+                               ntiles + 1, st);
+    }
 
-        if (timed(BI_HIP_K_FILL, 4.0 * (ntiles + 1), st,
-                  [&] { return launch_fill(status, 0u, ntiles + 1, st); }) != hipSuccess)
+    // descriptor buffer for batched launches (device), grow-only
+    uint8_t* desc = nullptr;
+    size_t desc_bytes = 0;
+
+    // Batched products on device pointers, grouped by transform length and squaring.
+    int run_batch(const bi_hip_dpair* pairs, size_t count, hipStream_t st) {
+        struct Item {
+            uint32_t idx, la, lb, lg;
+            bool sq;
+        };
+        std::map<std::pair<uint32_t, bool>, std::vector<Item>> groups;
+        for (size_t i = 0; i < count; ++i) {
+            const bi_hip_dpair& p = pairs[i];
+            const bool sq = p.b == nullptr || (p.b == p.a && p.lb == p.la);
+            const size_t lb = sq ? p.la : p.lb;
+            if (!p.a || p.la == 0 || lb == 0 || !p.out || p.la > 0xFFFFFFFFull || lb > 0xFFFFFFFFull)
+                return BI_HIP_EINVAL;
+            if (p.out_cap < p.la + lb) return BI_HIP_ECAP;
+            const int lg = bi_hip_transform_log2(p.la, lb);
+            if (lg < 0) return BI_HIP_ETOOBIG;
+            groups[{(uint32_t)lg, sq}].push_back(Item{(uint32_t)i, (uint32_t)p.la, (uint32_t)lb, (uint32_t)lg, sq});
+        }
+        // sizes: workspace per group (residues + status), descriptors for all groups
+        size_t ws_need = 0, desc_need = 0;
+        for (auto& [key, items] : groups) {
+            const size_t n = (size_t)1 << key.first;
+            size_t words = 0;
+            for (auto& it : items) words += 3 * n * (it.sq ? 1 : 2) + (it.la + it.lb + 4095u) / 4096u + 1;
+            ws_need = std::max(ws_need, words + 64);
+            desc_need += items.size() * (sizeof(PassJob) + sizeof(GarnerJob)) + 256;
+        }
+        uint32_t* w = workspace(ws_need);
+        if (!w) return BI_HIP_ENOMEM;
+        if (desc_need > desc_bytes) {
+            if (desc) hipFree(desc);
+            desc = nullptr;
+            desc_bytes = 0;
+            if (hipMalloc((void**)&desc, desc_need) != hipSuccess) return BI_HIP_ENOMEM;
+            desc_bytes = desc_need;
+        }
+        std::vector<uint8_t> hdesc(desc_need);
+        struct Launch {
+            Plan* pl;
+            bool sq;
+            uint32_t njobs, max_out;
+            size_t pj_off, gj_off;
+            double in_limbs, out_limbs;
+            uint32_t status_words;
+        };
+        std::vector<Launch> launches;
+        size_t off = 0;
+        for (auto& [key, items] : groups) {
+            Plan* pl = plan(key.first);
+            if (!pl) return BI_HIP_ENOMEM;
+            const size_t n = (size_t)1 << key.first;
+            Launch L{pl, key.second, (uint32_t)items.size(), 0, 0, 0, 0.0, 0.0, 0};
+            L.pj_off = off;
+            off += items.size() * sizeof(PassJob);
+            off = (off + 255) & ~(size_t)255;
+            L.gj_off = off;
+            off += items.size() * sizeof(GarnerJob);
+            off = (off + 255) & ~(size_t)255;
+            auto* pj = reinterpret_cast<PassJob*>(hdesc.data() + L.pj_off);
+            auto* gj = reinterpret_cast<GarnerJob*>(hdesc.data() + L.gj_off);
+            // residues first, then every job's status words (one contiguous fill)
+            size_t res_words = 0;
+            for (auto& it : items) res_words += 3 * n * (it.sq ? 1 : 2);
+            uint32_t* res = w;
+            uint32_t* status = w + res_words;
+            uint32_t* st_cur = status;
+            for (size_t j = 0; j < items.size(); ++j) {
+                const Item& it = items[j];
+                const bi_hip_dpair& p = pairs[it.idx];
+                uint32_t* ra = res;
+                res += 3 * n;
+                uint32_t* rb = nullptr;
+                if (!it.sq) {
+                    rb = res;
+                    res += 3 * n;
+                }
+                pj[j] = PassJob{p.a, it.sq ? p.a : p.b, it.la, it.lb, ra, rb};
+                const uint32_t out_len = it.la + it.lb;
+                const uint32_t nt = (out_len + 4095u) / 4096u;
+                GarnerJob g{};
+                g.y = ra;
+                g.out = p.out;
+                g.n = (uint32_t)n;
+                g.out_len = out_len;
+                for (int I = 0; I < 3; ++I) g.kscale[I] = pl->kscale[I];
+                g.status = st_cur;
+                g.counter = st_cur + nt;
+                st_cur += nt + 1;
+                gj[j] = g;
+                L.max_out = std::max(L.max_out, out_len);
+                L.in_limbs += (double)it.la + (it.sq ? 0.0 : (double)it.lb);
+                L.out_limbs += (double)out_len;
+            }
+            L.status_words = (uint32_t)(st_cur - status);
+            launches.push_back(L);
+        }
+        // the descriptor buffer may still be read by an earlier batch on this stream
+        if (hipStreamSynchronize(st) != hipSuccess) return BI_HIP_EDEVICE;
+        if (hipMemcpy(desc, hdesc.data(), off, hipMemcpyHostToDevice) != hipSuccess)
             return BI_HIP_EDEVICE;
-        const double nops_d = square ? 1.0 : 2.0;
-        PassArgs a{};
-        a.job = PassJob{d_a, square ? d_a : d_b, la, square ? la : lb, res_a, res_b};
-        a.jobs = nullptr;
-        a.log_n = (uint32_t)lg;
-        a.wtab = wtab;
-        const uint32_t sq = square ? (uint32_t)kSquare : 0u;
-        const uint32_t nops = square ? 1u : 2u;
-        for (size_t k = 0; k < pl->outer.size(); ++k) {
-            const OuterPass& op = pl->outer[k];
-            a.log_r = op.log_r;
-            a.log_c = op.log_c;
-            a.log_s = op.log_s;
-            a.flags = sq | (k == 0 ? (uint32_t)kFromLimbs : 0u);
-            a.beta = op.beta;
-            a.beta_inv = nullptr;
-            a.beta_stride = op.nblk;
-            // first pass reads the limbs once; every pass writes 3 planes per operand
-            const double bytes = k == 0 ? 4.0 * ((double)la + (square ? 0.0 : (double)lb)) + 12.0 * n * nops_d
-                                        : 24.0 * n * nops_d;
-            if (timed(BI_HIP_K_OUTER_FWD, bytes, st,
-                      [&] { return launch_outer_pass(a, false, 1, nops, st); }) != hipSuccess)
-                return BI_HIP_EDEVICE;
+        for (auto& L : launches) {
+            const auto* pj = reinterpret_cast<const PassJob*>(desc + L.pj_off);
+            const auto* gj = reinterpret_cast<const GarnerJob*>(desc + L.gj_off);
+            const auto* hp = reinterpret_cast<const PassJob*>(hdesc.data() + L.pj_off);
+            const auto* hg = reinterpret_cast<const GarnerJob*>(hdesc.data() + L.gj_off);
+            uint32_t* status = hg[0].status;
+            const bool one = L.njobs == 1;
+            const int rc = launch_products(L.pl, L.sq, L.njobs, hp[0], one ? nullptr : pj, hg[0],
+                                           one ? nullptr : gj, L.max_out, L.in_limbs, L.out_limbs,
+                                           status, L.status_words, st);
+            if (rc) return rc;
         }
-        a.log_r = pl->mid_log_r;
-        a.log_c = 0;
-        a.log_s = 0;
-        a.flags = sq | (pl->outer.empty() ? (uint32_t)kFromLimbs : 0u);
-        a.beta = pl->mid_beta;
-        a.beta_inv = pl->mid_beta_inv;
-        a.beta_stride = pl->mid_nblk;
-        {
-            const double in_bytes = pl->outer.empty()
-                                        ? 4.0 * ((double)la + (square ? 0.0 : (double)lb))
-                                        : 12.0 * n * nops_d;
-            if (timed(BI_HIP_K_MID, in_bytes + 12.0 * n, st,
-                      [&] { return launch_mid_pass(a, 1, st); }) != hipSuccess)
-                return BI_HIP_EDEVICE;
-        }
-        for (size_t k = pl->outer.size(); k-- > 0;) {
-            const OuterPass& op = pl->outer[k];
-            a.log_r = op.log_r;
-            a.log_c = op.log_c;
-            a.log_s = op.log_s;
-            a.flags = 0;
-            a.beta = op.beta_inv;
-            a.beta_inv = nullptr;
-            a.beta_stride = op.nblk;
-            if (timed(BI_HIP_K_OUTER_INV, 24.0 * n, st,
-                      [&] { return launch_outer_pass(a, true, 1, 1, st); }) != hipSuccess)
-                return BI_HIP_EDEVICE;
-        }
-        GarnerArgs g{};
-        g.job.y = res_a;
-        g.job.out = d_out;
-        g.job.n = n;
-        g.job.out_len = out_len;
-        for (int I = 0; I < 3; ++I) g.job.kscale[I] = pl->kscale[I];
-        g.job.status = status;
-        g.job.counter = counter;
-        g.jobs = nullptr;
-        g.k = gk;
-        if (timed(BI_HIP_K_GARNER, 12.0 * n + 4.0 * out_len, st,
-                  [&] { return launch_garner_carry(g, out_len, 1, st); }) != hipSuccess)
-            return BI_HIP_EDEVICE;
         return BI_HIP_OK;
     }
 };
@@ -406,16 +536,110 @@ int bi_hip_mul(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, int i
     return BI_HIP_OK;
 }
 
+int bi_hip_mul_batch_device(const bi_hip_dpair* pairs, size_t count, int plan_mode,
+                            void* stream) {
+    if (!pairs && count) return BI_HIP_EINVAL;
+    if (count == 0) return BI_HIP_OK;
+    std::lock_guard<std::mutex> lk(engine().mtx);
+    bi::Engine& e = engine();
+    int rc = e.init(0);
+    if (rc) return rc;
+    hipStream_t st = stream ? (hipStream_t)stream : e.stream;
+    if (plan_mode == BI_HIP_PLAN_FIXED) {
+        for (size_t i = 0; i < count; ++i) {
+            const bi_hip_dpair& p = pairs[i];
+            const bool sq = p.b == nullptr || (p.b == p.a && p.lb == p.la);
+            const size_t lb = sq ? p.la : p.lb;
+            if (!p.a || !p.out || p.la == 0 || lb == 0) return BI_HIP_EINVAL;
+            if (p.out_cap < p.la + lb) return BI_HIP_ECAP;
+            rc = e.run(p.a, (uint32_t)p.la, sq ? p.a : p.b, (uint32_t)lb, sq, p.out, st);
+            if (rc) return rc;
+        }
+        return BI_HIP_OK;
+    }
+    return e.run_batch(pairs, count, st);
+}
+
 int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode) {
-    (void)plan_mode;
-    int first = BI_HIP_OK;
+    if (!pairs && count) return BI_HIP_EINVAL;
+    if (count == 0) return BI_HIP_OK;
+    // validate, stage every operand in one device buffer, run the device batch, copy back
+    size_t in_words = 0, out_words = 0;
     for (size_t i = 0; i < count; ++i) {
         bi_hip_pair& p = pairs[i];
         const bool sq = p.a == p.b && p.la == p.lb;
-        p.status = bi_hip_mul(p.a, p.la, p.b, p.lb, sq, p.out, p.out_cap, &p.out_len);
-        if (p.status && !first) first = p.status;
+        if (bi::check_operand(p.a, p.la) || (!sq && bi::check_operand(p.b, p.lb)) || !p.out) {
+            p.status = BI_HIP_EINVAL;
+            return BI_HIP_EINVAL;
+        }
+        if (p.a[p.la - 1] == 0 || p.b[p.lb - 1] == 0) {
+            p.status = BI_HIP_EINVAL;
+            return BI_HIP_EINVAL;
+        }
+        if (p.out_cap < p.la + p.lb) {
+            p.status = BI_HIP_ECAP;
+            return BI_HIP_ECAP;
+        }
+        if (bi_hip_transform_log2(p.la, p.lb) < 0) {
+            p.status = BI_HIP_ETOOBIG;
+            return BI_HIP_ETOOBIG;
+        }
+        in_words += p.la + (sq ? 0 : p.lb);
+        out_words += p.la + p.lb;
     }
-    return first;
+    uint32_t* dbuf = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(engine().mtx);
+        int rc = engine().init(0);
+        if (rc) return rc;
+    }
+    if (hipMalloc((void**)&dbuf, (in_words + out_words) * 4) != hipSuccess) return BI_HIP_ENOMEM;
+    std::vector<uint32_t> hin(in_words);
+    std::vector<bi_hip_dpair> dp(count);
+    size_t io = 0, oo = in_words;
+    for (size_t i = 0; i < count; ++i) {
+        bi_hip_pair& p = pairs[i];
+        const bool sq = p.a == p.b && p.la == p.lb;
+        std::memcpy(hin.data() + io, p.a, p.la * 4);
+        dp[i].a = dbuf + io;
+        dp[i].la = p.la;
+        io += p.la;
+        if (sq) {
+            dp[i].b = dp[i].a;
+        } else {
+            std::memcpy(hin.data() + io, p.b, p.lb * 4);
+            dp[i].b = dbuf + io;
+            io += p.lb;
+        }
+        dp[i].lb = p.lb;
+        dp[i].out = dbuf + oo;
+        dp[i].out_cap = p.la + p.lb;
+        oo += p.la + p.lb;
+    }
+    int rc = BI_HIP_OK;
+    if (hipMemcpy(dbuf, hin.data(), in_words * 4, hipMemcpyHostToDevice) != hipSuccess) rc = BI_HIP_EDEVICE;
+    if (!rc) rc = bi_hip_mul_batch_device(dp.data(), count, plan_mode, nullptr);
+    if (!rc) rc = bi_hip_synchronize();
+    std::vector<uint32_t> hout;
+    if (!rc) {
+        hout.resize(out_words);
+        if (hipMemcpy(hout.data(), dbuf + in_words, out_words * 4, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = BI_HIP_EDEVICE;
+    }
+    hipFree(dbuf);
+    size_t o = 0;
+    for (size_t i = 0; i < count; ++i) {
+        bi_hip_pair& p = pairs[i];
+        p.status = rc;
+        if (rc) continue;
+        const size_t len = p.la + p.lb;
+        std::memcpy(p.out, hout.data() + o, len * 4);
+        o += len;
+        size_t n = len;
+        while (n > 1 && p.out[n - 1] == 0) --n;
+        p.out_len = n;
+    }
+    return rc;
 }
 
 int bi_hip_profile(int enable) {

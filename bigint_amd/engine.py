@@ -18,7 +18,7 @@ PLAN_FIXED, PLAN_TABLE, PLAN_LEARNED = 0, 1, 2
 
 EXPORTS = ["bi_hip_init", "bi_hip_mul", "bi_hip_mul_device", "bi_hip_mul_batch",
            "bi_hip_synchronize", "bi_hip_transform_log2", "bi_hip_strerror", "bi_hip_profile",
-           "bi_hip_profile_read"]
+           "bi_hip_profile_read", "bi_hip_mul_batch_device"]
 KERNEL_NAMES = ["fill", "outer_fwd", "mid", "outer_inv", "garner_carry"]
 
 u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -28,6 +28,11 @@ class Pair(ctypes.Structure):
     _fields_ = [("a", u32p), ("la", ctypes.c_size_t), ("b", u32p), ("lb", ctypes.c_size_t),
                 ("out", u32p), ("out_cap", ctypes.c_size_t), ("out_len", ctypes.c_size_t),
                 ("status", ctypes.c_int)]
+
+
+class DPair(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_void_p), ("la", ctypes.c_size_t), ("b", ctypes.c_void_p),
+                ("lb", ctypes.c_size_t), ("out", ctypes.c_void_p), ("out_cap", ctypes.c_size_t)]
 
 
 class KStat(ctypes.Structure):
@@ -58,6 +63,8 @@ def lib() -> ctypes.CDLL:
                                         ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p,
                                         ctypes.c_size_t, ctypes.c_void_p]
         L.bi_hip_mul_batch.argtypes = [ctypes.POINTER(Pair), ctypes.c_size_t, ctypes.c_int]
+        L.bi_hip_mul_batch_device.argtypes = [ctypes.POINTER(DPair), ctypes.c_size_t, ctypes.c_int,
+                                              ctypes.c_void_p]
         L.bi_hip_transform_log2.argtypes = [ctypes.c_size_t, ctypes.c_size_t]
         L.bi_hip_profile.argtypes = [ctypes.c_int]
         L.bi_hip_profile_read.argtypes = [ctypes.POINTER(KStat), ctypes.c_size_t]
@@ -121,6 +128,15 @@ def mul_batch(pairs: list[tuple[np.ndarray, np.ndarray | None]], plan_mode: int 
     if rc:
         raise EngineError(rc, "bi_hip_mul_batch")
     return [outs[i][: arr[i].out_len] for i in range(len(pairs))]
+
+
+def mul_batch_device(pairs: list[tuple[int, int, int, int, int, int]], plan_mode: int = PLAN_TABLE,
+                     stream: int = 0) -> None:
+    """Device batch: each pair is (d_a, la, d_b or 0, lb, d_out, out_cap) with raw device pointers."""
+    arr = (DPair * len(pairs))(*[DPair(a, la, b or None, lb, o, cap) for a, la, b, lb, o, cap in pairs])
+    rc = lib().bi_hip_mul_batch_device(arr, len(pairs), plan_mode, stream or None)
+    if rc:
+        raise EngineError(rc, "bi_hip_mul_batch_device")
 
 
 def synchronize() -> None:

@@ -79,6 +79,23 @@ int bi_hip_mul(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, int i
 int bi_hip_mul_device(const uint32_t* d_a, size_t la, const uint32_t* d_b, size_t lb,
                       int is_square, uint32_t* d_out, size_t out_cap, void* stream);
 
+/* One product of a device-resident batch: a, b, out are device pointers.  b == NULL or
+ * (b == a, lb == la) selects squaring.  Exactly la + lb limbs are written to out. */
+typedef struct bi_hip_dpair {
+    const uint32_t* a;
+    size_t la;
+    const uint32_t* b;
+    size_t lb;
+    uint32_t* out;
+    size_t out_cap;
+} bi_hip_dpair;
+
+/* Device-resident batch, asynchronous on `stream` (NULL = the engine's stream).
+ * BI_HIP_PLAN_FIXED runs one launch sequence per product; the other modes group products of
+ * equal transform length into one launch sequence per group.  The call waits for earlier work
+ * on the stream before it reuses its descriptor buffer. */
+int bi_hip_mul_batch_device(const bi_hip_dpair* pairs, size_t count, int plan_mode, void* stream);
+
 /* Batch of independent products (host pointers).  Each pair gets its own status; the call
  * returns the first nonzero status or BI_HIP_OK. */
 int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode);

@@ -143,3 +143,24 @@ def test_batch_matches_single():
     outs = engine.mul_batch(pairs)
     for (a, b), p in zip(pairs, outs):
         assert np.array_equal(p, oracle.mul_limbs(a, b))
+
+
+@pytest.mark.parametrize("mode", [engine.PLAN_FIXED, engine.PLAN_TABLE])
+def test_batch_grouped_matches_oracle(mode):
+    """Many products of mixed transform lengths in one grouped launch sequence."""
+    rng = random.Random(40 + mode)
+    pairs = []
+    for i in range(60):
+        bits_a = int(2 ** rng.uniform(5, 18))
+        a = L.random_operand(bits_a, rng.getrandbits(63))
+        if i % 7 == 0:
+            b = None
+        else:
+            b = L.random_operand(int(2 ** rng.uniform(5, 18)), rng.getrandbits(63))
+        pairs.append((a, b))
+    # equal lengths too, so some groups have many jobs
+    for i in range(20):
+        pairs.append((L.random_operand(28 * 3000, 1000 + i), L.random_operand(28 * 2000, 2000 + i)))
+    outs = engine.mul_batch(pairs, mode)
+    for (a, b), p in zip(pairs, outs):
+        assert np.array_equal(p, oracle.mul_limbs(a, b))
