@@ -232,9 +232,9 @@ public:
     // Launch the whole pass sequence of one plan for njobs products of the same transform
     // length.  single/gsingle are used when njobs == 1 and no descriptor arrays are given.
     int launch_products(const Plan* pl, bool square, uint32_t njobs, const PassJob& single,
-                        const PassJob* d_jobs, const GarnerJob& gsingle, const GarnerJob* d_gjobs,
-                        uint32_t max_out_len, double in_limbs, double out_limbs, uint32_t* status,
-                        uint32_t status_words, hipStream_t st) {
+                        const PassJob* d_jobs, uint32_t ngjobs, const GarnerJob& gsingle,
+                        const GarnerJob* d_gjobs, uint32_t max_out_len, double in_limbs,
+                        double out_limbs, uint32_t* status, uint32_t status_words, hipStream_t st) {
         const uint32_t n = 1u << pl->log_n;
         const double nj = (double)njobs;
         if (timed(BI_HIP_K_FILL, 4.0 * status_words, st,
@@ -294,7 +294,7 @@ public:
         g.jobs = d_gjobs;
         g.k = gk;
         if (timed(BI_HIP_K_GARNER, 12.0 * n * nj + 4.0 * out_limbs, st,
-                  [&] { return launch_garner_carry(g, max_out_len, njobs, st); }) != hipSuccess)
+                  [&] { return launch_garner_carry(g, max_out_len, ngjobs, st); }) != hipSuccess)
             return BI_HIP_EDEVICE;
         return BI_HIP_OK;
     }
@@ -323,7 +323,7 @@ public:
         for (int I = 0; I < 3; ++I) gj.kscale[I] = pl->kscale[I];
         gj.status = status;
         gj.counter = status + ntiles;
-        return launch_products(pl, square, 1, job, nullptr, gj, nullptr, out_len,
+        return launch_products(pl, square, 1, job, nullptr, 1, gj, nullptr, out_len,
                                (double)la + (square ? 0.0 : (double)lb), (double)out_len, status,
                                ntiles + 1, st);
     }
@@ -332,11 +332,36 @@ public:
     uint8_t* desc = nullptr;
     size_t desc_bytes = 0;
 
+    // Plan of one product: transform length and optional chunking of the longer operand.
+    struct PairPlan {
+        uint32_t lg;         // log2 of the transform length
+        uint32_t log_chunk;  // 0: ordinary product; else chunk length 2^log_chunk, lg = log_chunk+1
+    };
+    // Default policy: cut the longer operand into 2^c-limb chunks (transform 2^(c+1)) when the
+    // operands are unbalanced enough that the chunk transforms are at least 4x shorter.
+    PairPlan default_plan(uint32_t la, uint32_t lb, bool sq) const {
+        const int lg = bi_hip_transform_log2(la, lb);
+        PairPlan p{(uint32_t)lg, 0};
+        if (sq || lg < 14) return p;
+        const uint32_t lo = std::min(la, lb);
+        uint32_t c = 1;
+        while ((1u << c) < lo) ++c;  // 2^c >= min(la, lb) >= lb - 1
+        if (c < 11) c = 11;
+        if (c + 1 + 2 <= (uint32_t)lg) p = PairPlan{c + 1, c};
+        return p;
+    }
+
+    int plan_mode = BI_HIP_PLAN_TABLE;
+    // external per-pair plan override (set by the planner), consulted by run_batch
+    std::vector<PairPlan> plan_override;
+
     // Batched products on device pointers, grouped by transform length and squaring.
     int run_batch(const bi_hip_dpair* pairs, size_t count, hipStream_t st) {
         struct Item {
-            uint32_t idx, la, lb, lg;
-            bool sq;
+            uint32_t idx, la, lb;
+            bool sq, swap;
+            PairPlan pp;
+            uint32_t nchunks;
         };
         std::map<std::pair<uint32_t, bool>, std::vector<Item>> groups;
         for (size_t i = 0; i < count; ++i) {
@@ -348,16 +373,31 @@ public:
             if (p.out_cap < p.la + lb) return BI_HIP_ECAP;
             const int lg = bi_hip_transform_log2(p.la, lb);
             if (lg < 0) return BI_HIP_ETOOBIG;
-            groups[{(uint32_t)lg, sq}].push_back(Item{(uint32_t)i, (uint32_t)p.la, (uint32_t)lb, (uint32_t)lg, sq});
+            Item it{(uint32_t)i, (uint32_t)p.la, (uint32_t)lb, sq, false, PairPlan{(uint32_t)lg, 0}, 1};
+            PairPlan pp = i < plan_override.size() ? plan_override[i] : default_plan(it.la, it.lb, sq);
+            if (pp.log_chunk && !sq) {
+                it.swap = it.la < it.lb;  // chunk the longer operand
+                const uint32_t la_ = std::max(it.la, it.lb), lb_ = std::min(it.la, it.lb);
+                const uint32_t L = 1u << pp.log_chunk;
+                if (pp.lg == pp.log_chunk + 1 && lb_ <= L + 1 && pp.lg <= 26) {
+                    it.pp = pp;
+                    it.nchunks = (la_ + L - 1) / L;
+                }
+            } else if (pp.lg >= (uint32_t)lg && pp.lg <= 26) {
+                it.pp = PairPlan{pp.lg, 0};
+            }
+            groups[{it.pp.lg, sq}].push_back(it);
         }
-        // sizes: workspace per group (residues + status), descriptors for all groups
         size_t ws_need = 0, desc_need = 0;
         for (auto& [key, items] : groups) {
             const size_t n = (size_t)1 << key.first;
-            size_t words = 0;
-            for (auto& it : items) words += 3 * n * (it.sq ? 1 : 2) + (it.la + it.lb + 4095u) / 4096u + 1;
+            size_t words = 0, njobs = 0;
+            for (auto& it : items) {
+                words += 3 * n * (it.sq ? 1 : 2) * it.nchunks + (it.la + it.lb + 4095u) / 4096u + 1;
+                njobs += it.nchunks;
+            }
             ws_need = std::max(ws_need, words + 64);
-            desc_need += items.size() * (sizeof(PassJob) + sizeof(GarnerJob)) + 256;
+            desc_need += njobs * sizeof(PassJob) + items.size() * sizeof(GarnerJob) + 512;
         }
         uint32_t* w = workspace(ws_need);
         if (!w) return BI_HIP_ENOMEM;
@@ -372,10 +412,11 @@ public:
         struct Launch {
             Plan* pl;
             bool sq;
-            uint32_t njobs, max_out;
+            uint32_t njobs, ngjobs, max_out;
             size_t pj_off, gj_off;
             double in_limbs, out_limbs;
             uint32_t status_words;
+            uint32_t* status;
         };
         std::vector<Launch> launches;
         size_t off = 0;
@@ -383,36 +424,55 @@ public:
             Plan* pl = plan(key.first);
             if (!pl) return BI_HIP_ENOMEM;
             const size_t n = (size_t)1 << key.first;
-            Launch L{pl, key.second, (uint32_t)items.size(), 0, 0, 0, 0.0, 0.0, 0};
+            Launch L{pl, key.second, 0, (uint32_t)items.size(), 0, 0, 0, 0.0, 0.0, 0, nullptr};
+            uint32_t njobs = 0;
+            for (auto& it : items) njobs += it.nchunks;
+            L.njobs = njobs;
             L.pj_off = off;
-            off += items.size() * sizeof(PassJob);
+            off += njobs * sizeof(PassJob);
             off = (off + 255) & ~(size_t)255;
             L.gj_off = off;
             off += items.size() * sizeof(GarnerJob);
             off = (off + 255) & ~(size_t)255;
             auto* pj = reinterpret_cast<PassJob*>(hdesc.data() + L.pj_off);
             auto* gj = reinterpret_cast<GarnerJob*>(hdesc.data() + L.gj_off);
-            // residues first, then every job's status words (one contiguous fill)
             size_t res_words = 0;
-            for (auto& it : items) res_words += 3 * n * (it.sq ? 1 : 2);
+            for (auto& it : items) res_words += 3 * n * (it.sq ? 1 : 2) * it.nchunks;
             uint32_t* res = w;
             uint32_t* status = w + res_words;
             uint32_t* st_cur = status;
+            size_t jp = 0;
             for (size_t j = 0; j < items.size(); ++j) {
                 const Item& it = items[j];
                 const bi_hip_dpair& p = pairs[it.idx];
-                uint32_t* ra = res;
-                res += 3 * n;
-                uint32_t* rb = nullptr;
-                if (!it.sq) {
-                    rb = res;
+                const uint32_t* A = it.swap ? p.b : p.a;
+                const uint32_t* B = it.sq ? p.a : (it.swap ? p.a : p.b);
+                const uint32_t la = it.swap ? it.lb : it.la, lb = it.swap ? it.la : it.lb;
+                GarnerJob g{};
+                g.y = res;
+                if (it.nchunks > 1 || it.pp.log_chunk) {
+                    const uint32_t Lc = 1u << it.pp.log_chunk;
+                    for (uint32_t c = 0; c < it.nchunks; ++c) {
+                        const uint32_t lac = std::min(Lc, la - c * Lc);
+                        pj[jp++] = PassJob{A + (size_t)c * Lc, B, lac, lb, res, res + 3 * n};
+                        res += 6 * n;
+                    }
+                    g.nchunks = it.nchunks;
+                    g.log_chunk = it.pp.log_chunk;
+                    g.chunk_stride = 6 * n;
+                } else {
+                    uint32_t* ra = res;
                     res += 3 * n;
+                    uint32_t* rb = nullptr;
+                    if (!it.sq) {
+                        rb = res;
+                        res += 3 * n;
+                    }
+                    pj[jp++] = PassJob{A, B, la, lb, ra, rb};
+                    g.nchunks = 1;
                 }
-                pj[j] = PassJob{p.a, it.sq ? p.a : p.b, it.la, it.lb, ra, rb};
                 const uint32_t out_len = it.la + it.lb;
                 const uint32_t nt = (out_len + 4095u) / 4096u;
-                GarnerJob g{};
-                g.y = ra;
                 g.out = p.out;
                 g.n = (uint32_t)n;
                 g.out_len = out_len;
@@ -426,9 +486,9 @@ public:
                 L.out_limbs += (double)out_len;
             }
             L.status_words = (uint32_t)(st_cur - status);
+            L.status = status;
             launches.push_back(L);
         }
-        // the descriptor buffer may still be read by an earlier batch on this stream
         if (hipStreamSynchronize(st) != hipSuccess) return BI_HIP_EDEVICE;
         if (hipMemcpy(desc, hdesc.data(), off, hipMemcpyHostToDevice) != hipSuccess)
             return BI_HIP_EDEVICE;
@@ -437,11 +497,9 @@ public:
             const auto* gj = reinterpret_cast<const GarnerJob*>(desc + L.gj_off);
             const auto* hp = reinterpret_cast<const PassJob*>(hdesc.data() + L.pj_off);
             const auto* hg = reinterpret_cast<const GarnerJob*>(hdesc.data() + L.gj_off);
-            uint32_t* status = hg[0].status;
-            const bool one = L.njobs == 1;
-            const int rc = launch_products(L.pl, L.sq, L.njobs, hp[0], one ? nullptr : pj, hg[0],
-                                           one ? nullptr : gj, L.max_out, L.in_limbs, L.out_limbs,
-                                           status, L.status_words, st);
+            const int rc = launch_products(L.pl, L.sq, L.njobs, hp[0], L.njobs == 1 ? nullptr : pj,
+                                           L.ngjobs, hg[0], L.ngjobs == 1 ? nullptr : gj, L.max_out,
+                                           L.in_limbs, L.out_limbs, L.status, L.status_words, st);
             if (rc) return rc;
         }
         return BI_HIP_OK;

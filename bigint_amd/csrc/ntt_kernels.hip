@@ -334,18 +334,38 @@ struct Pieces {
     uint32_t d0, d1, d2;
 };
 
-__device__ __forceinline__ Pieces garner(const GarnerJob& jb, const GarnerConsts& k, uint32_t j) {
-    if (j >= jb.n) return {0u, 0u, 0u};
-    const uint32_t y0 = jb.y[j], y1 = jb.y[jb.n + j], y2 = jb.y[2u * jb.n + j];
-    const uint32_t c0 = mont_mul<0>(y0, jb.kscale[0]);
-    const uint32_t c1 = mont_mul<1>(y1, jb.kscale[1]);
-    const uint32_t c2 = mont_mul<2>(y2, jb.kscale[2]);
+// Coefficient of one product from its three residues (at plane offset j of the planes at y).
+__device__ __forceinline__ unsigned __int128 crt_coeff(const uint32_t* __restrict__ y, uint32_t n,
+                                                       const uint32_t (&ks)[3],
+                                                       const GarnerConsts& k, uint32_t j) {
+    const uint32_t c0 = mont_mul<0>(y[j], ks[0]);
+    const uint32_t c1 = mont_mul<1>(y[n + j], ks[1]);
+    const uint32_t c2 = mont_mul<2>(y[2u * n + j], ks[2]);
     const uint32_t v0 = c0;
     const uint32_t v1 = mont_mul<1>(sub_mod<1>(c1, v0), k.inv01);
     const uint32_t t = mont_mul<2>(sub_mod<2>(c2, v0), k.inv02);
     const uint32_t v2 = mont_mul<2>(sub_mod<2>(t, v1), k.inv12);
     const uint64_t inner = (uint64_t)v1 + (uint64_t)kP[1] * v2;  // < 2^62.1
-    const unsigned __int128 x = (unsigned __int128)inner * kP[0] + v0;
+    return (unsigned __int128)inner * kP[0] + v0;
+}
+
+// Convolution coefficient j of the job, split into 28-bit pieces.  For a chunked product the
+// coefficient is the sum of the (at most two) chunk products covering j; every such sum stays
+// below n * 2^56 <= 2^82.
+__device__ __forceinline__ Pieces garner(const GarnerJob& jb, const GarnerConsts& k, uint32_t j) {
+    unsigned __int128 x = 0;
+    if (jb.nchunks <= 1) {
+        if (j >= jb.n) return {0u, 0u, 0u};
+        x = crt_coeff(jb.y, jb.n, jb.kscale, k, j);
+    } else {
+        const uint32_t c = j >> jb.log_chunk;
+        for (uint32_t d = 0; d < 4 && d <= c; ++d) {
+            const uint32_t cc = c - d;
+            const uint32_t off = j - (cc << jb.log_chunk);
+            if (off >= jb.n) break;
+            if (cc < jb.nchunks) x += crt_coeff(jb.y + (size_t)cc * jb.chunk_stride, jb.n, jb.kscale, k, off);
+        }
+    }
     const uint64_t lo = (uint64_t)x;
     const uint64_t hi = (uint64_t)(x >> 64);
     Pieces p;

@@ -44,6 +44,12 @@ struct GarnerJob {
     uint32_t kscale[3];  // n^-1 * R^2 mod P_i, so mont_mul(y, kscale) = y * n^-1 * R
     uint32_t* status;    // look-back status words, one per 4096-limb tile, zeroed before launch
     uint32_t* counter;   // dynamic tile counter, zeroed before launch
+    // Chunked product: the long operand was cut into nchunks pieces of 2^log_chunk limbs, each
+    // multiplied with a transform of length n = 2^(log_chunk+1); chunk c's residues start at
+    // y + c * chunk_stride and its product lands at limb offset c * 2^log_chunk.  nchunks <= 1
+    // means an ordinary product.
+    uint32_t nchunks, log_chunk;
+    uint64_t chunk_stride;
 };
 
 struct GarnerArgs {

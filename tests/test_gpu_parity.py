@@ -164,3 +164,28 @@ def test_batch_grouped_matches_oracle(mode):
     outs = engine.mul_batch(pairs, mode)
     for (a, b), p in zip(pairs, outs):
         assert np.array_equal(p, oracle.mul_limbs(a, b))
+
+
+@pytest.mark.parametrize("a_bits,b_bits", [(28 * 20000, 28 * 300), (1 << 22, 1 << 14), (1 << 16, 1 << 21),
+                                           (28 * 4096 * 9 + 5, 28 * 2049)])
+def test_chunked_unbalanced_against_oracle(a_bits, b_bits):
+    """The grouped path cuts the longer operand into chunks; partial products overlap-add
+    before the carry scan."""
+    a = L.random_operand(a_bits, 71)
+    b = L.random_operand(b_bits, 72)
+    (p,) = engine.mul_batch([(a, b)], engine.PLAN_TABLE)
+    assert np.array_equal(p, oracle.mul_limbs(a, b))
+
+
+def test_chunked_all_ones_worst_case():
+    x, y = (1 << (28 * 50000)) - 1, (1 << (28 * 3000)) - 1
+    (p,) = engine.mul_batch([(L.int_to_limbs(x), L.int_to_limbs(y))], engine.PLAN_TABLE)
+    assert L.limbs_to_int(p) == x * y
+
+
+def test_chunked_2e28_by_2e20_residues():
+    a = L.random_operand(1 << 28, L.SEED)
+    b = L.random_operand(1 << 20, L.SEED + 1)
+    (p,) = engine.mul_batch([(a, b)], engine.PLAN_TABLE)
+    assert np.array_equal(p, engine.mul(a, b))
+    check_residues(a, b, p)
