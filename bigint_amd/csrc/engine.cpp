@@ -3,6 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <cmath>
+#include <tuple>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -12,6 +15,7 @@
 
 #include "../../include/bigint_hip.h"
 #include "ntt_plan.h"
+#include "planner.h"
 
 namespace bi {
 namespace {
@@ -175,45 +179,58 @@ public:
         return BI_HIP_OK;
     }
 
-    Plan* plan(uint32_t log_n) {
-        auto it = plans.find(log_n);
+    // Stage split for a transform of 2^log_n points whose contiguous block pass has 2^mid_lr
+    // points: the remaining outer stages go in one pass (<= 8 stages) or two.  Returns false when
+    // a pass would have more tile columns than its row stride allows.
+    static bool split_outer(uint32_t log_n, uint32_t mid_lr, std::vector<uint32_t>& rs) {
+        rs.clear();
+        if (log_n <= mid_lr) return true;
+        const uint32_t q = log_n - mid_lr;
+        if (q <= 8) rs = {q};
+        else if (q <= 16) rs = {(q + 1) / 2, q / 2};
+        else return false;
+        uint32_t done = 0;
+        for (uint32_t r : rs) {
+            const uint32_t log_s = log_n - done - r;
+            if (12 - r > log_s) return false;
+            done += r;
+        }
+        return true;
+    }
+
+    Plan* plan(uint32_t log_n, uint32_t mid_lr = 12) {
+        if (log_n <= mid_lr) mid_lr = log_n;
+        const uint32_t key = log_n * 32u + mid_lr;
+        auto it = plans.find(key);
         if (it != plans.end()) return it->second.get();
+        std::vector<uint32_t> rs;
+        if (mid_lr < 4 || mid_lr > 12 || !split_outer(log_n, mid_lr, rs)) return nullptr;
         auto pl = std::make_unique<Plan>();
         pl->log_n = log_n;
         const uint32_t n = 1u << log_n;
-        if (log_n <= 12) {
-            pl->mid_log_r = log_n;
-            pl->mid_nblk = 1;
-        } else {
-            const uint32_t q = log_n - 12;
-            std::vector<uint32_t> rs;
-            if (q <= 8) rs = {q};
-            else rs = {(q + 1) / 2, q / 2};
-            uint32_t done = 0;
-            for (uint32_t r : rs) {
-                OuterPass op;
-                op.log_r = r;
-                op.log_c = 12 - r;
-                op.log_s = log_n - done - r;
-                op.nblk = 1u << done;
-                if (upload_beta(log_n, r, op.log_s, op.nblk, &op.beta, &op.beta_inv) != hipSuccess)
-                    return nullptr;
-                pl->outer.push_back(op);
-                done += r;
-            }
-            pl->mid_log_r = 12;
-            pl->mid_nblk = n >> 12;
-            if (upload_beta(log_n, 12, 0, pl->mid_nblk, &pl->mid_beta, &pl->mid_beta_inv) !=
-                hipSuccess)
+        uint32_t done = 0;
+        for (uint32_t r : rs) {
+            OuterPass op;
+            op.log_r = r;
+            op.log_c = 12 - r;
+            op.log_s = log_n - done - r;
+            op.nblk = 1u << done;
+            if (upload_beta(log_n, r, op.log_s, op.nblk, &op.beta, &op.beta_inv) != hipSuccess)
                 return nullptr;
+            pl->outer.push_back(op);
+            done += r;
         }
+        pl->mid_log_r = mid_lr;
+        pl->mid_nblk = n >> mid_lr;
+        if (upload_beta(log_n, mid_lr, 0, pl->mid_nblk, &pl->mid_beta, &pl->mid_beta_inv) != hipSuccess)
+            return nullptr;
         for (int I = 0; I < 3; ++I) {
             const uint32_t p = kPh[I];
             const uint32_t r2 = mulmod(to_mont(1, p), to_mont(1, p), p);  // R^2 mod p
             pl->kscale[I] = mulmod(invmod(n % p, p), r2, p);
         }
         Plan* raw = pl.get();
-        plans[log_n] = std::move(pl);
+        plans[key] = std::move(pl);
         return raw;
     }
 
@@ -332,38 +349,182 @@ public:
     uint8_t* desc = nullptr;
     size_t desc_bytes = 0;
 
-    // Plan of one product: transform length and optional chunking of the longer operand.
+    // Plan of one product: transform length, optional chunking of the longer operand, and the
+    // size of the contiguous block pass (which fixes the radix split of the outer stages).
     struct PairPlan {
         uint32_t lg;         // log2 of the transform length
         uint32_t log_chunk;  // 0: ordinary product; else chunk length 2^log_chunk, lg = log_chunk+1
+        uint32_t mid_lr;     // log2 of the block-pass size
     };
-    // Default policy: cut the longer operand into 2^c-limb chunks (transform 2^(c+1)) when the
-    // operands are unbalanced enough that the chunk transforms are at least 4x shorter.
-    PairPlan default_plan(uint32_t la, uint32_t lb, bool sq) const {
-        const int lg = bi_hip_transform_log2(la, lb);
-        PairPlan p{(uint32_t)lg, 0};
-        if (sq || lg < 14) return p;
+    static constexpr int kNumPlanIds = 6;
+
+    // Plan ids: 0/1/2 = unchunked with 4096/2048/1024-point block pass; 3/4/5 = chunks of
+    // 2^c, 2^(c+1), 2^(c+2) limbs with 2^c the smallest power of two >= the shorter operand.
+    static bool plan_for_id(uint32_t la, uint32_t lb, bool sq, int id, PairPlan& out) {
+        const int lg0 = bi_hip_transform_log2(la, lb);
+        if (lg0 < 0 || id < 0 || id >= kNumPlanIds) return false;
+        std::vector<uint32_t> rs;
+        if (id <= 2) {
+            const uint32_t mid = 12u - (uint32_t)id;
+            if (id > 0 && (uint32_t)lg0 <= mid) return false;  // same as plan 0
+            if (!split_outer((uint32_t)lg0, mid, rs)) return false;
+            out = PairPlan{(uint32_t)lg0, 0, mid};
+            return true;
+        }
+        if (sq) return false;
         const uint32_t lo = std::min(la, lb);
-        uint32_t c = 1;
-        while ((1u << c) < lo) ++c;  // 2^c >= min(la, lb) >= lb - 1
-        if (c < 11) c = 11;
-        if (c + 1 + 2 <= (uint32_t)lg) p = PairPlan{c + 1, c};
+        uint32_t c = 10;
+        while ((1u << c) < lo) ++c;
+        c += (uint32_t)(id - 3);
+        if (c + 2 > (uint32_t)lg0 || c + 1 > 26) return false;  // chunk transform at most half
+        if (!split_outer(c + 1, 12, rs)) return false;
+        out = PairPlan{c + 1, c, 12};
+        return true;
+    }
+
+    // Heuristic used until a tuned table or model is installed: chunk when the chunk transforms
+    // are at least 4x shorter than the plain one.
+    static PairPlan heuristic_plan(uint32_t la, uint32_t lb, bool sq) {
+        PairPlan p{};
+        plan_for_id(la, lb, sq, 0, p);
+        PairPlan c{};
+        if (!sq && p.lg >= 14 && plan_for_id(la, lb, sq, 3, c) && c.lg + 2 <= p.lg) return c;
         return p;
     }
 
-    int plan_mode = BI_HIP_PLAN_TABLE;
-    // external per-pair plan override (set by the planner), consulted by run_batch
-    std::vector<PairPlan> plan_override;
+    // installed planner state
+    std::vector<int8_t> table;  // [19][19][13] plan ids by rounded (log2 bits a, log2 bits b, log2 batch)
+    int mlp_k = 0;
+    float mlp_norm[6] = {0, 0, 0, 1, 1, 1};
+    float* d_mlp = nullptr;
+    size_t d_mlp_words = 0;
+    float* d_scratch = nullptr;
+    size_t d_scratch_words = 0;
+
+    static int bucket(double v, int lo, int hi) {
+        int b = (int)std::lround(v);
+        return std::min(hi, std::max(lo, b)) - lo;
+    }
+    int table_plan(uint32_t la, uint32_t lb, uint32_t batch) const {
+        if (table.empty()) return -1;
+        const int ia = bucket(std::log2(28.0 * la), 10, 28), ib = bucket(std::log2(28.0 * lb), 10, 28);
+        const int ik = bucket(std::log2((double)std::max(1u, batch)), 0, 12);
+        return table[((size_t)ia * 19 + ib) * 13 + ik];
+    }
+
+    float* scratch(size_t words) {
+        if (words > d_scratch_words) {
+            if (d_scratch) hipFree(d_scratch);
+            d_scratch = nullptr;
+            d_scratch_words = 0;
+            if (hipMalloc((void**)&d_scratch, words * 4) != hipSuccess) return nullptr;
+            d_scratch_words = words;
+        }
+        return d_scratch;
+    }
+
+    // Learned choice for a list of shapes: MLP forward + masked argmin on the GPU.
+    int mlp_choose_ids(const std::vector<std::array<uint32_t, 3>>& shapes,
+                       const std::vector<uint8_t>& allowed, std::vector<int32_t>& ids,
+                       hipStream_t st) {
+        const int B = (int)shapes.size(), K = mlp_k;
+        ids.assign(B, 0);
+        if (B == 0 || K == 0 || !d_mlp) return BI_HIP_EINVAL;
+        const size_t words = (size_t)B * 3 + (size_t)B * K + ((size_t)B * K + 3) / 4 + 4 + (size_t)B;
+        float* w = scratch(words + 64);
+        if (!w) return BI_HIP_ENOMEM;
+        std::vector<float> x((size_t)B * 3);
+        for (int i = 0; i < B; ++i) {
+            const double f[3] = {std::log2(28.0 * shapes[i][0]), std::log2(28.0 * shapes[i][1]),
+                                 std::log2((double)std::max(1u, shapes[i][2]))};
+            for (int j = 0; j < 3; ++j) x[(size_t)i * 3 + j] = (float)((f[j] - mlp_norm[j]) / mlp_norm[3 + j]);
+        }
+        float* dx = w;
+        float* dy = dx + (size_t)B * 3;
+        uint8_t* dallow = reinterpret_cast<uint8_t*>(dy + (size_t)B * K);
+        int32_t* dchoice = reinterpret_cast<int32_t*>(dy + (size_t)B * K + ((size_t)B * K + 3) / 4 + 1);
+        if (hipMemcpyAsync(dx, x.data(), x.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(dallow, allowed.data(), allowed.size(), hipMemcpyHostToDevice, st) != hipSuccess)
+            return BI_HIP_EDEVICE;
+        if (mlp_forward(dx, B, d_mlp, K, dy, nullptr, nullptr, st) != hipSuccess) return BI_HIP_EDEVICE;
+        if (mlp_choose(dy, dallow, B, K, dchoice, st) != hipSuccess) return BI_HIP_EDEVICE;
+        if (hipMemcpyAsync(ids.data(), dchoice, (size_t)B * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+            return BI_HIP_EDEVICE;
+        return hipStreamSynchronize(st) == hipSuccess ? BI_HIP_OK : BI_HIP_EDEVICE;
+    }
+
+    // shapes (la, lb, batch) of the pairs; batch = number of pairs sharing the plain length
+    static std::vector<std::array<uint32_t, 3>> batch_shapes(const bi_hip_dpair* pairs, size_t count) {
+        std::map<int, uint32_t> per_lg;
+        std::vector<std::array<uint32_t, 3>> shapes(count);
+        for (size_t i = 0; i < count; ++i) {
+            const bi_hip_dpair& p = pairs[i];
+            const bool sq = p.b == nullptr || (p.b == p.a && p.lb == p.la);
+            shapes[i] = {(uint32_t)p.la, (uint32_t)(sq ? p.la : p.lb), 0};
+            per_lg[bi_hip_transform_log2(shapes[i][0], shapes[i][1])]++;
+        }
+        for (size_t i = 0; i < count; ++i) shapes[i][2] = per_lg[bi_hip_transform_log2(shapes[i][0], shapes[i][1])];
+        return shapes;
+    }
+
+    // Plan ids for shapes under a mode (-1 = heuristic).
+    int choose_ids(const std::vector<std::array<uint32_t, 3>>& shapes, const std::vector<bool>& sqv,
+                   int mode, std::vector<int32_t>& ids, hipStream_t st) {
+        const size_t count = shapes.size();
+        ids.assign(count, -1);
+        if (mode == BI_HIP_PLAN_LEARNED && mlp_k > 0) {
+            std::vector<uint8_t> allowed(count * (size_t)mlp_k, 0);
+            for (size_t i = 0; i < count; ++i) {
+                PairPlan tmp;
+                for (int k = 0; k < mlp_k && k < kNumPlanIds; ++k)
+                    allowed[i * mlp_k + k] = plan_for_id(shapes[i][0], shapes[i][1], sqv[i], k, tmp) ? 1 : 0;
+            }
+            return mlp_choose_ids(shapes, allowed, ids, st);
+        }
+        if (mode == BI_HIP_PLAN_TABLE || mode == BI_HIP_PLAN_LEARNED)
+            for (size_t i = 0; i < count; ++i) ids[i] = table_plan(shapes[i][0], shapes[i][1], shapes[i][2]);
+        return BI_HIP_OK;
+    }
+
+    // Per-pair plans for a batch under a plan mode; explicit ids win when given.
+    int choose_plans(const bi_hip_dpair* pairs, size_t count, int mode, const int32_t* explicit_ids,
+                     std::vector<PairPlan>& out, hipStream_t st) {
+        out.resize(count);
+        const auto shapes = batch_shapes(pairs, count);
+        std::vector<bool> sqv(count);
+        for (size_t i = 0; i < count; ++i)
+            sqv[i] = pairs[i].b == nullptr || (pairs[i].b == pairs[i].a && pairs[i].lb == pairs[i].la);
+        std::vector<int32_t> ids;
+        if (explicit_ids) {
+            ids.assign(explicit_ids, explicit_ids + count);
+        } else {
+            const int rc = choose_ids(shapes, sqv, mode, ids, st);
+            if (rc) return rc;
+        }
+        for (size_t i = 0; i < count; ++i) {
+            PairPlan pp{};
+            if (ids[i] < 0 || !plan_for_id(shapes[i][0], shapes[i][1], sqv[i], ids[i], pp))
+                pp = heuristic_plan(shapes[i][0], shapes[i][1], sqv[i]);
+            out[i] = pp;
+        }
+        return BI_HIP_OK;
+    }
 
     // Batched products on device pointers, grouped by transform length and squaring.
-    int run_batch(const bi_hip_dpair* pairs, size_t count, hipStream_t st) {
+    int run_batch(const bi_hip_dpair* pairs, size_t count, int mode, const int32_t* explicit_ids,
+                  hipStream_t st) {
+        std::vector<PairPlan> chosen;
+        {
+            const int rc = choose_plans(pairs, count, mode, explicit_ids, chosen, st);
+            if (rc) return rc;
+        }
         struct Item {
             uint32_t idx, la, lb;
             bool sq, swap;
             PairPlan pp;
             uint32_t nchunks;
         };
-        std::map<std::pair<uint32_t, bool>, std::vector<Item>> groups;
+        std::map<std::tuple<uint32_t, uint32_t, bool>, std::vector<Item>> groups;
         for (size_t i = 0; i < count; ++i) {
             const bi_hip_dpair& p = pairs[i];
             const bool sq = p.b == nullptr || (p.b == p.a && p.lb == p.la);
@@ -373,24 +534,18 @@ public:
             if (p.out_cap < p.la + lb) return BI_HIP_ECAP;
             const int lg = bi_hip_transform_log2(p.la, lb);
             if (lg < 0) return BI_HIP_ETOOBIG;
-            Item it{(uint32_t)i, (uint32_t)p.la, (uint32_t)lb, sq, false, PairPlan{(uint32_t)lg, 0}, 1};
-            PairPlan pp = i < plan_override.size() ? plan_override[i] : default_plan(it.la, it.lb, sq);
-            if (pp.log_chunk && !sq) {
+            Item it{(uint32_t)i, (uint32_t)p.la, (uint32_t)lb, sq, false, chosen[i], 1};
+            if (it.pp.log_chunk) {
                 it.swap = it.la < it.lb;  // chunk the longer operand
-                const uint32_t la_ = std::max(it.la, it.lb), lb_ = std::min(it.la, it.lb);
-                const uint32_t L = 1u << pp.log_chunk;
-                if (pp.lg == pp.log_chunk + 1 && lb_ <= L + 1 && pp.lg <= 26) {
-                    it.pp = pp;
-                    it.nchunks = (la_ + L - 1) / L;
-                }
-            } else if (pp.lg >= (uint32_t)lg && pp.lg <= 26) {
-                it.pp = PairPlan{pp.lg, 0};
+                const uint32_t la_ = std::max(it.la, it.lb);
+                const uint32_t L = 1u << it.pp.log_chunk;
+                it.nchunks = (la_ + L - 1) / L;
             }
-            groups[{it.pp.lg, sq}].push_back(it);
+            groups[{it.pp.lg, it.pp.mid_lr, sq}].push_back(it);
         }
         size_t ws_need = 0, desc_need = 0;
         for (auto& [key, items] : groups) {
-            const size_t n = (size_t)1 << key.first;
+            const size_t n = (size_t)1 << std::get<0>(key);
             size_t words = 0, njobs = 0;
             for (auto& it : items) {
                 words += 3 * n * (it.sq ? 1 : 2) * it.nchunks + (it.la + it.lb + 4095u) / 4096u + 1;
@@ -421,10 +576,10 @@ public:
         std::vector<Launch> launches;
         size_t off = 0;
         for (auto& [key, items] : groups) {
-            Plan* pl = plan(key.first);
+            Plan* pl = plan(std::get<0>(key), std::get<1>(key));
             if (!pl) return BI_HIP_ENOMEM;
-            const size_t n = (size_t)1 << key.first;
-            Launch L{pl, key.second, 0, (uint32_t)items.size(), 0, 0, 0, 0.0, 0.0, 0, nullptr};
+            const size_t n = (size_t)1 << std::get<0>(key);
+            Launch L{pl, std::get<2>(key), 0, (uint32_t)items.size(), 0, 0, 0, 0.0, 0.0, 0, nullptr};
             uint32_t njobs = 0;
             for (auto& it : items) njobs += it.nchunks;
             L.njobs = njobs;
@@ -615,7 +770,7 @@ int bi_hip_mul_batch_device(const bi_hip_dpair* pairs, size_t count, int plan_mo
         }
         return BI_HIP_OK;
     }
-    return e.run_batch(pairs, count, st);
+    return e.run_batch(pairs, count, plan_mode, nullptr, st);
 }
 
 int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode) {
@@ -698,6 +853,121 @@ int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode) {
         p.out_len = n;
     }
     return rc;
+}
+
+int bi_hip_plan_count(void) { return bi::Engine::kNumPlanIds; }
+
+int bi_hip_plan_allowed(size_t la, size_t lb, int is_square, int plan_id) {
+    if (la == 0 || lb == 0 || la > 0xFFFFFFFFull || lb > 0xFFFFFFFFull) return 0;
+    bi::Engine::PairPlan p{};
+    return bi::Engine::plan_for_id((uint32_t)la, (uint32_t)lb, is_square != 0, plan_id, p) ? 1 : 0;
+}
+
+int bi_hip_mul_batch_device_planned(const bi_hip_dpair* pairs, size_t count,
+                                    const int32_t* plan_ids, void* stream) {
+    if ((!pairs || !plan_ids) && count) return BI_HIP_EINVAL;
+    if (count == 0) return BI_HIP_OK;
+    std::lock_guard<std::mutex> lk(engine().mtx);
+    bi::Engine& e = engine();
+    int rc = e.init(0);
+    if (rc) return rc;
+    return e.run_batch(pairs, count, BI_HIP_PLAN_TABLE, plan_ids, stream ? (hipStream_t)stream : e.stream);
+}
+
+int bi_hip_choose_plans(const uint32_t* shapes, const int* is_square, size_t count, int mode,
+                        int32_t* ids) {
+    if ((!shapes || !ids) && count) return BI_HIP_EINVAL;
+    std::lock_guard<std::mutex> lk(engine().mtx);
+    bi::Engine& e = engine();
+    int rc = e.init(0);
+    if (rc) return rc;
+    std::vector<std::array<uint32_t, 3>> sh(count);
+    std::vector<bool> sqv(count);
+    for (size_t i = 0; i < count; ++i) {
+        sh[i] = {shapes[3 * i], shapes[3 * i + 1], shapes[3 * i + 2]};
+        sqv[i] = is_square && is_square[i];
+    }
+    std::vector<int32_t> out;
+    rc = e.choose_ids(sh, sqv, mode, out, e.stream);
+    if (rc) return rc;
+    for (size_t i = 0; i < count; ++i) {
+        bi::Engine::PairPlan p{};
+        if (out[i] < 0 || !bi::Engine::plan_for_id(sh[i][0], sh[i][1], sqv[i], out[i], p)) {
+            // report which id the heuristic corresponds to
+            const bi::Engine::PairPlan h = bi::Engine::heuristic_plan(sh[i][0], sh[i][1], sqv[i]);
+            out[i] = h.log_chunk ? 3 : 0;
+        }
+        ids[i] = out[i];
+    }
+    return BI_HIP_OK;
+}
+
+int bi_hip_set_planner(const int8_t* table, const float* mlp_params, int K, const float* norm6) {
+    std::lock_guard<std::mutex> lk(engine().mtx);
+    bi::Engine& e = engine();
+    int rc = e.init(0);
+    if (rc) return rc;
+    if (table) e.table.assign(table, table + 19 * 19 * 13);
+    if (mlp_params) {
+        if (K <= 0 || K > bi::kMaxPlans || !norm6) return BI_HIP_EINVAL;
+        const size_t np = (size_t)bi::mlp_param_count(K);
+        if (np > e.d_mlp_words) {
+            if (e.d_mlp) hipFree(e.d_mlp);
+            e.d_mlp = nullptr;
+            e.d_mlp_words = 0;
+            if (hipMalloc((void**)&e.d_mlp, np * 4) != hipSuccess) return BI_HIP_ENOMEM;
+            e.d_mlp_words = np;
+        }
+        if (hipMemcpy(e.d_mlp, mlp_params, np * 4, hipMemcpyHostToDevice) != hipSuccess) return BI_HIP_EDEVICE;
+        e.mlp_k = K;
+        for (int i = 0; i < 6; ++i) e.mlp_norm[i] = norm6[i];
+    }
+    return BI_HIP_OK;
+}
+
+int bi_hip_mlp_param_count(int K) { return (K > 0 && K <= bi::kMaxPlans) ? bi::mlp_param_count(K) : -1; }
+
+int bi_hip_mlp_forward(const float* d_x, int B, const float* d_params, int K, float* d_y, void* stream) {
+    if (K <= 0 || K > bi::kMaxPlans || B < 0 || !d_x || !d_params || !d_y) return BI_HIP_EINVAL;
+    return bi::mlp_forward(d_x, B, d_params, K, d_y, nullptr, nullptr, (hipStream_t)stream) == hipSuccess
+               ? BI_HIP_OK : BI_HIP_EDEVICE;
+}
+
+int bi_hip_mlp_grad(const float* d_x, const float* d_t, const float* d_mask, int B,
+                    const float* d_params, int K, float* d_grad, void* stream) {
+    if (K <= 0 || K > bi::kMaxPlans || B <= 0 || !d_x || !d_t || !d_mask || !d_params || !d_grad)
+        return BI_HIP_EINVAL;
+    std::lock_guard<std::mutex> lk(engine().mtx);
+    bi::Engine& e = engine();
+    int rc = e.init(0);
+    if (rc) return rc;
+    const int NP = bi::mlp_param_count(K);
+    const size_t nb = ((size_t)B + 63) / 64;
+    const size_t words = (size_t)B * K + 2 * (size_t)B * 64 + nb * NP + nb + 64;
+    float* w = e.scratch(words);
+    if (!w) return BI_HIP_ENOMEM;
+    float* y = w;
+    float* h1 = y + (size_t)B * K;
+    float* h2 = h1 + (size_t)B * 64;
+    float* part = h2 + (size_t)B * 64;
+    float* part_loss = part + nb * NP;
+    // count of measured entries, for the mean: computed on the host from the mask
+    std::vector<float> hm((size_t)B * K);
+    if (hipMemcpy(hm.data(), d_mask, hm.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return BI_HIP_EDEVICE;
+    double cnt = 0;
+    for (float m : hm) cnt += m;
+    const float inv = cnt > 0 ? (float)(1.0 / cnt) : 0.f;
+    hipStream_t st = (hipStream_t)stream;
+    if (bi::mlp_grad(d_x, d_t, d_mask, B, d_params, K, inv, y, h1, h2, part, part_loss, d_grad, st) != hipSuccess)
+        return BI_HIP_EDEVICE;
+    return hipStreamSynchronize(st) == hipSuccess ? BI_HIP_OK : BI_HIP_EDEVICE;
+}
+
+int bi_hip_mlp_adam(float* d_params, const float* d_grad, float* d_m, float* d_v, int nparams,
+                    float lr, float beta1, float beta2, float eps, int step, void* stream) {
+    if (!d_params || !d_grad || !d_m || !d_v || nparams <= 0 || step <= 0) return BI_HIP_EINVAL;
+    return bi::mlp_adam(d_params, d_grad, d_m, d_v, nparams, lr, beta1, beta2, eps, step,
+                        (hipStream_t)stream) == hipSuccess ? BI_HIP_OK : BI_HIP_EDEVICE;
 }
 
 int bi_hip_profile(int enable) {

@@ -1,0 +1,24 @@
+// Host-side entry points of the plan-selection MLP kernels (planner_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bi {
+
+constexpr int kMaxPlans = 8;
+
+int mlp_param_count(int K);
+hipError_t mlp_forward(const float* x, int B, const float* p, int K, float* y, float* h1,
+                       float* h2, hipStream_t st);
+// Forward + backward of the masked MSE.  grad holds NP + 1 floats: the gradient, then the loss.
+// y, h1, h2 are B*K, B*64, B*64 scratch; part / part_loss are ceil(B/64)*NP and ceil(B/64).
+hipError_t mlp_grad(const float* x, const float* t, const float* mask, int B, const float* p,
+                    int K, float inv_count, float* y, float* h1, float* h2, float* part,
+                    float* part_loss, float* grad, hipStream_t st);
+hipError_t mlp_adam(float* p, const float* g, float* m, float* v, int NP, float lr, float b1,
+                    float b2, float eps, int step, hipStream_t st);
+hipError_t mlp_choose(const float* y, const uint8_t* allowed, int B, int K, int32_t* choice,
+                      hipStream_t st);
+
+}  // namespace bi

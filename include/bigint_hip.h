@@ -127,6 +127,37 @@ int bi_hip_profile(int enable);
 /* Waits for the recorded launches and copies BI_HIP_K_COUNT entries into stats. */
 int bi_hip_profile_read(bi_hip_kstat* stats, size_t cap);
 
+/* ---- plan selection ------------------------------------------------------------------------
+ * A plan fixes, for one product: the transform length, chunking of the longer operand, and the
+ * block-pass size (hence the radix split of the strided stages).  Plan ids:
+ *   0, 1, 2  unchunked, 4096 / 2048 / 1024-point block pass
+ *   3, 4, 5  longer operand cut into 2^c, 2^(c+1), 2^(c+2)-limb chunks (2^c = smallest power of two
+ *            >= the shorter operand), each chunk product on a transform of twice the chunk length
+ * These replace the reference's hand-set size thresholds (include/bigint/mul.h:80, 153, 192). */
+int bi_hip_plan_count(void);
+/* 1 if plan_id is valid for an la x lb product (is_square: b is a), else 0. */
+int bi_hip_plan_allowed(size_t la, size_t lb, int is_square, int plan_id);
+/* Device batch with an explicit plan id per pair (-1 = the engine's heuristic). */
+int bi_hip_mul_batch_device_planned(const bi_hip_dpair* pairs, size_t count,
+                                    const int32_t* plan_ids, void* stream);
+/* Plan ids the engine would pick under `mode` for shapes[i] = {la, lb, batch}. */
+int bi_hip_choose_plans(const uint32_t* shapes, const int* is_square, size_t count, int mode,
+                        int32_t* ids);
+/* Install the autotuned table (int8 [19][19][13] plan ids indexed by round(log2 bits of a) - 10,
+ * round(log2 bits of b) - 10, round(log2 batch); NULL keeps the current one) and/or the MLP
+ * (fp32 parameters for K outputs, input normalisation {mean[3], std[3]}; NULL keeps it). */
+int bi_hip_set_planner(const int8_t* table, const float* mlp_params, int K, const float* norm6);
+
+/* MLP cost model kernels (training and inference on the GPU; device pointers). */
+int bi_hip_mlp_param_count(int K);
+int bi_hip_mlp_forward(const float* d_x, int B, const float* d_params, int K, float* d_y,
+                       void* stream);
+/* Masked-MSE loss and gradient; d_grad holds param_count + 1 floats (gradient, then loss). */
+int bi_hip_mlp_grad(const float* d_x, const float* d_t, const float* d_mask, int B,
+                    const float* d_params, int K, float* d_grad, void* stream);
+int bi_hip_mlp_adam(float* d_params, const float* d_grad, float* d_m, float* d_v, int nparams,
+                    float lr, float beta1, float beta2, float eps, int step, void* stream);
+
 /* Human-readable text for a status code. */
 const char* bi_hip_strerror(int code);
 
