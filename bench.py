@@ -77,6 +77,18 @@ def cpu_baseline(a: np.ndarray, b: np.ndarray, bits: int, max_s: float = 25.0) -
                       f"{info.get('cores_per_socket')} cores x {info.get('threads_per_core')} threads"}
 
 
+def copy_calibration(torch, dev) -> int:
+    """A plain 1 GiB device-to-device copy, three times: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
+    this command gives the counters' reading for a known byte count (1 GiB read + 1 GiB written)."""
+    src = torch.ones(1 << 28, dtype=torch.int32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": 1 << 30, "launches": 3}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +96,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--bits", type=int, default=28, help="log2 of the operand size in bits")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 1 GiB device copy (reference point for rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -102,12 +116,16 @@ def main() -> int:
     da = torch.from_numpy(a.view(np.int32)).to(dev)
     db = torch.from_numpy(b.view(np.int32)).to(dev)
     dout = torch.empty(a.size + b.size, dtype=torch.int32, device=dev)
-    stream = torch.cuda.current_stream(dev)
+    # a dedicated stream: the engine treats handle 0 as "its own stream", and torch's default
+    # stream has handle 0 on ROCm
+    stream = torch.cuda.Stream(dev)
 
     def step():
         engine.mul_device(da.data_ptr(), a.size, db.data_ptr(), b.size, False, dout.data_ptr(),
                           dout.numel(), stream.cuda_stream)
 
+    if args.copy_calibration:
+        return copy_calibration(torch, dev)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
