@@ -255,7 +255,7 @@ __global__ __launch_bounds__(kThreads) void k_outer_pass(PassArgs a) {
 // transform, store into A's plane.  grid.x = n >> LR, grid.y = job, grid.z = prime.
 template <int I, int LR>
 __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, uint32_t* lds,
-                                         uint32_t* twf, uint32_t* twi, uint32_t* bpow) {
+                                         uint32_t* tw, uint32_t* bpow) {
     using S = Sched<LR>;
     constexpr uint32_t T = 1u << (LR - 4);
     const uint32_t n = 1u << a.log_n;
@@ -268,8 +268,7 @@ __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, 
         bf = a.beta[I * a.beta_stride + blk];
         bi_ = a.beta_inv[I * a.beta_stride + blk];
     }
-    build_twiddles<I, LR>(twf, bpow, wf, bf, T);
-    build_twiddles<I, LR>(twi, bpow + 16, wi, bi_, T);
+    build_twiddles<I, LR>(tw, bpow, wf, bf, T);
 
     const bool from_limbs = (a.flags & kFromLimbs) != 0;
     const bool square = (a.flags & kSquare) != 0;
@@ -278,35 +277,38 @@ __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, 
     uint32_t xa[kE];
     global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pa, from_limbs ? job.limbs_a : nullptr,
                                                         job.la, xa, tile_base, 0);
-    __syncthreads();  // twiddles ready
-    tile_transform<I, false, LR, 0>(xa, lds, twf);
+    __syncthreads();  // forward twiddles ready
+    tile_transform<I, false, LR, 0>(xa, lds, tw);
     if (!square) {
         uint32_t* pb = job.res_b + (size_t)I * n;
         uint32_t xb[kE];
         global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pb, from_limbs ? job.limbs_b : nullptr,
                                                             job.lb, xb, tile_base, 0);
-        tile_transform<I, false, LR, 0>(xb, lds, twf);
+        tile_transform<I, false, LR, 0>(xb, lds, tw);
 #pragma unroll
         for (int k = 0; k < kE; ++k) xa[k] = mont_mul<I>(xa[k], xb[k]);
     } else {
 #pragma unroll
         for (int k = 0; k < kE; ++k) xa[k] = mont_mul<I>(xa[k], xa[k]);
     }
-    tile_transform<I, true, LR, 0>(xa, lds, twi);
+    // the same LDS table now receives the inverse twiddles
+    __syncthreads();
+    build_twiddles<I, LR>(tw, bpow, wi, bi_, T);
+    __syncthreads();
+    tile_transform<I, true, LR, 0>(xa, lds, tw);
     global_io<LR, 0, S::stages(G0), S::trlo(G0), true>(pa, nullptr, 0, xa, tile_base, 0);
 }
 
 template <int LR>
 __global__ __launch_bounds__(1 << (LR - 4)) void k_mid_pass(PassArgs a) {
     __shared__ uint32_t lds[(1 << LR) + (1 << LR) / 16];
-    __shared__ uint32_t twf[1 << LR];
-    __shared__ uint32_t twi[1 << LR];
-    __shared__ uint32_t bpow[32];
+    __shared__ uint32_t tw[1 << LR];
+    __shared__ uint32_t bpow[16];
     const PassJob& job = a.jobs ? a.jobs[blockIdx.y] : a.job;
     switch (blockIdx.z) {
-        case 0: mid_body<0, LR>(a, job, lds, twf, twi, bpow); break;
-        case 1: mid_body<1, LR>(a, job, lds, twf, twi, bpow); break;
-        default: mid_body<2, LR>(a, job, lds, twf, twi, bpow); break;
+        case 0: mid_body<0, LR>(a, job, lds, tw, bpow); break;
+        case 1: mid_body<1, LR>(a, job, lds, tw, bpow); break;
+        default: mid_body<2, LR>(a, job, lds, tw, bpow); break;
     }
 }
 
@@ -378,31 +380,69 @@ __device__ __forceinline__ Pieces garner(const GarnerJob& jb, const GarnerConsts
 constexpr uint32_t kCarryLimbsPerThread = 16;
 constexpr uint32_t kCarryTile = kThreads * kCarryLimbsPerThread;  // 4096 limbs per tile
 
+// Status words carry their whole payload (flag << 8 | map or carry), so they are published and
+// polled with relaxed agent-scope atomics: no acquire/release fence, hence no L2 write-back or
+// invalidation per poll (the L2 is private to each XCD).
+__device__ __forceinline__ uint32_t status_load(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void status_store(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+constexpr uint32_t kAgg = 1u, kInc = 2u;
+
 __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     const GarnerJob& jb = a.jobs ? a.jobs[blockIdx.y] : a.job;
+    // pieces of the tile's coefficients, padded one word per 16 against bank conflicts
+    __shared__ uint32_t s_d[3][kCarryTile + kCarryTile / 16];
+    __shared__ uint32_t s_halo[4];  // d1, d2 of coefficient base-1; d2 of base-2
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave[kThreads / 64];
     __shared__ uint32_t s_carry_in;
-    if (threadIdx.x == 0) s_tile = atomicAdd(jb.counter, 1u);
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) s_tile = atomicAdd(jb.counter, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint32_t ntiles = (jb.out_len + kCarryTile - 1u) / kCarryTile;
     if (tile >= ntiles) return;
+    const uint32_t base = tile * kCarryTile;
 
-    const uint32_t i0 = tile * kCarryTile + threadIdx.x * kCarryLimbsPerThread;
-    uint32_t s[kCarryLimbsPerThread];
-    {
-        Pieces pm2 = i0 >= 2 ? garner(jb, a.k, i0 - 2) : Pieces{0u, 0u, 0u};
-        Pieces pm1 = i0 >= 1 ? garner(jb, a.k, i0 - 1) : Pieces{0u, 0u, 0u};
-#pragma unroll
-        for (uint32_t q = 0; q < kCarryLimbsPerThread; ++q) {
-            const Pieces p = garner(jb, a.k, i0 + q);
-            s[q] = p.d0 + pm1.d1 + pm2.d2;
-            pm2 = pm1;
-            pm1 = p;
+    // 1. Garner for the tile's coefficients, coalesced: thread tid takes base + tid + 256 q
+#pragma unroll 4
+    for (uint32_t q = 0; q < kCarryLimbsPerThread; ++q) {
+        const uint32_t o = tid + kThreads * q;
+        const Pieces p = garner(jb, a.k, base + o);
+        const uint32_t li = o + (o >> 4);
+        s_d[0][li] = p.d0;
+        s_d[1][li] = p.d1;
+        s_d[2][li] = p.d2;
+    }
+    if (tid < 2) {  // the previous tile's last two coefficients feed limbs base and base+1
+        const Pieces p = base >= 2 - tid ? garner(jb, a.k, base - 2 + tid) : Pieces{0u, 0u, 0u};
+        if (tid == 1) {
+            s_halo[0] = p.d1;
+            s_halo[1] = p.d2;
+        } else {
+            s_halo[2] = p.d2;
         }
     }
-    // this thread's carry map
+    __syncthreads();
+
+    // 2. limb sums s_i = d0_i + d1_{i-1} + d2_{i-2}, 16 consecutive limbs per thread
+    uint32_t s[kCarryLimbsPerThread];
+    const uint32_t o0 = tid * kCarryLimbsPerThread;
+#pragma unroll
+    for (uint32_t q = 0; q < kCarryLimbsPerThread; ++q) {
+        const uint32_t o = o0 + q;
+        const uint32_t d0 = s_d[0][o + (o >> 4)];
+        uint32_t d1, d2;
+        if (o >= 1) d1 = s_d[1][(o - 1) + ((o - 1) >> 4)];
+        else d1 = s_halo[0];
+        if (o >= 2) d2 = s_d[2][(o - 2) + ((o - 2) >> 4)];
+        else d2 = (o == 1) ? s_halo[1] : s_halo[2];
+        s[q] = d0 + d1 + d2;
+    }
+    // 3. this thread's carry map, then an inclusive scan of maps across the block
     uint32_t m = kMapIdentity;
 #pragma unroll
     for (uint32_t q = 0; q < kCarryLimbsPerThread; ++q) {
@@ -411,8 +451,7 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
         for (uint32_t c = 0; c < 4; ++c) f |= ((s[q] + c) >> kDigitBits) << (2u * c);
         m = map_compose(f, m);
     }
-    // inclusive scan within the wave
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
     uint32_t inc = m;
 #pragma unroll
     for (uint32_t d = 1; d < 64; d <<= 1) {
@@ -421,49 +460,71 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     }
     if (lane == 63) s_wave[wave] = inc;
     __syncthreads();
-    uint32_t wave_prefix = kMapIdentity;  // composition of the waves before this one
+    uint32_t wave_prefix = kMapIdentity;
     for (uint32_t w = 0; w < wave; ++w) wave_prefix = map_compose(s_wave[w], wave_prefix);
     uint32_t excl = __shfl_up(inc, 1, 64);
     if (lane == 0) excl = kMapIdentity;
-    excl = map_compose(excl, wave_prefix);  // maps before this thread, within the tile
+    excl = map_compose(excl, wave_prefix);
 
-    if (threadIdx.x == 0) {
+    // 4. decoupled look-back by the first wave: 64 predecessor tiles are read at once
+    if (wave == 0) {
         uint32_t agg = kMapIdentity;
         for (uint32_t w = 0; w < kThreads / 64; ++w) agg = map_compose(s_wave[w], agg);
         uint32_t carry_in = 0;
         if (tile == 0) {
-            __hip_atomic_store(&jb.status[0], (2u << 8) | map_apply(agg, 0u), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) status_store(&jb.status[0], (kInc << 8) | map_apply(agg, 0u));
         } else {
-            __hip_atomic_store(&jb.status[tile], (1u << 8) | agg, __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            uint32_t acc = kMapIdentity;
-            int32_t j = (int32_t)tile - 1;
+            if (lane == 0) status_store(&jb.status[tile], (kAgg << 8) | agg);
+            uint32_t acc = kMapIdentity;  // composition of the nearer tiles already visited
+            int32_t top = (int32_t)tile - 1;
             while (true) {
-                uint32_t v;
-                do {
-                    v = __hip_atomic_load(&jb.status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                } while ((v >> 8) == 0u);
-                if ((v >> 8) == 2u) {
-                    carry_in = map_apply(acc, v & 0xFFu);
+                const int32_t j = top - (int32_t)lane;
+                uint32_t v = kInc << 8;  // before the first tile: inclusive carry 0
+                if (j >= 0) {
+                    do {
+                        v = status_load(&jb.status[j]);
+                    } while ((v >> 8) == 0u);
+                }
+                const uint64_t incs = __ballot((v >> 8) == kInc);
+                const uint32_t first = incs ? (uint32_t)__builtin_ctzll(incs) : 64u;
+                uint32_t mm = kMapIdentity;  // A_top o A_top-1 o ... over lanes [0, first)
+                for (uint32_t l = 0; l < 64; ++l) {
+                    const uint32_t x = __shfl(v, (int)l, 64);
+                    if (l < first) mm = map_compose(mm, x & 0xFFu);
+                }
+                if (first < 64) {
+                    const uint32_t cv = __shfl(v, (int)first, 64) & 0xFFu;
+                    carry_in = map_apply(acc, map_apply(mm, cv));
                     break;
                 }
-                acc = map_compose(acc, v & 0xFFu);
-                --j;
+                acc = map_compose(acc, mm);
+                top -= 64;
             }
-            __hip_atomic_store(&jb.status[tile], (2u << 8) | map_apply(agg, carry_in),
-                               __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) status_store(&jb.status[tile], (kInc << 8) | map_apply(agg, carry_in));
         }
-        s_carry_in = carry_in;
+        if (lane == 0) s_carry_in = carry_in;
     }
     __syncthreads();
+
+    // 5. apply the carries and write the limbs
     uint32_t carry = map_apply(excl, s_carry_in);
+    uint32_t outv[kCarryLimbsPerThread];
 #pragma unroll
     for (uint32_t q = 0; q < kCarryLimbsPerThread; ++q) {
         const uint32_t v = s[q] + carry;
-        const uint32_t i = i0 + q;
-        if (i < jb.out_len) jb.out[i] = v & kDigitMask;
+        outv[q] = v & kDigitMask;
         carry = v >> kDigitBits;
+    }
+    const uint32_t i0 = base + o0;
+    if (i0 + kCarryLimbsPerThread <= jb.out_len && ((reinterpret_cast<uintptr_t>(jb.out + i0) & 15u) == 0)) {
+        uint4* dst = reinterpret_cast<uint4*>(jb.out + i0);
+#pragma unroll
+        for (uint32_t q = 0; q < kCarryLimbsPerThread / 4; ++q)
+            dst[q] = make_uint4(outv[4 * q], outv[4 * q + 1], outv[4 * q + 2], outv[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (uint32_t q = 0; q < kCarryLimbsPerThread; ++q)
+            if (i0 + q < jb.out_len) jb.out[i0 + q] = outv[q];
     }
 }
 

@@ -18,7 +18,10 @@ PLAN_FIXED, PLAN_TABLE, PLAN_LEARNED = 0, 1, 2
 
 EXPORTS = ["bi_hip_init", "bi_hip_mul", "bi_hip_mul_device", "bi_hip_mul_batch",
            "bi_hip_synchronize", "bi_hip_transform_log2", "bi_hip_strerror", "bi_hip_profile",
-           "bi_hip_profile_read", "bi_hip_mul_batch_device"]
+           "bi_hip_profile_read", "bi_hip_mul_batch_device", "bi_hip_plan_count",
+           "bi_hip_plan_allowed", "bi_hip_mul_batch_device_planned", "bi_hip_choose_plans",
+           "bi_hip_set_planner", "bi_hip_mlp_param_count", "bi_hip_mlp_forward", "bi_hip_mlp_grad",
+           "bi_hip_mlp_adam"]
 KERNEL_NAMES = ["fill", "outer_fwd", "mid", "outer_inv", "garner_carry"]
 
 u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -68,6 +71,13 @@ def lib() -> ctypes.CDLL:
         L.bi_hip_transform_log2.argtypes = [ctypes.c_size_t, ctypes.c_size_t]
         L.bi_hip_profile.argtypes = [ctypes.c_int]
         L.bi_hip_profile_read.argtypes = [ctypes.POINTER(KStat), ctypes.c_size_t]
+        vp = ctypes.c_void_p
+        L.bi_hip_plan_allowed.argtypes = [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]
+        L.bi_hip_mlp_param_count.argtypes = [ctypes.c_int]
+        L.bi_hip_mlp_forward.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, vp, vp]
+        L.bi_hip_mlp_grad.argtypes = [vp, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp, vp]
+        L.bi_hip_mlp_adam.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_float, ctypes.c_float, ctypes.c_int, vp]
         L.bi_hip_strerror.restype = ctypes.c_char_p
         L.bi_hip_strerror.argtypes = [ctypes.c_int]
         _lib = L

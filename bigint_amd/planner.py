@@ -1,0 +1,270 @@
+"""Plan selection for the NTT multiply engine: timing collection, autotuned table, and the MLP
+cost model trained on the GPU with the engine's own HIP kernels (bi_hip_mlp_*).
+
+Plan ids (include/bigint_hip.h): 0/1/2 unchunked with a 4096/2048/1024-point block pass;
+3/4/5 longer operand cut into 2^c / 2^(c+1) / 2^(c+2)-limb chunks.  The reference picks its
+algorithm with fixed size thresholds (include/bigint/mul.h:80, 153, 192); here the choice is
+measured (table) or predicted (MLP), and every plan computes the same exact product.
+
+Features of a product: x = (log2 bits of a, log2 bits of b, log2 batch), where batch is the
+number of products of that transform length launched together.  The MLP predicts log2 of the
+per-product time for each plan; the chosen plan is the argmin over the plans allowed.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from . import engine
+
+K = 6  # plan ids
+TABLE_SHAPE = (19, 19, 13)
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_PATH = os.path.join(HERE, "plans", "planner.json")
+
+
+def allowed(la: int, lb: int, square: bool = False) -> list[int]:
+    return [k for k in range(K) if engine.lib().bi_hip_plan_allowed(la, lb, int(square), k)]
+
+
+def features(bits_a: float, bits_b: float, batch: float) -> list[float]:
+    return [math.log2(bits_a), math.log2(bits_b), math.log2(max(1.0, batch))]
+
+
+# ---- timing collection (GPU) -------------------------------------------------------------------
+
+class Timer:
+    """Times planned device batches of k identical-shape products on a dedicated stream."""
+
+    def __init__(self, max_limbs: int = 1 << 25, seed: int = 1):
+        import torch
+        self.torch = torch
+        self.dev = torch.device("cuda:0")
+        g = torch.Generator(device=self.dev)
+        g.manual_seed(seed)
+        self.pool = torch.randint(1, 1 << 28, (max_limbs,), dtype=torch.int32, device=self.dev,
+                                  generator=g)
+        self.out = torch.empty(2 * max_limbs + 64, dtype=torch.int32, device=self.dev)
+        self.max_limbs = max_limbs
+        self.stream = torch.cuda.Stream(self.dev)
+        engine.lib().bi_hip_mul_batch_device_planned.argtypes = [
+            ctypes.POINTER(engine.DPair), ctypes.c_size_t, ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p]
+
+    def run(self, la: int, lb: int, k: int, plan: int, reps: int = 2) -> float:
+        """Median per-product device time (us) of k products la x lb under `plan`."""
+        torch = self.torch
+        if k * (la + lb) > self.out.numel() or la > self.max_limbs or lb > self.max_limbs:
+            raise ValueError("shape too large for the timing pools")
+        base = self.pool.data_ptr()
+        obase = self.out.data_ptr()
+        pairs = []
+        for i in range(k):
+            offa = (i * 7919) % max(1, self.max_limbs - la)
+            offb = (i * 104729 + 17) % max(1, self.max_limbs - lb)
+            pairs.append(engine.DPair(base + 4 * offa, la, base + 4 * offb, lb,
+                                      obase + 4 * i * (la + lb), la + lb))
+        arr = (engine.DPair * k)(*pairs)
+        ids = (ctypes.c_int32 * k)(*([plan] * k))
+        L = engine.lib()
+        times = []
+        for r in range(reps + 1):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(self.stream)
+            rc = L.bi_hip_mul_batch_device_planned(arr, k, ids, self.stream.cuda_stream)
+            e1.record(self.stream)
+            if rc:
+                raise engine.EngineError(rc, "planned batch")
+            e1.synchronize()
+            if r:
+                times.append(e0.elapsed_time(e1) * 1e3 / k)
+        return float(np.median(times))
+
+
+def sample_shapes(count: int, seed: int, lo: float = 10.0, hi: float = 28.0, max_batch_log: int = 6,
+                  mem_limit_bytes: float = 24e9):
+    """Log-uniform operand sizes in [2^lo, 2^hi] bits and log-uniform batch sizes, with the batch
+    capped so the residues of one launch stay within mem_limit_bytes."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < count:
+        ba = float(2 ** rng.uniform(lo, hi))
+        bb = float(2 ** rng.uniform(lo, hi))
+        la, lb = max(1, int(ba) // 28 + 1), max(1, int(bb) // 28 + 1)
+        lg = engine.transform_log2(la, lb)
+        if lg < 0:
+            continue
+        k = int(2 ** rng.uniform(0, max_batch_log))
+        while k > 1 and k * 24 * (1 << lg) > mem_limit_bytes:
+            k //= 2
+        out.append((la, lb, k))
+    return out
+
+
+def collect(shapes, timer: Timer, log=None, reps: int = 2):
+    """Time every allowed plan for every shape; returns records."""
+    recs = []
+    t0 = time.time()
+    for i, (la, lb, k) in enumerate(shapes):
+        for p in allowed(la, lb):
+            us = timer.run(la, lb, k, p, reps)
+            recs.append({"la": la, "lb": lb, "k": k, "plan": p, "us": us})
+        if log and (i % 200 == 0):
+            log(f"collected {i}/{len(shapes)} shapes, {len(recs)} runs, {time.time() - t0:.0f}s")
+    return recs
+
+
+# ---- table -------------------------------------------------------------------------------------
+
+def group_best(recs):
+    """{(la, lb, k): {plan: us}} from records."""
+    d = {}
+    for r in recs:
+        d.setdefault((r["la"], r["lb"], r["k"]), {})[r["plan"]] = r["us"]
+    return d
+
+
+def build_table(recs) -> np.ndarray:
+    """Bucket by rounded log2 features; each cell takes the plan with the lowest mean log-time
+    ratio to the per-shape best among shapes in the cell; empty cells copy the nearest filled
+    cell."""
+    shapes = group_best(recs)
+    score = {}
+    for (la, lb, k), plans in shapes.items():
+        best = min(plans.values())
+        f = features(28 * la, 28 * lb, k)
+        cell = (min(18, max(0, round(f[0]) - 10)), min(18, max(0, round(f[1]) - 10)),
+                min(12, max(0, round(f[2]))))
+        for p, us in plans.items():
+            s = score.setdefault(cell, {})
+            s.setdefault(p, []).append(math.log2(us / best))
+        # plans not allowed for a shape in the cell are penalised
+        for p in range(K):
+            if p not in plans:
+                score[cell].setdefault(p, []).append(10.0)
+    table = np.full(TABLE_SHAPE, -1, dtype=np.int8)
+    for cell, s in score.items():
+        table[cell] = min(s, key=lambda p: np.mean(s[p]))
+    filled = np.argwhere(table >= 0)
+    if filled.size:
+        for idx in np.argwhere(table < 0):
+            d = np.abs(filled - idx).sum(axis=1)
+            table[tuple(idx)] = table[tuple(filled[int(np.argmin(d))])]
+    else:
+        table[:] = 0
+    return table
+
+
+# ---- MLP ----------------------------------------------------------------------------------------
+
+def dataset(recs):
+    shapes = group_best(recs)
+    X, T, M = [], [], []
+    for (la, lb, k), plans in shapes.items():
+        X.append(features(28 * la, 28 * lb, k))
+        t = np.zeros(K, np.float32)
+        m = np.zeros(K, np.float32)
+        for p, us in plans.items():
+            t[p] = math.log2(us)
+            m[p] = 1.0
+        T.append(t)
+        M.append(m)
+    return np.array(X, np.float32), np.array(T, np.float32), np.array(M, np.float32)
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    H = 64
+    parts = [rng.normal(0, math.sqrt(2 / 3), (3, H)), np.zeros(H),
+             rng.normal(0, math.sqrt(2 / H), (H, H)), np.zeros(H),
+             rng.normal(0, math.sqrt(1 / H), (H, K)), np.zeros(K)]
+    return np.concatenate([p.reshape(-1) for p in parts]).astype(np.float32)
+
+
+def train_gpu(X: np.ndarray, T: np.ndarray, M: np.ndarray, epochs: int = 3000, lr: float = 3e-3,
+              seed: int = 0, log=None):
+    """Full-batch Adam on the GPU through bi_hip_mlp_grad / bi_hip_mlp_adam.  Inputs are
+    normalised; returns (params, norm6, loss history)."""
+    import torch
+    dev = torch.device("cuda:0")
+    mean = X.mean(axis=0)
+    std = X.std(axis=0) + 1e-6
+    Xn = ((X - mean) / std).astype(np.float32)
+    L = engine.lib()
+    NP = L.bi_hip_mlp_param_count(K)
+    p = torch.from_numpy(init_params(seed)).to(dev)
+    assert p.numel() == NP
+    m = torch.zeros(NP, device=dev)
+    v = torch.zeros(NP, device=dev)
+    g = torch.zeros(NP + 1, device=dev)
+    x = torch.from_numpy(Xn).to(dev)
+    t = torch.from_numpy(T).to(dev)
+    mk = torch.from_numpy(M).to(dev)
+    st = torch.cuda.Stream(dev)
+    fp = ctypes.c_void_p
+    hist = []
+    for ep in range(1, epochs + 1):
+        rc = L.bi_hip_mlp_grad(fp(x.data_ptr()), fp(t.data_ptr()), fp(mk.data_ptr()), x.shape[0],
+                               fp(p.data_ptr()), K, fp(g.data_ptr()), fp(st.cuda_stream))
+        if rc:
+            raise engine.EngineError(rc, "bi_hip_mlp_grad")
+        rc = L.bi_hip_mlp_adam(fp(p.data_ptr()), fp(g.data_ptr()), fp(m.data_ptr()), fp(v.data_ptr()),
+                               NP, ctypes.c_float(lr), ctypes.c_float(0.9), ctypes.c_float(0.999),
+                               ctypes.c_float(1e-8), ep, fp(st.cuda_stream))
+        if rc:
+            raise engine.EngineError(rc, "bi_hip_mlp_adam")
+        if ep % 100 == 0 or ep == 1:
+            st.synchronize()
+            hist.append((ep, float(g[NP].item())))
+            if log and ep % 500 == 0:
+                log(f"epoch {ep} loss {hist[-1][1]:.5f}")
+    st.synchronize()
+    return p.cpu().numpy(), np.concatenate([mean, std]).astype(np.float32), hist
+
+
+def install(table: np.ndarray | None, params: np.ndarray | None, norm6: np.ndarray | None) -> None:
+    L = engine.lib()
+    L.bi_hip_set_planner.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    tb = np.ascontiguousarray(table, dtype=np.int8) if table is not None else None
+    pp = np.ascontiguousarray(params, dtype=np.float32) if params is not None else None
+    nn = np.ascontiguousarray(norm6, dtype=np.float32) if norm6 is not None else None
+    rc = L.bi_hip_set_planner(tb.ctypes.data if tb is not None else None,
+                              pp.ctypes.data if pp is not None else None, K,
+                              nn.ctypes.data if nn is not None else None)
+    if rc:
+        raise engine.EngineError(rc, "bi_hip_set_planner")
+
+
+def choose(shapes, mode: int) -> list[int]:
+    L = engine.lib()
+    L.bi_hip_choose_plans.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                      ctypes.c_void_p]
+    sh = np.array([[la, lb, k] for la, lb, k in shapes], dtype=np.uint32).reshape(-1)
+    ids = np.zeros(len(shapes), dtype=np.int32)
+    rc = L.bi_hip_choose_plans(sh.ctypes.data, None, len(shapes), mode, ids.ctypes.data)
+    if rc:
+        raise engine.EngineError(rc, "bi_hip_choose_plans")
+    return [int(i) for i in ids]
+
+
+def save(path: str, table: np.ndarray, params: np.ndarray, norm6: np.ndarray, meta: dict) -> None:
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump({"K": K, "table": table.reshape(-1).tolist(), "params": params.tolist(),
+                   "norm": norm6.tolist(), "meta": meta}, f)
+
+
+def load(path: str = DEFAULT_PATH, do_install: bool = True):
+    with open(path) as f:
+        d = json.load(f)
+    table = np.array(d["table"], np.int8).reshape(TABLE_SHAPE)
+    params = np.array(d["params"], np.float32)
+    norm = np.array(d["norm"], np.float32)
+    if do_install:
+        install(table, params, norm)
+    return table, params, norm, d.get("meta", {})

@@ -1,0 +1,123 @@
+"""Autotune the plan table and train the MLP cost model on this GPU, then measure regret.
+
+  python -m bigint_amd.tune --train 7000 --heldout 3500 --out bigint_amd/plans/planner.json \
+         --report gpurun_out/planner_report.json
+
+1. Draws training shapes (operand sizes log-uniform over [2^10, 2^28] bits, batch sizes
+   log-uniform in [1, 64]) and times every allowed plan of each (bigint_amd.planner.Timer:
+   planned device batches, HIP events on a dedicated stream).
+2. Builds the autotuned table from the training timings.
+3. Trains the MLP on the GPU (bi_hip_mlp_grad / bi_hip_mlp_adam).
+4. Times every allowed plan on held-out shapes and reports, for the plain plan (0), the table and
+   the MLP: mean regret (t_chosen / t_best - 1), geometric-mean speedup over plain, and the
+   total-time ratio plain / chosen.  Exactness is independent of the plan (tests/test_planner.py).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from bigint_amd import engine  # noqa: E402
+from bigint_amd import planner as P  # noqa: E402
+
+
+def log(msg: str) -> None:
+    print(f"[tune {time.strftime('%H:%M:%S')}] {msg}", flush=True)
+
+
+def evaluate(recs_h, table, params, norm):
+    """Regret of plain / table / MLP choices on held-out records."""
+    shapes = P.group_best(recs_h)
+    keys = list(shapes)
+    P.install(table, params, norm)
+    ch_table = P.choose([(la, lb, k) for la, lb, k in keys], engine.PLAN_TABLE)
+    ch_mlp = P.choose([(la, lb, k) for la, lb, k in keys], engine.PLAN_LEARNED)
+    out = {}
+    for name, ch in [("plain", [0] * len(keys)), ("table", ch_table), ("mlp", ch_mlp)]:
+        reg, sp, tot_c, tot_p, hit = [], [], 0.0, 0.0, 0
+        for key, c in zip(keys, ch):
+            plans = shapes[key]
+            best = min(plans.values())
+            t = plans.get(c, plans[0])
+            reg.append(t / best - 1)
+            sp.append(plans[0] / t)
+            tot_c += t * key[2]
+            tot_p += plans[0] * key[2]
+            hit += int(t == best)
+        out[name] = {"mean_regret": float(np.mean(reg)), "p95_regret": float(np.percentile(reg, 95)),
+                     "gmean_speedup_vs_plain": float(np.exp(np.mean(np.log(sp)))),
+                     "total_time_speedup_vs_plain": tot_p / tot_c, "best_plan_rate": hit / len(keys)}
+    oracle_tot = sum(min(v.values()) * k[2] for k, v in shapes.items())
+    out["best_possible_total_speedup_vs_plain"] = sum(v[0] * k[2] for k, v in shapes.items()) / oracle_tot
+    out["shapes"] = len(keys)
+    out["runs"] = len(recs_h)
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=7000, help="training shapes")
+    ap.add_argument("--heldout", type=int, default=3500, help="held-out shapes")
+    ap.add_argument("--epochs", type=int, default=4000)
+    ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--out", default=P.DEFAULT_PATH)
+    ap.add_argument("--report", default=None)
+    ap.add_argument("--records", default=None, help="save raw timing records (JSON lines)")
+    args = ap.parse_args()
+
+    engine.init(0)
+    timer = P.Timer()
+    max_out = timer.out.numel() - 64
+    def shapes(n, seed):
+        sh = P.sample_shapes(n, seed)
+        res = []
+        for la, lb, k in sh:
+            while k > 1 and k * (la + lb) > max_out:
+                k //= 2
+            if k * (la + lb) <= max_out and la <= timer.max_limbs and lb <= timer.max_limbs:
+                res.append((la, lb, k))
+        return res
+
+    t0 = time.time()
+    train_shapes = shapes(args.train, args.seed)
+    recs = P.collect(train_shapes, timer, log)
+    log(f"training timings: {len(recs)} runs over {len(train_shapes)} shapes in {time.time() - t0:.0f}s")
+    table = P.build_table(recs)
+    X, T, M = P.dataset(recs)
+    t1 = time.time()
+    params, norm, hist = P.train_gpu(X, T, M, epochs=args.epochs, seed=args.seed, log=log)
+    log(f"MLP trained on the GPU in {time.time() - t1:.1f}s, final loss {hist[-1][1]:.5f}")
+
+    held = shapes(args.heldout, args.seed + 1)
+    t2 = time.time()
+    recs_h = P.collect(held, timer, log)
+    log(f"held-out timings: {len(recs_h)} runs over {len(held)} shapes in {time.time() - t2:.0f}s")
+    ev = evaluate(recs_h, table, params, norm)
+    log(json.dumps(ev))
+    meta = {"train_runs": len(recs), "train_shapes": len(train_shapes), "heldout": ev,
+            "loss_history": hist, "epochs": args.epochs}
+    P.save(args.out, table, params, norm, meta)
+    if args.report:
+        with open(args.report, "w") as f:
+            json.dump({"evaluation": ev, "train_runs": len(recs), "loss_history": hist}, f, indent=1)
+    if args.records:
+        with open(args.records, "w") as f:
+            for r in recs:
+                f.write(json.dumps(dict(r, split="train")) + "\n")
+            for r in recs_h:
+                f.write(json.dumps(dict(r, split="heldout")) + "\n")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
