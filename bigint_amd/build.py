@@ -20,7 +20,7 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++20", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + os.path.join(ROOT, "include"), "-Wall"]
 SOURCES = ["ntt_kernels.hip", "planner_kernels.hip", "engine.cpp"]
-HOST_SOURCES = ["host/mul.cpp", "host/bigint.cpp"]
+HOST_SOURCES = ["host/mul.cpp", "host/mul_backend_hip.cpp", "host/bigint.cpp"]
 HOST_LIB = os.path.join(HERE, "libbigint.so")
 CXX = os.environ.get("BIGINT_CXX", "/opt/rocm/llvm/bin/clang++")
 HOST_FLAGS = ["-O2", "-std=gnu++20", "-fPIC", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"),

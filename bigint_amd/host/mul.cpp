@@ -7,6 +7,7 @@
 #include <new>
 
 #include "bigint_hip.h"
+#include "mul_backend.h"
 
 namespace bigint::mul {
 namespace {
@@ -71,12 +72,10 @@ auto mul_limbs(std::span<const uint32_t> a, std::span<const uint32_t> b, bool sq
     while (la > 1 && a[la - 1] == 0) --la;
     while (lb > 1 && b[lb - 1] == 0) --lb;
     Digits r(la + lb);
-    size_t len = 0;
-    const int rc = bi_hip_mul(a.data(), la, b.data(), lb, square ? 1 : 0, r.data(), r.size(), &len);
-    if (rc == BI_HIP_ENOMEM) throw std::bad_alloc();
-    if (rc != BI_HIP_OK) throw engine_error(rc, bi_hip_strerror(rc));
+    std::vector<backend::Job> jobs{backend::Job{a.first(la), b.first(lb), square, r.data(), 0}};
+    backend::multiply(jobs, BI_HIP_PLAN_TABLE);
     g_gpu_products.fetch_add(1, std::memory_order_relaxed);
-    r.resize(len);
+    r.resize(jobs[0].out_len);
     return r;
 }
 
@@ -108,6 +107,58 @@ auto mul_digits(const Digits& a, const Digits& b, size_type output_precision,
     } else {
         res.insert(res.begin(), tz_a + tz_b, 0u);
     }
+    return res;
+}
+
+}  // namespace bigint::mul
+
+namespace bigint::mul {
+
+auto mul_digits_batch(const std::vector<std::pair<const Digits*, const Digits*>>& pairs,
+                      int plan_mode) -> std::vector<Digits> {
+    std::vector<Digits> res(pairs.size());
+    struct Big {
+        size_t idx;
+        size_type za, zb;
+        std::span<const uint32_t> a, b;
+        bool same;
+    };
+    std::vector<Big> big;
+    for (size_t i = 0; i < pairs.size(); ++i) {
+        const Digits& a = *pairs[i].first;
+        const Digits& b = *pairs[i].second;
+        const bool same = &a == &b;
+        size_type za = 0, zb = 0;
+        while (a[za] == 0) ++za;
+        if (same) zb = za;
+        else
+            while (b[zb] == 0) ++zb;
+        std::span<const uint32_t> sa(a.data() + za, a.size() - za), sb(b.data() + zb, b.size() - zb);
+        size_type la = sa.size(), lb = sb.size();
+        while (la > 1 && sa[la - 1] == 0) --la;
+        while (lb > 1 && sb[lb - 1] == 0) --lb;
+        sa = sa.first(la);
+        sb = sb.first(lb);
+        if ((words64(sa) + words64(sb)) * 64 <= HOST_MAX_TOTAL_BITS) {
+            Digits r = mul_limbs(sa, sb, same);
+            while (r.size() > 1 && r.back() == 0) r.pop_back();
+            r.insert(r.begin(), za + zb, 0u);
+            res[i] = std::move(r);
+        } else {
+            big.push_back(Big{i, za, zb, sa, sb, same});
+        }
+    }
+    if (big.empty()) return res;
+    std::vector<backend::Job> jobs(big.size());
+    for (size_t k = 0; k < big.size(); ++k) {
+        Big& g = big[k];
+        Digits& r = res[g.idx];
+        r.assign(g.za + g.zb + g.a.size() + g.b.size(), 0u);
+        jobs[k] = backend::Job{g.a, g.same ? g.a : g.b, g.same, r.data() + g.za + g.zb, 0};
+    }
+    backend::multiply(jobs, plan_mode);
+    g_gpu_products.fetch_add(big.size(), std::memory_order_relaxed);
+    for (size_t k = 0; k < big.size(); ++k) res[big[k].idx].resize(big[k].za + big[k].zb + jobs[k].out_len);
     return res;
 }
 

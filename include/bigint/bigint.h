@@ -22,6 +22,7 @@
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "bigint/bigint_base.h"
 #include "bigint/mul.h"
@@ -233,6 +234,11 @@ public:
                               bool check = true) const -> std::pair<BigInt, BigInt>;
 
     static auto get_pow_of_ten(uint32_t exponent) -> BigInt;
+
+    // a_i * b_i for every pair, each equal to operator*; the big ones run as one grouped GPU
+    // launch set (one product-tree level at a time).
+    static auto mul_batch(const std::vector<std::pair<const BigInt*, const BigInt*>>& pairs,
+                          int plan_mode = 1) -> std::vector<BigInt>;
 
     friend class BigFloat;
 

@@ -42,3 +42,14 @@ auto gpu_product_count() noexcept -> uint64_t;
 auto host_product_count() noexcept -> uint64_t;
 
 }  // namespace bigint::mul
+
+namespace bigint::mul {
+
+// Products of many independent pairs (a_i * b_i), each exactly what mul_digits(a_i, b_i) returns
+// with precision 0 (b_i == a_i by address selects squaring).  Pairs above the host threshold go
+// to the engine as one grouped launch set; this is how a product-tree level runs on the GPU
+// (the reference multiplies a level one `operator*` at a time, examples/pi.cpp:45-58).
+auto mul_digits_batch(const std::vector<std::pair<const Digits*, const Digits*>>& pairs,
+                      int plan_mode = 1) -> std::vector<Digits>;
+
+}  // namespace bigint::mul

@@ -13,6 +13,9 @@ import pytest
 
 import bigfloat_model as BF
 
+if hasattr(__import__("sys"), "set_int_max_str_digits"):
+    __import__("sys").set_int_max_str_digits(0)  # decimal round trips of 60,000-digit numbers
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CALC = os.path.join(ROOT, "bigint_amd", "_build", "bigint_calc")
 
