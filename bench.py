@@ -77,57 +77,48 @@ def cpu_baseline(a: np.ndarray, b: np.ndarray, bits: int, max_s: float = 25.0) -
                       f"{info.get('cores_per_socket')} cores x {info.get('threads_per_core')} threads"}
 
 
-def batch_leg(torch, dev, engine, L, pairs_count: int, steps: int) -> dict:
+def batch_leg(hip, engine, L, pairs_count: int, steps: int) -> dict:
     """The batched product-tree workload: `pairs_count` independent products whose operand sizes
     are log-uniform in [2^12, 2^24] bits (SplitMix64 from the fixed seed), uniform random limbs
-    from a seeded device RNG with the exact top bit set.  Timed per plan mode on device-resident
-    data; every mode's products must equal the plain mode's limb for limb, and a sample is
-    checked against residues on the host."""
-    import ctypes
+    from a seeded generator with the exact top bit set, resident in HBM.  Timed per plan mode;
+    every mode's products must equal the plain mode's limb for limb, and a sample is checked
+    against residues on the host."""
     bits = L.batch_bits(pairs_count)
     la = np.array([-(-ba // 28) for ba, _ in bits], np.int64)
     lb = np.array([-(-bb // 28) for _, bb in bits], np.int64)
     tot_in = int(la.sum() + lb.sum())
-    g = torch.Generator(device=dev)
-    g.manual_seed(L.SEED & 0x7FFFFFFFFFFFFFFF)
-    pool = torch.randint(0, 1 << 28, (tot_in,), dtype=torch.int32, device=dev, generator=g)
+    rng = np.random.default_rng(L.SEED & 0xFFFFFFFF)
+    host = rng.integers(0, 1 << 28, tot_in, dtype=np.uint32)
     offa = np.concatenate([[0], np.cumsum(la + lb)[:-1]])
     offb = offa + la
-    # exact bit lengths: fix each operand's top limb on the host, scatter to the device
-    rng = np.random.default_rng(7)
-    tops, idx = [], []
-    for i, (ba, bb) in enumerate(bits):
+    for i, (ba, bb) in enumerate(bits):  # exact bit lengths
         for off, nb, n in ((offa[i], ba, la[i]), (offb[i], bb, lb[i])):
             r = nb - 28 * (n - 1)
-            tops.append((int(rng.integers(0, 1 << 28)) & ((1 << r) - 1)) | (1 << (r - 1)))
-            idx.append(off + n - 1)
-    pool[torch.tensor(idx, device=dev)] = torch.tensor(tops, dtype=torch.int32, device=dev)
-    outs = {m: torch.empty(int((la + lb).sum()), dtype=torch.int32, device=dev) for m in (0, 1, 2)}
-    oo = np.concatenate([[0], np.cumsum(la + lb)[:-1]])
-    stream = torch.cuda.Stream(dev)
+            host[off + n - 1] = (host[off + n - 1] & ((1 << r) - 1)) | (1 << (r - 1))
+    pool = hip.Buffer.from_array(host)
+    outs = {m: hip.Buffer(4 * tot_in) for m in (0, 1, 2)}
+    stream = hip.Stream()
     res = {}
     for mode, name in ((engine.PLAN_FIXED, "plain"), (engine.PLAN_TABLE, "table"),
                        (engine.PLAN_LEARNED, "learned")):
-        base, ob = pool.data_ptr(), outs[mode].data_ptr()
+        base, ob = pool.ptr, outs[mode].ptr
         pairs = [(base + 4 * int(offa[i]), int(la[i]), base + 4 * int(offb[i]), int(lb[i]),
-                  ob + 4 * int(oo[i]), int(la[i] + lb[i])) for i in range(pairs_count)]
-        engine.mul_batch_device(pairs, mode, stream.cuda_stream)  # warm-up (plans, workspace)
-        torch.cuda.synchronize()
+                  ob + 4 * int(offa[i]), int(la[i] + lb[i])) for i in range(pairs_count)]
+        engine.mul_batch_device(pairs, mode, stream.handle)  # warm-up (plans, workspace)
+        stream.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
-            engine.mul_batch_device(pairs, mode, stream.cuda_stream)
-        torch.cuda.synchronize()
+            engine.mul_batch_device(pairs, mode, stream.handle)
+        stream.synchronize()
         dt = (time.perf_counter() - t0) / steps
         res[name] = {"ms": round(dt * 1e3, 3), "Gbit_s": round(sum(a + b for a, b in bits) / dt / 1e9, 2)}
-    same = all(torch.equal(outs[0], outs[m]) for m in (1, 2))
-    # residue check of a sample against the host
-    host = outs[0].cpu().numpy().view(np.uint32)
-    pl = pool.cpu().numpy().view(np.uint32)
+    o = {m: outs[m].download(np.uint32, tot_in) for m in (0, 1, 2)}
+    same = bool(np.array_equal(o[0], o[1]) and np.array_equal(o[0], o[2]))
     ok = True
     for i in rng.choice(pairs_count, size=min(48, pairs_count), replace=False):
-        a = pl[offa[i]:offa[i] + la[i]]
-        b = pl[offb[i]:offb[i] + lb[i]]
-        p = host[oo[i]:oo[i] + la[i] + lb[i]]
+        a = host[offa[i]:offa[i] + la[i]]
+        b = host[offb[i]:offb[i] + lb[i]]
+        p = o[0][offa[i]:offa[i] + la[i] + lb[i]]
         for q in L.RESIDUE_PRIMES[:3]:
             ok &= L.residue_mod(p, q) == L.residue_mod(a, q) * L.residue_mod(b, q) % q
     res["pairs"] = pairs_count
@@ -138,14 +129,18 @@ def batch_leg(torch, dev, engine, L, pairs_count: int, steps: int) -> dict:
     return res
 
 
-def copy_calibration(torch, dev) -> int:
+def copy_calibration(hip) -> int:
     """A plain 1 GiB device-to-device copy, three times: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
     this command gives the counters' reading for a known byte count (1 GiB read + 1 GiB written)."""
-    src = torch.ones(1 << 28, dtype=torch.int32, device=dev)
-    dst = torch.empty_like(src)
+    import ctypes
+    src = hip.Buffer(1 << 30)
+    dst = hip.Buffer(1 << 30)
+    src.zero()
     for _ in range(3):
-        dst.copy_(src)
-    torch.cuda.synchronize()
+        rc = hip.rt().hipMemcpy(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), 1 << 30, hip.D2D)
+        if rc:
+            raise RuntimeError(f"hipMemcpy D2D failed: {rc}")
+    hip.synchronize()
     print(json.dumps({"copy_bytes_each_way": 1 << 30, "launches": 3}), flush=True)
     return 0
 
@@ -167,9 +162,11 @@ def main() -> int:
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
         return 2
 
-    import torch
-    from bigint_amd import engine
+    from bigint_amd import engine, hip
     from bigint_amd import limbs as L
+
+    if args.copy_calibration:
+        return copy_calibration(hip)
 
     bits = 1 << args.bits
     engine.init(0)
@@ -179,35 +176,29 @@ def main() -> int:
         planner.load()
     a = L.random_operand(bits, L.SEED)
     b = L.random_operand(bits, L.SEED + 1)
-    dev = torch.device("cuda:0")
-    da = torch.from_numpy(a.view(np.int32)).to(dev)
-    db = torch.from_numpy(b.view(np.int32)).to(dev)
-    dout = torch.empty(a.size + b.size, dtype=torch.int32, device=dev)
-    # a dedicated stream: the engine treats handle 0 as "its own stream", and torch's default
-    # stream has handle 0 on ROCm
-    stream = torch.cuda.Stream(dev)
+    da = hip.Buffer.from_array(a)
+    db = hip.Buffer.from_array(b)
+    dout = hip.Buffer(4 * (a.size + b.size))
+    stream = hip.Stream()
 
     def step():
-        engine.mul_device(da.data_ptr(), a.size, db.data_ptr(), b.size, False, dout.data_ptr(),
-                          dout.numel(), stream.cuda_stream)
+        engine.mul_device(da.ptr, a.size, db.ptr, b.size, False, dout.ptr, a.size + b.size,
+                          stream.handle)
 
-    if args.copy_calibration:
-        return copy_calibration(torch, dev)
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    hip.synchronize()
     engine.profile(True)
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
-    torch.cuda.synchronize()
+    hip.synchronize()
     wall = time.perf_counter() - t0
     prof = engine.profile_read()
     engine.profile(False)
 
     # exactness of the timed product (identical every step), outside the timed region
-    p = dout.cpu().numpy().view(np.uint32)
+    p = dout.download(np.uint32, a.size + b.size)
     exact = all(L.residue_mod(p, q) == L.residue_mod(a, q) * L.residue_mod(b, q) % q
                 for q in L.RESIDUE_PRIMES)
 
@@ -245,7 +236,7 @@ def main() -> int:
         "mul_ms": round(ms, 4),
     }
     if args.batch_pairs:
-        line["batch"] = batch_leg(torch, dev, engine, L, args.batch_pairs, 2)
+        line["batch"] = batch_leg(hip, engine, L, args.batch_pairs, 2)
         line["batch"]["planner_loaded"] = planner_loaded
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a, b, bits)

@@ -42,26 +42,23 @@ class Timer:
     """Times planned device batches of k identical-shape products on a dedicated stream."""
 
     def __init__(self, max_limbs: int = 1 << 25, seed: int = 1):
-        import torch
-        self.torch = torch
-        self.dev = torch.device("cuda:0")
-        g = torch.Generator(device=self.dev)
-        g.manual_seed(seed)
-        self.pool = torch.randint(1, 1 << 28, (max_limbs,), dtype=torch.int32, device=self.dev,
-                                  generator=g)
-        self.out = torch.empty(2 * max_limbs + 64, dtype=torch.int32, device=self.dev)
+        from . import hip
+        self.hip = hip
+        rng = np.random.default_rng(seed)
+        self.pool = hip.Buffer.from_array(rng.integers(1, 1 << 28, max_limbs, dtype=np.uint32))
+        self.out_words = 2 * max_limbs + 64
+        self.out = hip.Buffer(4 * self.out_words)
         self.max_limbs = max_limbs
-        self.stream = torch.cuda.Stream(self.dev)
+        self.stream = hip.Stream()
+        self.e0, self.e1 = hip.Event(), hip.Event()
         engine.lib().bi_hip_mul_batch_device_planned.argtypes = [
             ctypes.POINTER(engine.DPair), ctypes.c_size_t, ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p]
 
     def run(self, la: int, lb: int, k: int, plan: int, reps: int = 2) -> float:
         """Median per-product device time (us) of k products la x lb under `plan`."""
-        torch = self.torch
-        if k * (la + lb) > self.out.numel() or la > self.max_limbs or lb > self.max_limbs:
+        if k * (la + lb) > self.out_words or la > self.max_limbs or lb > self.max_limbs:
             raise ValueError("shape too large for the timing pools")
-        base = self.pool.data_ptr()
-        obase = self.out.data_ptr()
+        base, obase = self.pool.ptr, self.out.ptr
         pairs = []
         for i in range(k):
             offa = (i * 7919) % max(1, self.max_limbs - la)
@@ -73,16 +70,14 @@ class Timer:
         L = engine.lib()
         times = []
         for r in range(reps + 1):
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(self.stream)
-            rc = L.bi_hip_mul_batch_device_planned(arr, k, ids, self.stream.cuda_stream)
-            e1.record(self.stream)
+            self.e0.record(self.stream)
+            rc = L.bi_hip_mul_batch_device_planned(arr, k, ids, self.stream.handle)
+            self.e1.record(self.stream)
             if rc:
                 raise engine.EngineError(rc, "planned batch")
-            e1.synchronize()
+            self.e1.synchronize()
             if r:
-                times.append(e0.elapsed_time(e1) * 1e3 / k)
+                times.append(self.e0.elapsed_ms(self.e1) * 1e3 / k)
         return float(np.median(times))
 
 
@@ -190,41 +185,37 @@ def train_gpu(X: np.ndarray, T: np.ndarray, M: np.ndarray, epochs: int = 3000, l
               seed: int = 0, log=None):
     """Full-batch Adam on the GPU through bi_hip_mlp_grad / bi_hip_mlp_adam.  Inputs are
     normalised; returns (params, norm6, loss history)."""
-    import torch
-    dev = torch.device("cuda:0")
+    from . import hip
     mean = X.mean(axis=0)
     std = X.std(axis=0) + 1e-6
     Xn = ((X - mean) / std).astype(np.float32)
     L = engine.lib()
     NP = L.bi_hip_mlp_param_count(K)
-    p = torch.from_numpy(init_params(seed)).to(dev)
-    assert p.numel() == NP
-    m = torch.zeros(NP, device=dev)
-    v = torch.zeros(NP, device=dev)
-    g = torch.zeros(NP + 1, device=dev)
-    x = torch.from_numpy(Xn).to(dev)
-    t = torch.from_numpy(T).to(dev)
-    mk = torch.from_numpy(M).to(dev)
-    st = torch.cuda.Stream(dev)
-    fp = ctypes.c_void_p
+    p0 = init_params(seed)
+    assert p0.size == NP
+    p = hip.Buffer.from_array(p0)
+    m = hip.Buffer.from_array(np.zeros(NP, np.float32))
+    v = hip.Buffer.from_array(np.zeros(NP, np.float32))
+    g = hip.Buffer.from_array(np.zeros(NP + 1, np.float32))
+    x = hip.Buffer.from_array(Xn)
+    t = hip.Buffer.from_array(T.astype(np.float32))
+    mk = hip.Buffer.from_array(M.astype(np.float32))
+    st = hip.Stream()
     hist = []
     for ep in range(1, epochs + 1):
-        rc = L.bi_hip_mlp_grad(fp(x.data_ptr()), fp(t.data_ptr()), fp(mk.data_ptr()), x.shape[0],
-                               fp(p.data_ptr()), K, fp(g.data_ptr()), fp(st.cuda_stream))
+        rc = L.bi_hip_mlp_grad(x.ptr, t.ptr, mk.ptr, Xn.shape[0], p.ptr, K, g.ptr, st.handle)
         if rc:
             raise engine.EngineError(rc, "bi_hip_mlp_grad")
-        rc = L.bi_hip_mlp_adam(fp(p.data_ptr()), fp(g.data_ptr()), fp(m.data_ptr()), fp(v.data_ptr()),
-                               NP, ctypes.c_float(lr), ctypes.c_float(0.9), ctypes.c_float(0.999),
-                               ctypes.c_float(1e-8), ep, fp(st.cuda_stream))
+        rc = L.bi_hip_mlp_adam(p.ptr, g.ptr, m.ptr, v.ptr, NP, lr, 0.9, 0.999, 1e-8, ep, st.handle)
         if rc:
             raise engine.EngineError(rc, "bi_hip_mlp_adam")
         if ep % 100 == 0 or ep == 1:
             st.synchronize()
-            hist.append((ep, float(g[NP].item())))
+            hist.append((ep, float(g.download(np.float32, NP + 1)[NP])))
             if log and ep % 500 == 0:
                 log(f"epoch {ep} loss {hist[-1][1]:.5f}")
     st.synchronize()
-    return p.cpu().numpy(), np.concatenate([mean, std]).astype(np.float32), hist
+    return p.download(np.float32, NP), np.concatenate([mean, std]).astype(np.float32), hist
 
 
 def install(table: np.ndarray | None, params: np.ndarray | None, norm6: np.ndarray | None) -> None:

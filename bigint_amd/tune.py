@@ -77,7 +77,7 @@ def main() -> int:
 
     engine.init(0)
     timer = P.Timer()
-    max_out = timer.out.numel() - 64
+    max_out = timer.out_words - 64
     def shapes(n, seed):
         sh = P.sample_shapes(n, seed)
         res = []

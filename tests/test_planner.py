@@ -39,36 +39,32 @@ def test_reference_gradient_matches_finite_differences():
 
 @pytest.mark.gpu
 def test_gpu_forward_and_gradient_match_reference():
-    import torch
-    from bigint_amd import engine
+    from bigint_amd import engine, hip
     engine.init(0)
     L = engine.lib()
-    dev = torch.device("cuda:0")
     p, x, t, mask = rand_problem(B=1000, seed=3)
     NP = L.bi_hip_mlp_param_count(K)
     assert NP == p.size
-    tp = torch.tensor(p, dtype=torch.float32, device=dev)
-    tx = torch.tensor(x, dtype=torch.float32, device=dev)
-    tt = torch.tensor(t, dtype=torch.float32, device=dev)
-    tm = torch.tensor(mask, dtype=torch.float32, device=dev)
-    ty = torch.zeros(x.shape[0], K, device=dev)
-    tg = torch.zeros(NP + 1, device=dev)
-    st = torch.cuda.Stream(dev)
-    f = ctypes.c_void_p
-    assert L.bi_hip_mlp_forward(f(tx.data_ptr()), x.shape[0], f(tp.data_ptr()), K, f(ty.data_ptr()),
-                                f(st.cuda_stream)) == 0
+    tp = hip.Buffer.from_array(p.astype(np.float32))
+    tx = hip.Buffer.from_array(x.astype(np.float32))
+    tt = hip.Buffer.from_array(t.astype(np.float32))
+    tm = hip.Buffer.from_array(mask.astype(np.float32))
+    ty = hip.Buffer(4 * x.shape[0] * K)
+    tg = hip.Buffer(4 * (NP + 1))
+    st = hip.Stream()
+    assert L.bi_hip_mlp_forward(tx.ptr, x.shape[0], tp.ptr, K, ty.ptr, st.handle) == 0
     st.synchronize()
     # reference run in float64 on the fp32-rounded parameters and inputs
     p32 = p.astype(np.float32).astype(np.float64)
     x32 = x.astype(np.float32).astype(np.float64)
     y_ref, _ = MO.forward(p32, x32, K)
+    y = ty.download(np.float32, x.shape[0] * K).reshape(-1, K)
     # fp32 tolerance: |dy| <= 1e-4 * (1 + |y|)
-    assert np.all(np.abs(ty.cpu().numpy() - y_ref) <= 1e-4 * (1 + np.abs(y_ref)))
-    assert L.bi_hip_mlp_grad(f(tx.data_ptr()), f(tt.data_ptr()), f(tm.data_ptr()), x.shape[0],
-                             f(tp.data_ptr()), K, f(tg.data_ptr()), f(st.cuda_stream)) == 0
+    assert np.all(np.abs(y - y_ref) <= 1e-4 * (1 + np.abs(y_ref)))
+    assert L.bi_hip_mlp_grad(tx.ptr, tt.ptr, tm.ptr, x.shape[0], tp.ptr, K, tg.ptr, st.handle) == 0
     st.synchronize()
     loss_ref, g_ref = MO.loss_grad(p32, x32, t.astype(np.float32), mask, K)
-    g = tg.cpu().numpy()
+    g = tg.download(np.float32, NP + 1)
     # fp32 tolerance on the gradient: 1e-4 relative to the gradient's scale
     scale = np.abs(g_ref).max()
     assert np.max(np.abs(g[:NP] - g_ref)) <= 1e-4 * scale
@@ -76,30 +72,46 @@ def test_gpu_forward_and_gradient_match_reference():
 
 
 @pytest.mark.gpu
+def test_training_reduces_loss_and_choices_are_allowed():
+    """A few hundred Adam steps on a synthetic cost surface lower the loss; the learned choice
+    only ever returns allowed plans."""
+    from bigint_amd import engine, planner
+    engine.init(0)
+    rng = np.random.default_rng(5)
+    X = rng.uniform([10, 10, 0], [28, 28, 6], (400, 3)).astype(np.float32)
+    T = np.stack([X[:, 0] + X[:, 1] * 0.5 + k * 0.1 * (X[:, 2] - 3) for k in range(K)], 1).astype(np.float32)
+    M = np.ones_like(T)
+    params, norm, hist = planner.train_gpu(X, T, M, epochs=400, lr=3e-3)
+    assert hist[-1][1] < 0.2 * hist[0][1]
+    table = np.zeros(planner.TABLE_SHAPE, np.int8)
+    planner.install(table, params, norm)
+    shapes = [(50000, 900, 1), (3000, 3000, 4), (9586981, 37450, 1), (10, 10, 2)]
+    for (la, lb, k), c in zip(shapes, planner.choose(shapes, engine.PLAN_LEARNED)):
+        assert c in planner.allowed(la, lb)
+
+
+@pytest.mark.gpu
 def test_every_plan_is_exact():
-    from bigint_amd import engine
+    from bigint_amd import engine, hip
     from bigint_amd import limbs as L
     from oracle import oracle
     from bigint_amd import planner
-    import torch
     engine.init(0)
-    dev = torch.device("cuda:0")
+    engine.lib().bi_hip_mul_batch_device_planned.argtypes = [
+        ctypes.POINTER(engine.DPair), ctypes.c_size_t, ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p]
     for la, lb in [(50000, 900), (3000, 3000), (70000, 20000), (9000, 130)]:
         a = L.random_operand(28 * la, la)
         b = L.random_operand(28 * lb, lb)
         exp = oracle.mul_limbs(a, b)
-        da = torch.from_numpy(a.view(np.int32)).to(dev)
-        db = torch.from_numpy(b.view(np.int32)).to(dev)
+        da, db = hip.Buffer.from_array(a), hip.Buffer.from_array(b)
         for plan in planner.allowed(a.size, b.size):
-            out = torch.zeros(a.size + b.size, dtype=torch.int32, device=dev)
-            pair = (engine.DPair * 1)(engine.DPair(da.data_ptr(), a.size, db.data_ptr(), b.size,
-                                                   out.data_ptr(), out.numel()))
+            out = hip.Buffer(4 * (a.size + b.size))
+            out.zero()
+            pair = (engine.DPair * 1)(engine.DPair(da.ptr, a.size, db.ptr, b.size, out.ptr, a.size + b.size))
             ids = (ctypes.c_int32 * 1)(plan)
-            engine.lib().bi_hip_mul_batch_device_planned.argtypes = [
-                ctypes.POINTER(engine.DPair), ctypes.c_size_t, ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p]
             assert engine.lib().bi_hip_mul_batch_device_planned(pair, 1, ids, None) == 0
             engine.synchronize()
-            got = L.strip(out.cpu().numpy().view(np.uint32))
+            got = L.strip(out.download(np.uint32, a.size + b.size))
             assert np.array_equal(got, exp), (la, lb, plan)
 
 
