@@ -210,6 +210,7 @@ def main() -> int:
     bytes_per_launch = d["bytes"] / d["launches"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     n = 1 << engine.transform_log2(a.size, b.size)
+    plan_id = planner.choose([(int(a.size), int(b.size), 1)], engine.PLAN_TABLE)[0]
     kernels = {k: {"avg_ms": v["ms"] / v["launches"], "launches_per_step": v["launches"] / args.steps,
                    "alg_GBs": v["bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] else None}
                for k, v in prof.items() if v["launches"]}
@@ -228,7 +229,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"2^{args.bits}-bit x 2^{args.bits}-bit multiply, one product per step, "
                                f"device-resident", "limbs": [int(a.size), int(b.size)],
-                   "ntt_length": n, "exact": bool(exact)},
+                   "plain_ntt_length": n, "plan": plan_id, "exact": bool(exact)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "alg_bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_ms, 4)},

@@ -246,31 +246,40 @@ public:
         return ws;
     }
 
-    // Launch the whole pass sequence of one plan for njobs products of the same transform
-    // length.  single/gsingle are used when njobs == 1 and no descriptor arrays are given.
-    int launch_products(const Plan* pl, bool square, uint32_t njobs, const PassJob& single,
+    enum Kind { kPlain = 0, kSquareKind = 1, kBForward = 2, kChunk = 3 };
+
+    // Launch the pass sequence of one plan for njobs jobs of the same transform length.
+    //   kPlain / kSquareKind: product of A and B (or A^2), Garner+carry into the output;
+    //   kBForward: forward transform of operand A only, left in res_a (no Garner);
+    //   kChunk: A-chunk transform, pointwise with the shared transformed B in res_b, inverse,
+    //           Garner summing the chunk products.
+    // single/gsingle are used when njobs == 1 and no descriptor arrays are given.
+    int launch_products(const Plan* pl, Kind kind, uint32_t njobs, const PassJob& single,
                         const PassJob* d_jobs, uint32_t ngjobs, const GarnerJob& gsingle,
                         const GarnerJob* d_gjobs, uint32_t max_out_len, double in_limbs,
                         double out_limbs, uint32_t* status, uint32_t status_words, hipStream_t st) {
         const uint32_t n = 1u << pl->log_n;
         const double nj = (double)njobs;
-        if (timed(BI_HIP_K_FILL, 4.0 * status_words, st,
+        if (kind != kBForward &&
+            timed(BI_HIP_K_FILL, 4.0 * status_words, st,
                   [&] { return launch_fill(status, 0u, status_words, st); }) != hipSuccess)
             return BI_HIP_EDEVICE;
-        const double nops_d = square ? 1.0 : 2.0;
+        const bool one_op = kind != kPlain;
+        const double nops_d = one_op ? 1.0 : 2.0;
         PassArgs a{};
         a.job = single;
         a.jobs = d_jobs;
         a.log_n = pl->log_n;
         a.wtab = wtab;
-        const uint32_t sq = square ? (uint32_t)kSquare : 0u;
-        const uint32_t nops = square ? 1u : 2u;
+        const uint32_t base_flags = kind == kSquareKind ? (uint32_t)kSquare
+                                  : (kind == kPlain ? 0u : (uint32_t)kSingle);
+        const uint32_t nops = one_op ? 1u : 2u;
         for (size_t k = 0; k < pl->outer.size(); ++k) {
             const OuterPass& op = pl->outer[k];
             a.log_r = op.log_r;
             a.log_c = op.log_c;
             a.log_s = op.log_s;
-            a.flags = sq | (k == 0 ? (uint32_t)kFromLimbs : 0u);
+            a.flags = base_flags | (k == 0 ? (uint32_t)kFromLimbs : 0u);
             a.beta = op.beta;
             a.beta_inv = nullptr;
             a.beta_stride = op.nblk;
@@ -283,16 +292,21 @@ public:
         a.log_r = pl->mid_log_r;
         a.log_c = 0;
         a.log_s = 0;
-        a.flags = sq | (pl->outer.empty() ? (uint32_t)kFromLimbs : 0u);
+        a.flags = (kind == kSquareKind ? (uint32_t)kSquare : 0u) |
+                  (kind == kBForward ? (uint32_t)kFwdOnly : 0u) |
+                  (kind == kChunk ? (uint32_t)kBPre : 0u) |
+                  (pl->outer.empty() ? (uint32_t)kFromLimbs : 0u);
         a.beta = pl->mid_beta;
         a.beta_inv = pl->mid_beta_inv;
         a.beta_stride = pl->mid_nblk;
         {
-            const double in_bytes = pl->outer.empty() ? 4.0 * in_limbs : 12.0 * n * nops_d * nj;
+            double in_bytes = pl->outer.empty() ? 4.0 * in_limbs : 12.0 * n * nops_d * nj;
+            if (kind == kChunk) in_bytes += 12.0 * n * nj;  // the shared transformed B
             if (timed(BI_HIP_K_MID, in_bytes + 12.0 * n * nj, st,
                       [&] { return launch_mid_pass(a, njobs, st); }) != hipSuccess)
                 return BI_HIP_EDEVICE;
         }
+        if (kind == kBForward) return BI_HIP_OK;
         for (size_t k = pl->outer.size(); k-- > 0;) {
             const OuterPass& op = pl->outer[k];
             a.log_r = op.log_r;
@@ -318,9 +332,19 @@ public:
 
     int run(const uint32_t* d_a, uint32_t la, const uint32_t* d_b, uint32_t lb, bool square,
             uint32_t* d_out, hipStream_t st) {
-        const int lg = bi_hip_transform_log2(la, lb);
-        if (lg < 0) return BI_HIP_ETOOBIG;
-        Plan* pl = plan((uint32_t)lg);
+        const int lg0 = bi_hip_transform_log2(la, lb);
+        if (lg0 < 0) return BI_HIP_ETOOBIG;
+        // the table (or the heuristic) may pick a chunked or re-split plan: those run as a batch
+        const bi_hip_dpair p{d_a, la, square ? nullptr : d_b, lb, d_out, (size_t)la + lb};
+        std::vector<PairPlan> chosen;
+        {
+            const int rc = choose_plans(&p, 1, BI_HIP_PLAN_TABLE, nullptr, chosen, st);
+            if (rc) return rc;
+        }
+        if (chosen[0].chunk_len || chosen[0].mid_lr < std::min<uint32_t>(12, (uint32_t)lg0))
+            return run_batch(&p, 1, BI_HIP_PLAN_TABLE, nullptr, st);
+        const uint32_t lg = (uint32_t)lg0;
+        Plan* pl = plan(lg);
         if (!pl) return BI_HIP_ENOMEM;
         const uint32_t n = 1u << lg;
         const uint32_t out_len = la + lb;
@@ -340,7 +364,7 @@ public:
         for (int I = 0; I < 3; ++I) gj.kscale[I] = pl->kscale[I];
         gj.status = status;
         gj.counter = status + ntiles;
-        return launch_products(pl, square, 1, job, nullptr, 1, gj, nullptr, out_len,
+        return launch_products(pl, square ? kSquareKind : kPlain, 1, job, nullptr, 1, gj, nullptr, out_len,
                                (double)la + (square ? 0.0 : (double)lb), (double)out_len, status,
                                ntiles + 1, st);
     }
@@ -351,15 +375,24 @@ public:
 
     // Plan of one product: transform length, optional chunking of the longer operand, and the
     // size of the contiguous block pass (which fixes the radix split of the outer stages).
+    // A chunked product transforms the shorter operand once and reuses it for every chunk.
     struct PairPlan {
         uint32_t lg;         // log2 of the transform length
-        uint32_t log_chunk;  // 0: ordinary product; else chunk length 2^log_chunk, lg = log_chunk+1
+        uint32_t chunk_len;  // 0: ordinary product; else limbs per chunk of the longer operand
         uint32_t mid_lr;     // log2 of the block-pass size
     };
-    static constexpr int kNumPlanIds = 6;
+    static constexpr int kNumPlanIds = 8;
+
+    static uint32_t ceil_log2(uint64_t x) {
+        uint32_t c = 0;
+        while (((uint64_t)1 << c) < x) ++c;
+        return c;
+    }
 
     // Plan ids: 0/1/2 = unchunked with 4096/2048/1024-point block pass; 3/4/5 = chunks of
-    // 2^c, 2^(c+1), 2^(c+2) limbs with 2^c the smallest power of two >= the shorter operand.
+    // 2^c, 2^(c+1), 2^(c+2) limbs, 2^c the smallest power of two >= the shorter operand;
+    // 6/7 = the longer operand split into 2 / 3 chunks (multiples of 4096 limbs).  Chunk plans
+    // are allowed only where their transform is shorter than the unchunked one.
     static bool plan_for_id(uint32_t la, uint32_t lb, bool sq, int id, PairPlan& out) {
         const int lg0 = bi_hip_transform_log2(la, lb);
         if (lg0 < 0 || id < 0 || id >= kNumPlanIds) return false;
@@ -372,24 +405,46 @@ public:
             return true;
         }
         if (sq) return false;
-        const uint32_t lo = std::min(la, lb);
-        uint32_t c = 10;
-        while ((1u << c) < lo) ++c;
-        c += (uint32_t)(id - 3);
-        if (c + 2 > (uint32_t)lg0 || c + 1 > 26) return false;  // chunk transform at most half
-        if (!split_outer(c + 1, 12, rs)) return false;
-        out = PairPlan{c + 1, c, 12};
+        const uint32_t hi = std::max(la, lb), lo = std::min(la, lb);
+        uint32_t L;
+        if (id <= 5) {
+            uint32_t c = std::max<uint32_t>(10, ceil_log2(lo)) + (uint32_t)(id - 3);
+            if (c >= 31) return false;
+            L = 1u << c;
+        } else {
+            const uint32_t parts = (uint32_t)id - 4;  // 2 or 3
+            L = (hi + parts - 1) / parts;
+            L = (L + 4095u) & ~4095u;
+        }
+        if (L >= hi) return false;
+        const uint32_t lg = std::max<uint32_t>(4, ceil_log2((uint64_t)L + lo - 1));
+        if (lg >= (uint32_t)lg0 || lg > 26) return false;
+        const uint32_t mid = std::min<uint32_t>(12, lg);
+        if (!split_outer(lg, mid, rs)) return false;
+        out = PairPlan{lg, L, mid};
         return true;
     }
 
-    // Heuristic used until a tuned table or model is installed: chunk when the chunk transforms
-    // are at least 4x shorter than the plain one.
+    // Heuristic used until a tuned table or model is installed: a transform count model.  An
+    // unchunked product costs 3 transforms of length 2^lg (2 when squaring); a chunked one
+    // costs 1 + 2 * chunks transforms of its shorter length.  A transform is priced n * lg, and
+    // chunked plans must win by 10% to cover their extra launches and Garner reads.
     static PairPlan heuristic_plan(uint32_t la, uint32_t lb, bool sq) {
-        PairPlan p{};
-        plan_for_id(la, lb, sq, 0, p);
-        PairPlan c{};
-        if (!sq && p.lg >= 14 && plan_for_id(la, lb, sq, 3, c) && c.lg + 2 <= p.lg) return c;
-        return p;
+        PairPlan best{};
+        plan_for_id(la, lb, sq, 0, best);
+        double best_cost = (sq ? 2.0 : 3.0) * std::ldexp((double)best.lg, (int)best.lg);
+        const uint32_t hi = std::max(la, lb);
+        for (int id = 3; id < kNumPlanIds; ++id) {
+            PairPlan c{};
+            if (!plan_for_id(la, lb, sq, id, c)) continue;
+            const double chunks = (double)((hi + c.chunk_len - 1) / c.chunk_len);
+            const double cost = 1.1 * (1.0 + 2.0 * chunks) * std::ldexp((double)c.lg, (int)c.lg);
+            if (cost < best_cost) {
+                best_cost = cost;
+                best = c;
+            }
+        }
+        return best;
     }
 
     // installed planner state
@@ -510,7 +565,10 @@ public:
         return BI_HIP_OK;
     }
 
-    // Batched products on device pointers, grouped by transform length and squaring.
+    // Batched products on device pointers.  Pairs are grouped by (transform length, block-pass
+    // size, kind); one group is one launch sequence with a job per product (per chunk for chunked
+    // products).  The shared B transforms of chunked products run first, into a region that
+    // lives until their chunk groups have run.
     int run_batch(const bi_hip_dpair* pairs, size_t count, int mode, const int32_t* explicit_ids,
                   hipStream_t st) {
         std::vector<PairPlan> chosen;
@@ -519,12 +577,16 @@ public:
             if (rc) return rc;
         }
         struct Item {
-            uint32_t idx, la, lb;
+            uint32_t idx, la, lb;  // la is the operand that gets chunked (the longer one)
             bool sq, swap;
             PairPlan pp;
             uint32_t nchunks;
+            size_t bhat;  // word offset of the shared B transform (chunked products)
         };
-        std::map<std::tuple<uint32_t, uint32_t, bool>, std::vector<Item>> groups;
+        using Key = std::tuple<int, uint32_t, uint32_t, int>;  // phase, lg, mid_lr, kind
+        std::map<Key, std::vector<uint32_t>> groups;           // -> item indices
+        std::vector<Item> items(count);
+        size_t bhat_words = 0;
         for (size_t i = 0; i < count; ++i) {
             const bi_hip_dpair& p = pairs[i];
             const bool sq = p.b == nullptr || (p.b == p.a && p.lb == p.la);
@@ -532,29 +594,39 @@ public:
             if (!p.a || p.la == 0 || lb == 0 || !p.out || p.la > 0xFFFFFFFFull || lb > 0xFFFFFFFFull)
                 return BI_HIP_EINVAL;
             if (p.out_cap < p.la + lb) return BI_HIP_ECAP;
-            const int lg = bi_hip_transform_log2(p.la, lb);
-            if (lg < 0) return BI_HIP_ETOOBIG;
-            Item it{(uint32_t)i, (uint32_t)p.la, (uint32_t)lb, sq, false, chosen[i], 1};
-            if (it.pp.log_chunk) {
-                it.swap = it.la < it.lb;  // chunk the longer operand
-                const uint32_t la_ = std::max(it.la, it.lb);
-                const uint32_t L = 1u << it.pp.log_chunk;
-                it.nchunks = (la_ + L - 1) / L;
+            if (bi_hip_transform_log2(p.la, lb) < 0) return BI_HIP_ETOOBIG;
+            Item it{(uint32_t)i, (uint32_t)p.la, (uint32_t)lb, sq, false, chosen[i], 1, 0};
+            const uint32_t lg = it.pp.lg, mid = it.pp.mid_lr;
+            if (it.pp.chunk_len) {
+                it.swap = it.la < it.lb;
+                if (it.swap) std::swap(it.la, it.lb);
+                it.nchunks = (it.la + it.pp.chunk_len - 1) / it.pp.chunk_len;
+                it.bhat = bhat_words;
+                bhat_words += (size_t)3 << lg;
+                groups[{0, lg, mid, kBForward}].push_back((uint32_t)i);
+                groups[{1, lg, mid, kChunk}].push_back((uint32_t)i);
+            } else {
+                groups[{1, lg, mid, sq ? kSquareKind : kPlain}].push_back((uint32_t)i);
             }
-            groups[{it.pp.lg, it.pp.mid_lr, sq}].push_back(it);
+            items[i] = it;
         }
-        size_t ws_need = 0, desc_need = 0;
-        for (auto& [key, items] : groups) {
-            const size_t n = (size_t)1 << std::get<0>(key);
+        // workspace: [shared B transforms][one group's residues + status words]
+        size_t grp_words = 0, desc_need = 0;
+        for (auto& [key, idx] : groups) {
+            const size_t n = (size_t)1 << std::get<1>(key);
+            const int kind = std::get<3>(key);
             size_t words = 0, njobs = 0;
-            for (auto& it : items) {
-                words += 3 * n * (it.sq ? 1 : 2) * it.nchunks + (it.la + it.lb + 4095u) / 4096u + 1;
-                njobs += it.nchunks;
+            for (uint32_t i : idx) {
+                const Item& it = items[i];
+                const size_t nj = kind == kBForward ? 1 : it.nchunks;
+                njobs += nj;
+                if (kind == kBForward) continue;
+                words += 3 * n * (kind == kPlain ? 2 : nj) + (it.la + it.lb + 4095u) / 4096u + 1;
             }
-            ws_need = std::max(ws_need, words + 64);
-            desc_need += njobs * sizeof(PassJob) + items.size() * sizeof(GarnerJob) + 512;
+            grp_words = std::max(grp_words, words);
+            desc_need += njobs * sizeof(PassJob) + idx.size() * sizeof(GarnerJob) + 512;
         }
-        uint32_t* w = workspace(ws_need);
+        uint32_t* w = workspace(bhat_words + grp_words + 64);
         if (!w) return BI_HIP_ENOMEM;
         if (desc_need > desc_bytes) {
             if (desc) hipFree(desc);
@@ -566,7 +638,7 @@ public:
         std::vector<uint8_t> hdesc(desc_need);
         struct Launch {
             Plan* pl;
-            bool sq;
+            Kind kind;
             uint32_t njobs, ngjobs, max_out;
             size_t pj_off, gj_off;
             double in_limbs, out_limbs;
@@ -575,46 +647,52 @@ public:
         };
         std::vector<Launch> launches;
         size_t off = 0;
-        for (auto& [key, items] : groups) {
-            Plan* pl = plan(std::get<0>(key), std::get<1>(key));
+        for (auto& [key, idx] : groups) {
+            Plan* pl = plan(std::get<1>(key), std::get<2>(key));
             if (!pl) return BI_HIP_ENOMEM;
-            const size_t n = (size_t)1 << std::get<0>(key);
-            Launch L{pl, std::get<2>(key), 0, (uint32_t)items.size(), 0, 0, 0, 0.0, 0.0, 0, nullptr};
-            uint32_t njobs = 0;
-            for (auto& it : items) njobs += it.nchunks;
-            L.njobs = njobs;
+            const size_t n = (size_t)1 << std::get<1>(key);
+            const Kind kind = (Kind)std::get<3>(key);
+            Launch L{pl, kind, 0, 0, 0, 0, 0, 0.0, 0.0, 0, nullptr};
+            for (uint32_t i : idx) L.njobs += kind == kBForward ? 1 : items[i].nchunks;
+            L.ngjobs = kind == kBForward ? 0 : (uint32_t)idx.size();
             L.pj_off = off;
-            off += njobs * sizeof(PassJob);
+            off += L.njobs * sizeof(PassJob);
             off = (off + 255) & ~(size_t)255;
             L.gj_off = off;
-            off += items.size() * sizeof(GarnerJob);
+            off += L.ngjobs * sizeof(GarnerJob);
             off = (off + 255) & ~(size_t)255;
             auto* pj = reinterpret_cast<PassJob*>(hdesc.data() + L.pj_off);
             auto* gj = reinterpret_cast<GarnerJob*>(hdesc.data() + L.gj_off);
+            uint32_t* res = w + bhat_words;
             size_t res_words = 0;
-            for (auto& it : items) res_words += 3 * n * (it.sq ? 1 : 2) * it.nchunks;
-            uint32_t* res = w;
-            uint32_t* status = w + res_words;
+            if (kind != kBForward)
+                for (uint32_t i : idx) res_words += 3 * n * (kind == kPlain ? 2 : items[i].nchunks);
+            uint32_t* status = res + res_words;
             uint32_t* st_cur = status;
             size_t jp = 0;
-            for (size_t j = 0; j < items.size(); ++j) {
-                const Item& it = items[j];
+            for (size_t j = 0; j < idx.size(); ++j) {
+                const Item& it = items[idx[j]];
                 const bi_hip_dpair& p = pairs[it.idx];
                 const uint32_t* A = it.swap ? p.b : p.a;
                 const uint32_t* B = it.sq ? p.a : (it.swap ? p.a : p.b);
-                const uint32_t la = it.swap ? it.lb : it.la, lb = it.swap ? it.la : it.lb;
+                if (kind == kBForward) {
+                    pj[jp++] = PassJob{B, nullptr, it.lb, 0, w + it.bhat, nullptr};
+                    L.in_limbs += (double)it.lb;
+                    continue;
+                }
                 GarnerJob g{};
                 g.y = res;
-                if (it.nchunks > 1 || it.pp.log_chunk) {
-                    const uint32_t Lc = 1u << it.pp.log_chunk;
+                if (kind == kChunk) {
+                    const uint32_t Lc = it.pp.chunk_len;
                     for (uint32_t c = 0; c < it.nchunks; ++c) {
-                        const uint32_t lac = std::min(Lc, la - c * Lc);
-                        pj[jp++] = PassJob{A + (size_t)c * Lc, B, lac, lb, res, res + 3 * n};
-                        res += 6 * n;
+                        const uint32_t lac = std::min(Lc, it.la - c * Lc);
+                        pj[jp++] = PassJob{A + (size_t)c * Lc, nullptr, lac, it.lb, res, w + it.bhat};
+                        res += 3 * n;
                     }
                     g.nchunks = it.nchunks;
-                    g.log_chunk = it.pp.log_chunk;
-                    g.chunk_stride = 6 * n;
+                    g.chunk_len = Lc;
+                    g.log_chunk = (Lc & (Lc - 1)) == 0 ? ceil_log2(Lc) : 0xFFu;
+                    g.chunk_stride = 3 * n;
                 } else {
                     uint32_t* ra = res;
                     res += 3 * n;
@@ -623,7 +701,7 @@ public:
                         rb = res;
                         res += 3 * n;
                     }
-                    pj[jp++] = PassJob{A, B, la, lb, ra, rb};
+                    pj[jp++] = PassJob{A, B, it.la, it.lb, ra, rb};
                     g.nchunks = 1;
                 }
                 const uint32_t out_len = it.la + it.lb;
@@ -637,7 +715,7 @@ public:
                 st_cur += nt + 1;
                 gj[j] = g;
                 L.max_out = std::max(L.max_out, out_len);
-                L.in_limbs += (double)it.la + (it.sq ? 0.0 : (double)it.lb);
+                L.in_limbs += (double)it.la + (kind == kPlain ? (double)it.lb : 0.0);
                 L.out_limbs += (double)out_len;
             }
             L.status_words = (uint32_t)(st_cur - status);
@@ -651,9 +729,10 @@ public:
             const auto* pj = reinterpret_cast<const PassJob*>(desc + L.pj_off);
             const auto* gj = reinterpret_cast<const GarnerJob*>(desc + L.gj_off);
             const auto* hp = reinterpret_cast<const PassJob*>(hdesc.data() + L.pj_off);
-            const auto* hg = reinterpret_cast<const GarnerJob*>(hdesc.data() + L.gj_off);
-            const int rc = launch_products(L.pl, L.sq, L.njobs, hp[0], L.njobs == 1 ? nullptr : pj,
-                                           L.ngjobs, hg[0], L.ngjobs == 1 ? nullptr : gj, L.max_out,
+            const GarnerJob g0 = L.ngjobs ? reinterpret_cast<const GarnerJob*>(hdesc.data() + L.gj_off)[0]
+                                          : GarnerJob{};
+            const int rc = launch_products(L.pl, L.kind, L.njobs, hp[0], L.njobs == 1 ? nullptr : pj,
+                                           L.ngjobs, g0, L.ngjobs == 1 ? nullptr : gj, L.max_out,
                                            L.in_limbs, L.out_limbs, L.status, L.status_words, st);
             if (rc) return rc;
         }
@@ -895,7 +974,15 @@ int bi_hip_choose_plans(const uint32_t* shapes, const int* is_square, size_t cou
         if (out[i] < 0 || !bi::Engine::plan_for_id(sh[i][0], sh[i][1], sqv[i], out[i], p)) {
             // report which id the heuristic corresponds to
             const bi::Engine::PairPlan h = bi::Engine::heuristic_plan(sh[i][0], sh[i][1], sqv[i]);
-            out[i] = h.log_chunk ? 3 : 0;
+            out[i] = 0;
+            for (int id = 0; id < bi::Engine::kNumPlanIds; ++id) {
+                bi::Engine::PairPlan q{};
+                if (bi::Engine::plan_for_id(sh[i][0], sh[i][1], sqv[i], id, q) && q.lg == h.lg &&
+                    q.chunk_len == h.chunk_len && q.mid_lr == h.mid_lr) {
+                    out[i] = id;
+                    break;
+                }
+            }
         }
         ids[i] = out[i];
     }

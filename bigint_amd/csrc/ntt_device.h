@@ -37,6 +37,25 @@ __device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t b) {
     return r2 < r ? r2 : r;  // unsigned min: r - p wraps above r exactly when r < p
 }
 
+// Montgomery product without the final subtraction: for a < 4P, b < P and 4P < 2^32 (P0 only)
+// the result lies in [0, 2P).
+template <int I>
+__device__ __forceinline__ uint32_t mont_mul_lazy(uint32_t a, uint32_t b) {
+    static_assert(kP[I] < 0x40000000u, "lazy reduction needs 4P < 2^32");
+    constexpr uint32_t p = kP[I];
+    const uint64_t t = (uint64_t)a * b;
+    const uint32_t u = (uint32_t)t * kNegInvP[I];
+    return (uint32_t)((t + (uint64_t)u * p) >> 32);
+}
+
+// x mod 2P folded once: [0, 4P) -> [0, 2P) (4P < 2^32).
+template <int I>
+__device__ __forceinline__ uint32_t fold2p(uint32_t x) {
+    constexpr uint32_t p2 = 2u * kP[I];
+    const uint32_t y = x - p2;
+    return y < x ? y : x;
+}
+
 // (a + b) mod P for a, b < P.
 template <int I>
 __device__ __forceinline__ uint32_t add_mod(uint32_t a, uint32_t b) {

@@ -124,7 +124,20 @@ __device__ __forceinline__ void group_stages(uint32_t (&x)[kE], const uint32_t* 
                 const uint32_t w = tw[k];
                 const uint32_t u = x[sg * EG + q];
                 const uint32_t v = x[sg * EG + q + h];
-                if (!INV) {
+                if constexpr (I == 0) {
+                    // P0 < 2^29: Harvey-style lazy butterflies.  Forward values live in [0, 4P),
+                    // inverse values in [0, 2P); both fold to [0, P) only where a consumer needs it.
+                    constexpr uint32_t p2 = 2u * kP[0];
+                    if (!INV) {
+                        const uint32_t u2 = fold2p<0>(u);
+                        const uint32_t vw = mont_mul_lazy<0>(v, w);
+                        x[sg * EG + q] = u2 + vw;
+                        x[sg * EG + q + h] = u2 + p2 - vw;
+                    } else {
+                        x[sg * EG + q] = fold2p<0>(u + v);
+                        x[sg * EG + q + h] = mont_mul_lazy<0>(u + p2 - v, w);
+                    }
+                } else if (!INV) {
                     const uint32_t vw = mont_mul<I>(v, w);
                     x[sg * EG + q] = add_mod<I>(u, vw);
                     x[sg * EG + q + h] = sub_mod<I>(u, vw);
@@ -240,7 +253,7 @@ __global__ __launch_bounds__(kThreads) void k_outer_pass(PassArgs a) {
     __shared__ uint32_t tw[1 << LR];
     __shared__ uint32_t bpow[16];
     const PassJob& job = a.jobs ? a.jobs[blockIdx.y] : a.job;
-    const uint32_t nops = INV ? 1u : ((a.flags & kSquare) ? 1u : 2u);
+    const uint32_t nops = INV ? 1u : ((a.flags & (kSquare | kSingle)) ? 1u : 2u);
     const uint32_t prime = blockIdx.z / nops, op = blockIdx.z % nops;
     switch (prime) {
         case 0: outer_body<0, INV, LR>(a, job, op, lds, tw, bpow); break;
@@ -253,6 +266,14 @@ __global__ __launch_bounds__(kThreads) void k_outer_pass(PassArgs a) {
 // Block pass with the pointwise product: contiguous blocks of 2^LR points (one column).
 // Forward-transform the block of A (and of B unless squaring), multiply pointwise, inverse
 // transform, store into A's plane.  grid.x = n >> LR, grid.y = job, grid.z = prime.
+// Pointwise product of forward outputs; P0 values arrive in [0, 4P) and leave in [0, 2P), the
+// input range of the lazy inverse butterflies.
+template <int I>
+__device__ __forceinline__ uint32_t pointwise(uint32_t a, uint32_t b) {
+    if constexpr (I == 0) return mont_mul_lazy<0>(fold2p<0>(a), fold2p<0>(b));
+    else return mont_mul<I>(a, b);
+}
+
 template <int I, int LR>
 __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, uint32_t* lds,
                                          uint32_t* tw, uint32_t* bpow) {
@@ -274,22 +295,31 @@ __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, 
     const bool square = (a.flags & kSquare) != 0;
     uint32_t* pa = job.res_a + (size_t)I * n;
     constexpr int G0 = 0;
+    constexpr int GL = S::NG - 1;
     uint32_t xa[kE];
     global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pa, from_limbs ? job.limbs_a : nullptr,
                                                         job.la, xa, tile_base, 0);
     __syncthreads();  // forward twiddles ready
     tile_transform<I, false, LR, 0>(xa, lds, tw);
+    if (a.flags & kFwdOnly) {  // store the transform in the last group's layout
+        global_io<LR, 0, S::stages(GL), S::trlo(GL), true>(pa, nullptr, 0, xa, tile_base, 0);
+        return;
+    }
     if (!square) {
         uint32_t* pb = job.res_b + (size_t)I * n;
         uint32_t xb[kE];
-        global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pb, from_limbs ? job.limbs_b : nullptr,
-                                                            job.lb, xb, tile_base, 0);
-        tile_transform<I, false, LR, 0>(xb, lds, tw);
+        if (a.flags & kBPre) {
+            global_io<LR, 0, S::stages(GL), S::trlo(GL), false>(pb, nullptr, 0, xb, tile_base, 0);
+        } else {
+            global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pb, from_limbs ? job.limbs_b : nullptr,
+                                                                job.lb, xb, tile_base, 0);
+            tile_transform<I, false, LR, 0>(xb, lds, tw);
+        }
 #pragma unroll
-        for (int k = 0; k < kE; ++k) xa[k] = mont_mul<I>(xa[k], xb[k]);
+        for (int k = 0; k < kE; ++k) xa[k] = pointwise<I>(xa[k], xb[k]);
     } else {
 #pragma unroll
-        for (int k = 0; k < kE; ++k) xa[k] = mont_mul<I>(xa[k], xa[k]);
+        for (int k = 0; k < kE; ++k) xa[k] = pointwise<I>(xa[k], xa[k]);
     }
     // the same LDS table now receives the inverse twiddles
     __syncthreads();
@@ -354,16 +384,18 @@ __device__ __forceinline__ unsigned __int128 crt_coeff(const uint32_t* __restric
 // Convolution coefficient j of the job, split into 28-bit pieces.  For a chunked product the
 // coefficient is the sum of the (at most two) chunk products covering j; every such sum stays
 // below n * 2^56 <= 2^82.
-__device__ __forceinline__ Pieces garner(const GarnerJob& jb, const GarnerConsts& k, uint32_t j) {
+__device__ __forceinline__ Pieces garner(const GarnerJob& jb, const GarnerConsts& k, uint32_t j,
+                                         uint32_t tile_chunk) {
     unsigned __int128 x = 0;
     if (jb.nchunks <= 1) {
         if (j >= jb.n) return {0u, 0u, 0u};
         x = crt_coeff(jb.y, jb.n, jb.kscale, k, j);
     } else {
-        const uint32_t c = j >> jb.log_chunk;
-        for (uint32_t d = 0; d < 4 && d <= c; ++d) {
+        // primary chunk of j; earlier chunks whose products reach j add in
+        const uint32_t c = jb.log_chunk < 32u ? (j >> jb.log_chunk) : tile_chunk;
+        for (uint32_t d = 0; d <= c; ++d) {
             const uint32_t cc = c - d;
-            const uint32_t off = j - (cc << jb.log_chunk);
+            const uint32_t off = j - cc * jb.chunk_len;
             if (off >= jb.n) break;
             if (cc < jb.nchunks) x += crt_coeff(jb.y + (size_t)cc * jb.chunk_stride, jb.n, jb.kscale, k, off);
         }
@@ -406,19 +438,24 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     const uint32_t ntiles = (jb.out_len + kCarryTile - 1u) / kCarryTile;
     if (tile >= ntiles) return;
     const uint32_t base = tile * kCarryTile;
+    // chunk lengths that are multiples of the tile: the whole tile has one primary chunk
+    const uint32_t tile_chunk = (jb.nchunks > 1 && jb.log_chunk == 0xFFu) ? base / jb.chunk_len : 0u;
 
     // 1. Garner for the tile's coefficients, coalesced: thread tid takes base + tid + 256 q
 #pragma unroll 4
     for (uint32_t q = 0; q < kCarryLimbsPerThread; ++q) {
         const uint32_t o = tid + kThreads * q;
-        const Pieces p = garner(jb, a.k, base + o);
+        const Pieces p = garner(jb, a.k, base + o, tile_chunk);
         const uint32_t li = o + (o >> 4);
         s_d[0][li] = p.d0;
         s_d[1][li] = p.d1;
         s_d[2][li] = p.d2;
     }
     if (tid < 2) {  // the previous tile's last two coefficients feed limbs base and base+1
-        const Pieces p = base >= 2 - tid ? garner(jb, a.k, base - 2 + tid) : Pieces{0u, 0u, 0u};
+        // the halo lies in the previous tile, hence possibly in the previous chunk
+        const uint32_t j = base - 2 + tid;
+        const uint32_t hc = (jb.log_chunk == 0xFFu && jb.nchunks > 1) ? j / jb.chunk_len : tile_chunk;
+        const Pieces p = base >= 2 - tid ? garner(jb, a.k, j, hc) : Pieces{0u, 0u, 0u};
         if (tid == 1) {
             s_halo[0] = p.d1;
             s_halo[1] = p.d2;
@@ -442,14 +479,19 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
         else d2 = (o == 1) ? s_halo[1] : s_halo[2];
         s[q] = d0 + d1 + d2;
     }
-    // 3. this thread's carry map, then an inclusive scan of maps across the block
-    uint32_t m = kMapIdentity;
+    // 3. this thread's carry map (carry-out for each carry-in 0..3, four chains run side by
+    //    side), then an inclusive scan of maps across the block
+    uint32_t m;
+    {
+        uint32_t c0 = 0, c1 = 1, c2 = 2, c3 = 3;
 #pragma unroll
-    for (uint32_t q = 0; q < kCarryLimbsPerThread; ++q) {
-        uint32_t f = 0;
-#pragma unroll
-        for (uint32_t c = 0; c < 4; ++c) f |= ((s[q] + c) >> kDigitBits) << (2u * c);
-        m = map_compose(f, m);
+        for (uint32_t q = 0; q < kCarryLimbsPerThread; ++q) {
+            c0 = (s[q] + c0) >> kDigitBits;
+            c1 = (s[q] + c1) >> kDigitBits;
+            c2 = (s[q] + c2) >> kDigitBits;
+            c3 = (s[q] + c3) >> kDigitBits;
+        }
+        m = c0 | (c1 << 2) | (c2 << 4) | (c3 << 6);
     }
     const uint32_t lane = tid & 63u, wave = tid >> 6;
     uint32_t inc = m;

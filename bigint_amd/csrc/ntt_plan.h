@@ -9,6 +9,9 @@ namespace bi {
 enum PassFlags : uint32_t {
     kFromLimbs = 1u,  // the pass reads 28-bit limbs (zero past la/lb) instead of residues
     kSquare = 2u,     // B is A: one forward transform, pointwise square
+    kSingle = 4u,     // strided forward passes transform operand A only
+    kBPre = 8u,       // block pass: res_b already holds B's full forward transform
+    kFwdOnly = 16u,   // block pass: forward-transform A and store it (no product, no inverse)
 };
 
 // One product inside a batched launch.  Residue buffers are planar: 3 planes of n u32.
@@ -44,11 +47,12 @@ struct GarnerJob {
     uint32_t kscale[3];  // n^-1 * R^2 mod P_i, so mont_mul(y, kscale) = y * n^-1 * R
     uint32_t* status;    // look-back status words, one per 4096-limb tile, zeroed before launch
     uint32_t* counter;   // dynamic tile counter, zeroed before launch
-    // Chunked product: the long operand was cut into nchunks pieces of 2^log_chunk limbs, each
-    // multiplied with a transform of length n = 2^(log_chunk+1); chunk c's residues start at
-    // y + c * chunk_stride and its product lands at limb offset c * 2^log_chunk.  nchunks <= 1
+    // Chunked product: the long operand was cut into nchunks pieces of chunk_len limbs, each
+    // multiplied with a transform of length n >= chunk_len + lb - 1; chunk c's residues start at
+    // y + c * chunk_stride and its product lands at limb offset c * chunk_len.  chunk_len is a
+    // power of two (log_chunk < 32) or a multiple of 4096 (log_chunk == 0xFF).  nchunks <= 1
     // means an ordinary product.
-    uint32_t nchunks, log_chunk;
+    uint32_t nchunks, log_chunk, chunk_len;
     uint64_t chunk_stride;
 };
 

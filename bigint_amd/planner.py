@@ -2,7 +2,8 @@
 cost model trained on the GPU with the engine's own HIP kernels (bi_hip_mlp_*).
 
 Plan ids (include/bigint_hip.h): 0/1/2 unchunked with a 4096/2048/1024-point block pass;
-3/4/5 longer operand cut into 2^c / 2^(c+1) / 2^(c+2)-limb chunks.  The reference picks its
+3/4/5 longer operand cut into 2^c / 2^(c+1) / 2^(c+2)-limb chunks; 6/7 longer operand cut into 2 / 3
+chunks.  Chunked plans transform the shorter operand once and reuse it.  The reference picks its
 algorithm with fixed size thresholds (include/bigint/mul.h:80, 153, 192); here the choice is
 measured (table) or predicted (MLP), and every plan computes the same exact product.
 
@@ -22,7 +23,7 @@ import numpy as np
 
 from . import engine
 
-K = 6  # plan ids
+K = 8  # plan ids
 TABLE_SHAPE = (19, 19, 13)
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_PATH = os.path.join(HERE, "plans", "planner.json")

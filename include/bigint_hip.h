@@ -132,7 +132,10 @@ int bi_hip_profile_read(bi_hip_kstat* stats, size_t cap);
  * block-pass size (hence the radix split of the strided stages).  Plan ids:
  *   0, 1, 2  unchunked, 4096 / 2048 / 1024-point block pass
  *   3, 4, 5  longer operand cut into 2^c, 2^(c+1), 2^(c+2)-limb chunks (2^c = smallest power of two
- *            >= the shorter operand), each chunk product on a transform of twice the chunk length
+ *            >= the shorter operand)
+ *   6, 7     longer operand cut into 2 / 3 chunks (lengths rounded up to multiples of 4096 limbs)
+ * A chunked product transforms the shorter operand once and multiplies every chunk with that
+ * transform; chunk plans are valid only where their transform is shorter than the unchunked one.
  * These replace the reference's hand-set size thresholds (include/bigint/mul.h:80, 153, 192). */
 int bi_hip_plan_count(void);
 /* 1 if plan_id is valid for an la x lb product (is_square: b is a), else 0. */

@@ -8,7 +8,7 @@ import pytest
 
 from oracle import mlp_oracle as MO
 
-K = 6
+K = 8
 
 
 def rand_problem(B=96, seed=0):
@@ -99,7 +99,8 @@ def test_every_plan_is_exact():
     engine.init(0)
     engine.lib().bi_hip_mul_batch_device_planned.argtypes = [
         ctypes.POINTER(engine.DPair), ctypes.c_size_t, ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p]
-    for la, lb in [(50000, 900), (3000, 3000), (70000, 20000), (9000, 130)]:
+    for la, lb in [(50000, 900), (3000, 3000), (70000, 20000), (20000, 70000), (9000, 130),
+                   (200000, 150000)]:
         a = L.random_operand(28 * la, la)
         b = L.random_operand(28 * lb, lb)
         exp = oracle.mul_limbs(a, b)
@@ -127,3 +128,79 @@ def test_plan_allowed_rules():
     assert {0, 1, 2, 3, 4, 5} <= set(al) | {1, 2}
     assert 3 in al and 4 in al
     assert not engine.lib().bi_hip_plan_allowed(9586981, 9586981, 1, 3)  # squares never chunk
+    # balanced 2^28-bit operands: two chunks on 2^24-point transforms instead of one 2^25 product
+    assert 6 in planner.allowed(9586981, 9586981)
+    assert set(planner.allowed(70000, 20000)) >= {0, 6, 7}
+    assert set(planner.allowed(200000, 150000)) >= {0, 6}
+
+
+def _planned(pairs_ids):
+    """Run [(a, b, plan)] as ONE planned device batch; returns the products."""
+    from bigint_amd import engine, hip
+    from bigint_amd import limbs as L
+    engine.lib().bi_hip_mul_batch_device_planned.argtypes = [
+        ctypes.POINTER(engine.DPair), ctypes.c_size_t, ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p]
+    keep, dp = [], []
+    for a, b, _ in pairs_ids:
+        da, db = hip.Buffer.from_array(a), hip.Buffer.from_array(b)
+        out = hip.Buffer(4 * (a.size + b.size))
+        out.zero()
+        keep.append((da, db, out))
+        dp.append(engine.DPair(da.ptr, a.size, db.ptr, b.size, out.ptr, a.size + b.size))
+    arr = (engine.DPair * len(dp))(*dp)
+    ids = (ctypes.c_int32 * len(dp))(*[p for _, _, p in pairs_ids])
+    assert engine.lib().bi_hip_mul_batch_device_planned(arr, len(dp), ids, None) == 0
+    engine.synchronize()
+    return [L.strip(o.download(np.uint32, a.size + b.size)) for (a, b, _), (_, _, o) in zip(pairs_ids, keep)]
+
+
+@pytest.mark.gpu
+def test_mixed_plan_batch_is_exact():
+    """One batch holding plain, re-split and chunked products (shared B transforms of several
+    lengths, two- and three-way chunk overlaps, swapped operands) against the oracle."""
+    from bigint_amd import engine, planner
+    from bigint_amd import limbs as L
+    from oracle import oracle
+    engine.init(0)
+    shapes = [(50000, 900), (900, 50000), (70000, 20000), (20000, 70000), (90000, 60000),
+              (3000, 3000), (9000, 130), (200000, 150000), (33000, 4096)]
+    jobs = []
+    for i, (la, lb) in enumerate(shapes):
+        for p in planner.allowed(la, lb):
+            jobs.append((L.random_operand(28 * la, 100 + 17 * i + p), L.random_operand(28 * lb, 500 + 13 * i + p), p))
+    assert {p for _, _, p in jobs} == set(range(planner.K))
+    for (a, b, p), got in zip(jobs, _planned(jobs)):
+        assert np.array_equal(got, oracle.mul_limbs(a, b)), (a.size, b.size, p)
+
+
+@pytest.mark.gpu
+def test_three_chunk_overlap_all_ones():
+    """All-ones operands maximise every coefficient; with 45056-limb chunks on 2^17-point
+    transforms three chunk products overlap at the top of each chunk."""
+    from bigint_amd import engine, planner
+    from bigint_amd import limbs as L
+    engine.init(0)
+    la, lb = 90000, 60000
+    assert 6 in planner.allowed(la, lb)
+    x, y = (1 << (28 * la)) - 1, (1 << (28 * lb)) - 1
+    (got,) = _planned([(L.int_to_limbs(x), L.int_to_limbs(y), 6)])
+    assert L.limbs_to_int(got) == x * y
+
+
+@pytest.mark.gpu
+def test_balanced_2e28_every_plan_agrees():
+    """2^28-bit x 2^28-bit: every allowed plan (plain 2^25-point transforms, re-split block
+    passes, two or three chunks on 2^24-point transforms with one shared B transform) gives the
+    same product, and that product satisfies the residue checks."""
+    from bigint_amd import engine, planner
+    from bigint_amd import limbs as L
+    engine.init(0)
+    a = L.random_operand(1 << 28, L.SEED)
+    b = L.random_operand(1 << 28, L.SEED + 1)
+    plans = planner.allowed(a.size, b.size)
+    assert {0, 6, 7} <= set(plans)
+    outs = _planned([(a, b, p) for p in plans])
+    for q in L.RESIDUE_PRIMES:
+        assert L.residue_mod(outs[0], q) == L.residue_mod(a, q) * L.residue_mod(b, q) % q
+    for p, o in zip(plans[1:], outs[1:]):
+        assert np.array_equal(o, outs[0]), p
