@@ -330,18 +330,20 @@ public:
         return BI_HIP_OK;
     }
 
+    // One product.  planned: the engine's plan choice (table, else heuristic), which may chunk or
+    // re-split; otherwise plan 0 (one unchunked transform with a 4096-point block pass).
     int run(const uint32_t* d_a, uint32_t la, const uint32_t* d_b, uint32_t lb, bool square,
-            uint32_t* d_out, hipStream_t st) {
+            uint32_t* d_out, hipStream_t st, bool planned = true) {
         const int lg0 = bi_hip_transform_log2(la, lb);
         if (lg0 < 0) return BI_HIP_ETOOBIG;
         // the table (or the heuristic) may pick a chunked or re-split plan: those run as a batch
         const bi_hip_dpair p{d_a, la, square ? nullptr : d_b, lb, d_out, (size_t)la + lb};
         std::vector<PairPlan> chosen;
-        {
+        if (planned) {
             const int rc = choose_plans(&p, 1, BI_HIP_PLAN_TABLE, nullptr, chosen, st);
             if (rc) return rc;
         }
-        if (chosen[0].chunk_len || chosen[0].mid_lr < std::min<uint32_t>(12, (uint32_t)lg0))
+        if (planned && (chosen[0].chunk_len || chosen[0].mid_lr < std::min<uint32_t>(12, (uint32_t)lg0)))
             return run_batch(&p, 1, BI_HIP_PLAN_TABLE, nullptr, st);
         const uint32_t lg = (uint32_t)lg0;
         Plan* pl = plan(lg);
@@ -355,7 +357,7 @@ public:
         uint32_t* res_a = w;
         uint32_t* res_b = square ? nullptr : w + (size_t)3 * n;
         uint32_t* status = w + (size_t)3 * n * (square ? 1 : 2);
-        const PassJob job{d_a, square ? d_a : d_b, la, square ? la : lb, res_a, res_b};
+        const PassJob job{d_a, square ? d_a : d_b, la, square ? la : lb, res_a, res_b, 0u, ~0u};
         GarnerJob gj{};
         gj.y = res_a;
         gj.out = d_out;
@@ -364,6 +366,12 @@ public:
         for (int I = 0; I < 3; ++I) gj.kscale[I] = pl->kscale[I];
         gj.status = status;
         gj.counter = status + ntiles;
+        if ((uint64_t)words_of(la) + words_of(lb) - 1 > n) {
+            gj.wrap_a = d_a;
+            gj.wrap_b = square ? d_a : d_b;
+            gj.wrap_la = la;
+            gj.wrap_lb = lb;
+        }
         return launch_products(pl, square ? kSquareKind : kPlain, 1, job, nullptr, 1, gj, nullptr, out_len,
                                (double)la + (square ? 0.0 : (double)lb), (double)out_len, status,
                                ntiles + 1, st);
@@ -378,7 +386,7 @@ public:
     // A chunked product transforms the shorter operand once and reuses it for every chunk.
     struct PairPlan {
         uint32_t lg;         // log2 of the transform length
-        uint32_t chunk_len;  // 0: ordinary product; else limbs per chunk of the longer operand
+        uint32_t chunk_len;  // 0: ordinary product; else words per chunk of the longer operand
         uint32_t mid_lr;     // log2 of the block-pass size
     };
     static constexpr int kNumPlanIds = 8;
@@ -405,7 +413,8 @@ public:
             return true;
         }
         if (sq) return false;
-        const uint32_t hi = std::max(la, lb), lo = std::min(la, lb);
+        // chunking works on the operands' 32-bit transform words
+        const uint32_t hi = words_of(std::max(la, lb)), lo = words_of(std::min(la, lb));
         uint32_t L;
         if (id <= 5) {
             uint32_t c = std::max<uint32_t>(10, ceil_log2(lo)) + (uint32_t)(id - 3);
@@ -414,7 +423,7 @@ public:
         } else {
             const uint32_t parts = (uint32_t)id - 4;  // 2 or 3
             L = (hi + parts - 1) / parts;
-            L = (L + 4095u) & ~4095u;
+            L = (L + kCarryTileWords - 1) / kCarryTileWords * kCarryTileWords;
         }
         if (L >= hi) return false;
         const uint32_t lg = std::max<uint32_t>(4, ceil_log2((uint64_t)L + lo - 1));
@@ -433,7 +442,7 @@ public:
         PairPlan best{};
         plan_for_id(la, lb, sq, 0, best);
         double best_cost = (sq ? 2.0 : 3.0) * std::ldexp((double)best.lg, (int)best.lg);
-        const uint32_t hi = std::max(la, lb);
+        const uint32_t hi = words_of(std::max(la, lb));
         for (int id = 3; id < kNumPlanIds; ++id) {
             PairPlan c{};
             if (!plan_for_id(la, lb, sq, id, c)) continue;
@@ -600,7 +609,7 @@ public:
             if (it.pp.chunk_len) {
                 it.swap = it.la < it.lb;
                 if (it.swap) std::swap(it.la, it.lb);
-                it.nchunks = (it.la + it.pp.chunk_len - 1) / it.pp.chunk_len;
+                it.nchunks = (words_of(it.la) + it.pp.chunk_len - 1) / it.pp.chunk_len;
                 it.bhat = bhat_words;
                 bhat_words += (size_t)3 << lg;
                 groups[{0, lg, mid, kBForward}].push_back((uint32_t)i);
@@ -676,7 +685,7 @@ public:
                 const uint32_t* A = it.swap ? p.b : p.a;
                 const uint32_t* B = it.sq ? p.a : (it.swap ? p.a : p.b);
                 if (kind == kBForward) {
-                    pj[jp++] = PassJob{B, nullptr, it.lb, 0, w + it.bhat, nullptr};
+                    pj[jp++] = PassJob{B, nullptr, it.lb, 0, w + it.bhat, nullptr, 0u, ~0u};
                     L.in_limbs += (double)it.lb;
                     continue;
                 }
@@ -685,8 +694,7 @@ public:
                 if (kind == kChunk) {
                     const uint32_t Lc = it.pp.chunk_len;
                     for (uint32_t c = 0; c < it.nchunks; ++c) {
-                        const uint32_t lac = std::min(Lc, it.la - c * Lc);
-                        pj[jp++] = PassJob{A + (size_t)c * Lc, nullptr, lac, it.lb, res, w + it.bhat};
+                        pj[jp++] = PassJob{A, nullptr, it.la, it.lb, res, w + it.bhat, c * Lc, Lc};
                         res += 3 * n;
                     }
                     g.nchunks = it.nchunks;
@@ -701,8 +709,14 @@ public:
                         rb = res;
                         res += 3 * n;
                     }
-                    pj[jp++] = PassJob{A, B, it.la, it.lb, ra, rb};
+                    pj[jp++] = PassJob{A, B, it.la, it.lb, ra, rb, 0u, ~0u};
                     g.nchunks = 1;
+                    if ((uint64_t)words_of(it.la) + words_of(it.lb) - 1 > n) {
+                        g.wrap_a = A;
+                        g.wrap_b = B;
+                        g.wrap_la = it.la;
+                        g.wrap_lb = it.lb;
+                    }
                 }
                 const uint32_t out_len = it.la + it.lb;
                 const uint32_t nt = (out_len + 4095u) / 4096u;
@@ -764,8 +778,12 @@ int bi_hip_init(int device) {
 }
 
 int bi_hip_transform_log2(size_t la, size_t lb) {
-    if (la == 0 || lb == 0) return -1;
-    const size_t need = la + lb - 1;
+    if (la == 0 || lb == 0 || la > 0xFFFFFFFFull || lb > 0xFFFFFFFFull) return -1;
+    // The transforms run over 32-bit words of the operands (8 limbs of 28 bits = 7 words).  The
+    // product has wa + wb - 1 coefficients; the top one may wrap onto coefficient 0 (Garner
+    // separates them again) as long as both operands fit the transform.
+    const size_t wa = (la * 7 + 7) / 8, wb = (lb * 7 + 7) / 8;
+    const size_t need = std::max(wa + wb - 2, std::max(wa, wb));
     if (need > ((size_t)1 << bi::kMaxLog)) return -1;
     int lg = 4;  // at least one 16-point tile
     while (((size_t)1 << lg) < need) ++lg;
@@ -844,7 +862,7 @@ int bi_hip_mul_batch_device(const bi_hip_dpair* pairs, size_t count, int plan_mo
             const size_t lb = sq ? p.la : p.lb;
             if (!p.a || !p.out || p.la == 0 || lb == 0) return BI_HIP_EINVAL;
             if (p.out_cap < p.la + lb) return BI_HIP_ECAP;
-            rc = e.run(p.a, (uint32_t)p.la, sq ? p.a : p.b, (uint32_t)lb, sq, p.out, st);
+            rc = e.run(p.a, (uint32_t)p.la, sq ? p.a : p.b, (uint32_t)lb, sq, p.out, st, false);
             if (rc) return rc;
         }
         return BI_HIP_OK;

@@ -165,11 +165,58 @@ __device__ __forceinline__ void lds_io(uint32_t* lds, uint32_t (&x)[kE]) {
     }
 }
 
+// Transform input word g of an operand: bits [32g, 32g + 32) of its limb string.  The bits lie
+// in limbs k = floor(32g / 28) and k + 1, from bit o = 32g - 28k (o in {0, 4, ..., 24}) of limb k;
+// limbs at and past nlimbs read as zero.
+__device__ __forceinline__ uint32_t limb_word(const uint32_t* __restrict__ limbs, uint32_t nlimbs,
+                                              uint32_t g) {
+    const uint32_t k = (g * 8u) / 7u;
+    const uint32_t o = g * 32u - k * 28u;
+    const uint32_t lo = k < nlimbs ? limbs[k] : 0u;
+    const uint32_t hi = k + 1u < nlimbs ? limbs[k + 1u] : 0u;
+    return (lo >> o) | (hi << (28u - o));
+}
+
+// A 32-bit input word as a residue: below P for P1 and P2, below 4P for P0 (the range of the
+// lazy forward butterflies).  2^32 < 9.2 P0 < 2.2 P1 < 1.9 P2.
+template <int I>
+__device__ __forceinline__ uint32_t word_residue(uint32_t x) {
+    if constexpr (I == 0) {
+        constexpr uint32_t p4 = 4u * kP[0];
+        uint32_t y = x - p4;
+        x = y < x ? y : x;  // < 5.2 P0
+        y = x - p4;
+        return y < x ? y : x;  // < 4 P0
+    } else if constexpr (I == 1) {
+        constexpr uint32_t p2 = 2u * kP[1];
+        uint32_t y = x - p2;
+        x = y < x ? y : x;  // < 2 P1
+        y = x - kP[1];
+        return y < x ? y : x;
+    } else {
+        const uint32_t y = x - kP[2];
+        return y < x ? y : x;
+    }
+}
+
+template <int I>
+__device__ __forceinline__ void words_to_residues(uint32_t (&x)[kE]) {
+#pragma unroll
+    for (int k = 0; k < kE; ++k) x[k] = word_residue<I>(x[k]);
+}
+
+// Where a forward transform's input comes from when it is read from limbs: words
+// [off, off + cnt) of the operand (a chunk, or the whole operand with cnt = ~0u).
+struct LimbSrc {
+    const uint32_t* limbs;  // null: read the residue plane instead
+    uint32_t nlimbs, off, cnt;
+};
+
 // Global memory <-> registers in a group layout.  Element (r, c) of the tile is at
-// tile_base + c + (r << log_s) of the plane; limb input reads zero at and past `len`.
+// tile_base + c + (r << log_s) of the plane; with a limb source, element idx is input word
+// src.off + idx (zero at and past src.cnt).
 template <int LR, int LC, int ST, int TL, bool STORE>
-__device__ __forceinline__ void global_io(uint32_t* __restrict__ plane,
-                                          const uint32_t* __restrict__ limbs, uint32_t len,
+__device__ __forceinline__ void global_io(uint32_t* __restrict__ plane, const LimbSrc& src,
                                           uint32_t (&x)[kE], uint32_t tile_base, uint32_t log_s) {
     using Ly = Layout<LR, LC, ST, TL>;
 #pragma unroll
@@ -179,7 +226,8 @@ __device__ __forceinline__ void global_io(uint32_t* __restrict__ plane,
         for (int q = 0; q < Ly::EG; ++q) {
             const uint32_t idx = tile_base + c + (Ly::row(sg, q) << log_s);
             if (STORE) plane[idx] = x[sg * Ly::EG + q];
-            else if (limbs) x[sg * Ly::EG + q] = idx < len ? limbs[idx] : 0u;
+            else if (src.limbs)
+                x[sg * Ly::EG + q] = idx < src.cnt ? limb_word(src.limbs, src.nlimbs, src.off + idx) : 0u;
             else x[sg * Ly::EG + q] = plane[idx];
         }
     }
@@ -226,24 +274,25 @@ __device__ __forceinline__ void outer_body(const PassArgs& a, const PassJob& job
     build_twiddles<I, LR>(tw, bpow, wt, beta, kThreads);
 
     uint32_t* plane = (op == 0 ? job.res_a : job.res_b) + (size_t)I * n;
-    const uint32_t* limbs = nullptr;
-    uint32_t len = 0;
+    LimbSrc src{nullptr, 0u, 0u, 0u};
     if (!INV && (a.flags & kFromLimbs)) {
-        limbs = op == 0 ? job.limbs_a : job.limbs_b;
-        len = op == 0 ? job.la : job.lb;
+        src = op == 0 ? LimbSrc{job.limbs_a, job.la, job.a_off, job.a_cnt}
+                      : LimbSrc{job.limbs_b, job.lb, 0u, ~0u};
     }
+    constexpr LimbSrc kPlane{nullptr, 0u, 0u, 0u};
     constexpr int G0 = 0, GL = S::NG - 1;
     uint32_t x[kE];
     if (!INV) {
-        global_io<LR, LC, S::stages(G0), S::trlo(G0), false>(plane, limbs, len, x, tile_base, a.log_s);
+        global_io<LR, LC, S::stages(G0), S::trlo(G0), false>(plane, src, x, tile_base, a.log_s);
+        if (src.limbs) words_to_residues<I>(x);
         __syncthreads();
         tile_transform<I, false, LR, LC>(x, lds, tw);
-        global_io<LR, LC, S::stages(GL), S::trlo(GL), true>(plane, nullptr, 0, x, tile_base, a.log_s);
+        global_io<LR, LC, S::stages(GL), S::trlo(GL), true>(plane, kPlane, x, tile_base, a.log_s);
     } else {
-        global_io<LR, LC, S::stages(GL), S::trlo(GL), false>(plane, nullptr, 0, x, tile_base, a.log_s);
+        global_io<LR, LC, S::stages(GL), S::trlo(GL), false>(plane, kPlane, x, tile_base, a.log_s);
         __syncthreads();
         tile_transform<I, true, LR, LC>(x, lds, tw);
-        global_io<LR, LC, S::stages(G0), S::trlo(G0), true>(plane, nullptr, 0, x, tile_base, a.log_s);
+        global_io<LR, LC, S::stages(G0), S::trlo(G0), true>(plane, kPlane, x, tile_base, a.log_s);
     }
 }
 
@@ -297,22 +346,25 @@ __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, 
     constexpr int G0 = 0;
     constexpr int GL = S::NG - 1;
     uint32_t xa[kE];
-    global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pa, from_limbs ? job.limbs_a : nullptr,
-                                                        job.la, xa, tile_base, 0);
+    constexpr LimbSrc kPlane{nullptr, 0u, 0u, 0u};
+    const LimbSrc src_a = from_limbs ? LimbSrc{job.limbs_a, job.la, job.a_off, job.a_cnt} : kPlane;
+    global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pa, src_a, xa, tile_base, 0);
+    if (from_limbs) words_to_residues<I>(xa);
     __syncthreads();  // forward twiddles ready
     tile_transform<I, false, LR, 0>(xa, lds, tw);
     if (a.flags & kFwdOnly) {  // store the transform in the last group's layout
-        global_io<LR, 0, S::stages(GL), S::trlo(GL), true>(pa, nullptr, 0, xa, tile_base, 0);
+        global_io<LR, 0, S::stages(GL), S::trlo(GL), true>(pa, kPlane, xa, tile_base, 0);
         return;
     }
     if (!square) {
         uint32_t* pb = job.res_b + (size_t)I * n;
         uint32_t xb[kE];
         if (a.flags & kBPre) {
-            global_io<LR, 0, S::stages(GL), S::trlo(GL), false>(pb, nullptr, 0, xb, tile_base, 0);
+            global_io<LR, 0, S::stages(GL), S::trlo(GL), false>(pb, kPlane, xb, tile_base, 0);
         } else {
-            global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pb, from_limbs ? job.limbs_b : nullptr,
-                                                                job.lb, xb, tile_base, 0);
+            const LimbSrc src_b = from_limbs ? LimbSrc{job.limbs_b, job.lb, 0u, ~0u} : kPlane;
+            global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pb, src_b, xb, tile_base, 0);
+            if (from_limbs) words_to_residues<I>(xb);
             tile_transform<I, false, LR, 0>(xb, lds, tw);
         }
 #pragma unroll
@@ -326,7 +378,7 @@ __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, 
     build_twiddles<I, LR>(tw, bpow, wi, bi_, T);
     __syncthreads();
     tile_transform<I, true, LR, 0>(xa, lds, tw);
-    global_io<LR, 0, S::stages(G0), S::trlo(G0), true>(pa, nullptr, 0, xa, tile_base, 0);
+    global_io<LR, 0, S::stages(G0), S::trlo(G0), true>(pa, kPlane, xa, tile_base, 0);
 }
 
 template <int LR>
@@ -345,12 +397,14 @@ __global__ __launch_bounds__(1 << (LR - 4)) void k_mid_pass(PassArgs a) {
 // ---------------------------------------------------------------------------------------------
 // Garner CRT + carry propagation (decoupled look-back).
 //
-// Coefficient j of the convolution is recovered from the three inverse-transform outputs y_p
-// (which carry a factor n * R^-1) as x_j < 2^84 and split into 28-bit pieces d0, d1, d2.
-// Output limb i = d0_i + d1_{i-1} + d2_{i-2} + carry_i, the weights of
-// ntt_multithread.cpp:413-442.  Each sum is below 3*2^28, so carry_i is in {0,1,2,3} and the
-// carry-in -> carry-out behaviour of any run of limbs is a map {0..3} -> {0..3}, stored as four
-// 2-bit fields.  Maps compose associatively, which makes the carry chain a scan.
+// Coefficient j of the convolution of the operands' 32-bit words is recovered from the three
+// inverse-transform outputs y_p (which carry a factor n * R^-1) as x_j < 2^89 (at most 2^25 terms
+// below 2^64 each; P0 P1 P2 > 2^90.8) and split into 32-bit pieces w0, w1, w2.  Output word i of
+// the product is w0_i + w1_{i-1} + w2_{i-2} + carry_i: each sum is below 3 * 2^32, so carry_i is
+// in {0, 1, 2} and the carry-in -> carry-out behaviour of any run of words is a map
+// {0..3} -> {0..3}, stored as four 2-bit fields.  Maps compose associatively, which makes the
+// carry chain a scan.  The product's words are finally cut back into 28-bit limbs: a thread owns
+// 14 words = 16 limbs, a tile 3584 words = 4096 limbs, so limbs never straddle threads.
 
 __device__ __forceinline__ uint32_t map_apply(uint32_t m, uint32_t c) { return (m >> (2u * c)) & 3u; }
 // (second after first): apply `first`, then `second`
@@ -363,16 +417,23 @@ __device__ __forceinline__ uint32_t map_compose(uint32_t second, uint32_t first)
 constexpr uint32_t kMapIdentity = 0b11100100u;  // c -> c
 
 struct Pieces {
-    uint32_t d0, d1, d2;
+    uint32_t w0, w1, w2;  // x = w0 + w1 2^32 + w2 2^64
 };
 
-// Coefficient of one product from its three residues (at plane offset j of the planes at y).
-__device__ __forceinline__ unsigned __int128 crt_coeff(const uint32_t* __restrict__ y, uint32_t n,
+// [0, 2 P0) -> [0, P0)
+__device__ __forceinline__ uint32_t fold1p(uint32_t x) {
+    const uint32_t y = x - kP[0];
+    return y < x ? y : x;
+}
+
+// Coefficient from its three inverse-transform outputs r0, r1, r2 (each still carrying the
+// factor n * R^-1, removed by the Garner constants ks); r0 may lie in [0, 2 P0).
+__device__ __forceinline__ unsigned __int128 crt_coeff(uint32_t r0, uint32_t r1, uint32_t r2,
                                                        const uint32_t (&ks)[3],
-                                                       const GarnerConsts& k, uint32_t j) {
-    const uint32_t c0 = mont_mul<0>(y[j], ks[0]);
-    const uint32_t c1 = mont_mul<1>(y[n + j], ks[1]);
-    const uint32_t c2 = mont_mul<2>(y[2u * n + j], ks[2]);
+                                                       const GarnerConsts& k) {
+    const uint32_t c0 = mont_mul<0>(r0, ks[0]);
+    const uint32_t c1 = mont_mul<1>(r1, ks[1]);
+    const uint32_t c2 = mont_mul<2>(r2, ks[2]);
     const uint32_t v0 = c0;
     const uint32_t v1 = mont_mul<1>(sub_mod<1>(c1, v0), k.inv01);
     const uint32_t t = mont_mul<2>(sub_mod<2>(c2, v0), k.inv02);
@@ -381,36 +442,60 @@ __device__ __forceinline__ unsigned __int128 crt_coeff(const uint32_t* __restric
     return (unsigned __int128)inner * kP[0] + v0;
 }
 
-// Convolution coefficient j of the job, split into 28-bit pieces.  For a chunked product the
-// coefficient is the sum of the (at most two) chunk products covering j; every such sum stays
-// below n * 2^56 <= 2^82.
+// The coefficient that wrapped from index n onto index 0: product of the operands' top words.
+__device__ __forceinline__ uint64_t wrap_product(const GarnerJob& jb) {
+    const uint32_t ga = words_of(jb.wrap_la) - 1u, gb = words_of(jb.wrap_lb) - 1u;
+    return (uint64_t)limb_word(jb.wrap_a, jb.wrap_la, ga) * limb_word(jb.wrap_b, jb.wrap_lb, gb);
+}
+
+// Convolution coefficient j of the job, split into 32-bit pieces.  For a chunked product the
+// coefficient is the sum of the chunk products covering j.  CRT is linear, so the chunk
+// residues are summed mod each prime first and reconstructed once: the true sum is a
+// coefficient of the whole product, below 2^89 < P0*P1*P2.
 __device__ __forceinline__ Pieces garner(const GarnerJob& jb, const GarnerConsts& k, uint32_t j,
                                          uint32_t tile_chunk) {
-    unsigned __int128 x = 0;
+    uint32_t r0, r1, r2;
+    const uint32_t n = jb.n;
     if (jb.nchunks <= 1) {
-        if (j >= jb.n) return {0u, 0u, 0u};
-        x = crt_coeff(jb.y, jb.n, jb.kscale, k, j);
+        if (j >= n) {
+            if (j != n || !jb.wrap_a) return {0u, 0u, 0u};
+            const uint64_t w = wrap_product(jb);
+            return {(uint32_t)w, (uint32_t)(w >> 32), 0u};
+        }
+        r0 = jb.y[j];
+        r1 = jb.y[n + j];
+        r2 = jb.y[2u * n + j];
+        if (j == 0 && jb.wrap_a) {
+            const unsigned __int128 x = crt_coeff(r0, r1, r2, jb.kscale, k) - wrap_product(jb);
+            const uint64_t lo = (uint64_t)x;
+            return Pieces{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)(x >> 64)};
+        }
     } else {
+        r0 = r1 = r2 = 0;
         // primary chunk of j; earlier chunks whose products reach j add in
         const uint32_t c = jb.log_chunk < 32u ? (j >> jb.log_chunk) : tile_chunk;
         for (uint32_t d = 0; d <= c; ++d) {
             const uint32_t cc = c - d;
             const uint32_t off = j - cc * jb.chunk_len;
-            if (off >= jb.n) break;
-            if (cc < jb.nchunks) x += crt_coeff(jb.y + (size_t)cc * jb.chunk_stride, jb.n, jb.kscale, k, off);
+            if (off >= n) break;
+            if (cc < jb.nchunks) {
+                const uint32_t* y = jb.y + (size_t)cc * jb.chunk_stride;
+                r0 = add_mod<0>(r0, fold1p(y[off]));  // P0 outputs lie in [0, 2P)
+                r1 = add_mod<1>(r1, y[n + off]);
+                r2 = add_mod<2>(r2, y[2u * n + off]);
+            }
         }
     }
+    const unsigned __int128 x = crt_coeff(r0, r1, r2, jb.kscale, k);
     const uint64_t lo = (uint64_t)x;
-    const uint64_t hi = (uint64_t)(x >> 64);
-    Pieces p;
-    p.d0 = (uint32_t)lo & kDigitMask;
-    p.d1 = (uint32_t)(lo >> 28) & kDigitMask;
-    p.d2 = (uint32_t)((lo >> 56) | (hi << 8)) & kDigitMask;
-    return p;
+    return Pieces{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)(x >> 64)};
 }
 
 constexpr uint32_t kCarryLimbsPerThread = 16;
+constexpr uint32_t kCarryWordsPerThread = 14;  // 14 * 32 = 16 * 28 bits
 constexpr uint32_t kCarryTile = kThreads * kCarryLimbsPerThread;  // 4096 limbs per tile
+constexpr uint32_t kWordTile = kThreads * kCarryWordsPerThread;   // 3584 words per tile
+static_assert(kCarryTile == kCarryTileLimbs && kWordTile == kCarryTileWords, "tile geometry");
 
 // Status words carry their whole payload (flag << 8 | map or carry), so they are published and
 // polled with relaxed agent-scope atomics: no acquire/release fence, hence no L2 write-back or
@@ -426,8 +511,8 @@ constexpr uint32_t kAgg = 1u, kInc = 2u;
 __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     const GarnerJob& jb = a.jobs ? a.jobs[blockIdx.y] : a.job;
     // pieces of the tile's coefficients, padded one word per 16 against bank conflicts
-    __shared__ uint32_t s_d[3][kCarryTile + kCarryTile / 16];
-    __shared__ uint32_t s_halo[4];  // d1, d2 of coefficient base-1; d2 of base-2
+    __shared__ uint32_t s_d[3][kWordTile + kWordTile / 16];
+    __shared__ uint32_t s_halo[4];  // w1, w2 of coefficient base-1; w2 of base-2
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave[kThreads / 64];
     __shared__ uint32_t s_carry_in;
@@ -437,61 +522,64 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     const uint32_t tile = s_tile;
     const uint32_t ntiles = (jb.out_len + kCarryTile - 1u) / kCarryTile;
     if (tile >= ntiles) return;
-    const uint32_t base = tile * kCarryTile;
+    const uint32_t base = tile * kWordTile;  // first word (coefficient) of the tile
     // chunk lengths that are multiples of the tile: the whole tile has one primary chunk
     const uint32_t tile_chunk = (jb.nchunks > 1 && jb.log_chunk == 0xFFu) ? base / jb.chunk_len : 0u;
 
     // 1. Garner for the tile's coefficients, coalesced: thread tid takes base + tid + 256 q
-#pragma unroll 4
-    for (uint32_t q = 0; q < kCarryLimbsPerThread; ++q) {
+#pragma unroll 2
+    for (uint32_t q = 0; q < kCarryWordsPerThread; ++q) {
         const uint32_t o = tid + kThreads * q;
         const Pieces p = garner(jb, a.k, base + o, tile_chunk);
         const uint32_t li = o + (o >> 4);
-        s_d[0][li] = p.d0;
-        s_d[1][li] = p.d1;
-        s_d[2][li] = p.d2;
+        s_d[0][li] = p.w0;
+        s_d[1][li] = p.w1;
+        s_d[2][li] = p.w2;
     }
-    if (tid < 2) {  // the previous tile's last two coefficients feed limbs base and base+1
+    if (tid < 2) {  // the previous tile's last two coefficients feed words base and base+1
         // the halo lies in the previous tile, hence possibly in the previous chunk
         const uint32_t j = base - 2 + tid;
         const uint32_t hc = (jb.log_chunk == 0xFFu && jb.nchunks > 1) ? j / jb.chunk_len : tile_chunk;
         const Pieces p = base >= 2 - tid ? garner(jb, a.k, j, hc) : Pieces{0u, 0u, 0u};
         if (tid == 1) {
-            s_halo[0] = p.d1;
-            s_halo[1] = p.d2;
+            s_halo[0] = p.w1;
+            s_halo[1] = p.w2;
         } else {
-            s_halo[2] = p.d2;
+            s_halo[2] = p.w2;
         }
     }
     __syncthreads();
 
-    // 2. limb sums s_i = d0_i + d1_{i-1} + d2_{i-2}, 16 consecutive limbs per thread
-    uint32_t s[kCarryLimbsPerThread];
-    const uint32_t o0 = tid * kCarryLimbsPerThread;
+    // 2. word sums s_i = w0_i + w1_{i-1} + w2_{i-2} < 3 * 2^32, 14 consecutive words per thread
+    uint32_t s_lo[kCarryWordsPerThread], s_hi[kCarryWordsPerThread];
+    const uint32_t o0 = tid * kCarryWordsPerThread;
 #pragma unroll
-    for (uint32_t q = 0; q < kCarryLimbsPerThread; ++q) {
+    for (uint32_t q = 0; q < kCarryWordsPerThread; ++q) {
         const uint32_t o = o0 + q;
-        const uint32_t d0 = s_d[0][o + (o >> 4)];
-        uint32_t d1, d2;
-        if (o >= 1) d1 = s_d[1][(o - 1) + ((o - 1) >> 4)];
-        else d1 = s_halo[0];
-        if (o >= 2) d2 = s_d[2][(o - 2) + ((o - 2) >> 4)];
-        else d2 = (o == 1) ? s_halo[1] : s_halo[2];
-        s[q] = d0 + d1 + d2;
+        const uint32_t w0 = s_d[0][o + (o >> 4)];
+        uint32_t w1, w2;
+        if (o >= 1) w1 = s_d[1][(o - 1) + ((o - 1) >> 4)];
+        else w1 = s_halo[0];
+        if (o >= 2) w2 = s_d[2][(o - 2) + ((o - 2) >> 4)];
+        else w2 = (o == 1) ? s_halo[1] : s_halo[2];
+        const uint64_t sum = (uint64_t)w0 + w1 + w2;
+        s_lo[q] = (uint32_t)sum;
+        s_hi[q] = (uint32_t)(sum >> 32);
     }
     // 3. this thread's carry map (carry-out for each carry-in 0..3, four chains run side by
     //    side), then an inclusive scan of maps across the block
     uint32_t m;
     {
-        uint32_t c0 = 0, c1 = 1, c2 = 2, c3 = 3;
+        uint32_t c[4] = {0u, 1u, 2u, 3u};
 #pragma unroll
-        for (uint32_t q = 0; q < kCarryLimbsPerThread; ++q) {
-            c0 = (s[q] + c0) >> kDigitBits;
-            c1 = (s[q] + c1) >> kDigitBits;
-            c2 = (s[q] + c2) >> kDigitBits;
-            c3 = (s[q] + c3) >> kDigitBits;
+        for (uint32_t q = 0; q < kCarryWordsPerThread; ++q) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t v = s_lo[q] + c[k];
+                c[k] = s_hi[q] + (v < c[k] ? 1u : 0u);
+            }
         }
-        m = c0 | (c1 << 2) | (c2 << 4) | (c3 << 6);
+        m = c[0] | (c[1] << 2) | (c[2] << 4) | (c[3] << 6);
     }
     const uint32_t lane = tid & 63u, wave = tid >> 6;
     uint32_t inc = m;
@@ -548,16 +636,24 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     }
     __syncthreads();
 
-    // 5. apply the carries and write the limbs
+    // 5. apply the carries, cut the 14 words into 16 limbs and write them
     uint32_t carry = map_apply(excl, s_carry_in);
+    uint32_t wv[kCarryWordsPerThread];
+#pragma unroll
+    for (uint32_t q = 0; q < kCarryWordsPerThread; ++q) {
+        const uint32_t v = s_lo[q] + carry;
+        carry = s_hi[q] + (v < carry ? 1u : 0u);
+        wv[q] = v;
+    }
     uint32_t outv[kCarryLimbsPerThread];
 #pragma unroll
     for (uint32_t q = 0; q < kCarryLimbsPerThread; ++q) {
-        const uint32_t v = s[q] + carry;
+        const uint32_t bit = 28u * q, wi = bit >> 5, bo = bit & 31u;
+        uint32_t v = wv[wi] >> bo;
+        if (bo > 4u) v |= wv[wi + 1] << (32u - bo);
         outv[q] = v & kDigitMask;
-        carry = v >> kDigitBits;
     }
-    const uint32_t i0 = base + o0;
+    const uint32_t i0 = tile * kCarryTile + tid * kCarryLimbsPerThread;
     if (i0 + kCarryLimbsPerThread <= jb.out_len && ((reinterpret_cast<uintptr_t>(jb.out + i0) & 15u) == 0)) {
         uint4* dst = reinterpret_cast<uint4*>(jb.out + i0);
 #pragma unroll

@@ -7,20 +7,26 @@
 namespace bi {
 
 enum PassFlags : uint32_t {
-    kFromLimbs = 1u,  // the pass reads 28-bit limbs (zero past la/lb) instead of residues
+    kFromLimbs = 1u,  // the pass reads the operands' 32-bit words from their limbs
     kSquare = 2u,     // B is A: one forward transform, pointwise square
     kSingle = 4u,     // strided forward passes transform operand A only
     kBPre = 8u,       // block pass: res_b already holds B's full forward transform
     kFwdOnly = 16u,   // block pass: forward-transform A and store it (no product, no inverse)
 };
 
+// Transform inputs are 32-bit words of the operands' bit strings (word g = bits [32g, 32g+32)),
+// cut from the 28-bit limbs on load: 8 limbs make 7 words.
+constexpr uint32_t words_of(uint32_t limbs) { return (uint32_t)(((uint64_t)limbs * 7u + 7u) / 8u); }
+
 // One product inside a batched launch.  Residue buffers are planar: 3 planes of n u32.
 struct PassJob {
     const uint32_t* limbs_a;
     const uint32_t* limbs_b;
-    uint32_t la, lb;
+    uint32_t la, lb;      // limb counts of the whole operands
     uint32_t* res_a;
     uint32_t* res_b;
+    uint32_t a_off;       // A's transform takes words [a_off, a_off + a_cnt) (a chunk of A)
+    uint32_t a_cnt;       // ~0u: all of A
 };
 
 struct PassArgs {
@@ -40,20 +46,30 @@ struct GarnerConsts {
     uint32_t inv01, inv02, inv12;  // P0^-1 mod P1, P0^-1 mod P2, P1^-1 mod P2 (Montgomery form)
 };
 
+// Output tiles of the Garner + carry kernel: 4096 limbs = 3584 words (114688 bits).
+constexpr uint32_t kCarryTileLimbs = 4096;
+constexpr uint32_t kCarryTileWords = 3584;
+
 struct GarnerJob {
     const uint32_t* y;   // 3 planes of n: inverse transform output (times n * R^-1)
     uint32_t* out;       // out_len limbs
     uint32_t n, out_len;
     uint32_t kscale[3];  // n^-1 * R^2 mod P_i, so mont_mul(y, kscale) = y * n^-1 * R
-    uint32_t* status;    // look-back status words, one per 4096-limb tile, zeroed before launch
+    uint32_t* status;    // look-back status words, one per output tile, zeroed before launch
     uint32_t* counter;   // dynamic tile counter, zeroed before launch
-    // Chunked product: the long operand was cut into nchunks pieces of chunk_len limbs, each
-    // multiplied with a transform of length n >= chunk_len + lb - 1; chunk c's residues start at
-    // y + c * chunk_stride and its product lands at limb offset c * chunk_len.  chunk_len is a
-    // power of two (log_chunk < 32) or a multiple of 4096 (log_chunk == 0xFF).  nchunks <= 1
-    // means an ordinary product.
+    // Chunked product: the long operand was cut into nchunks pieces of chunk_len words, each
+    // multiplied with a transform of length n >= chunk_len + (words of B) - 1; chunk c's
+    // residues start at y + c * chunk_stride and its product lands at word offset c * chunk_len.
+    // chunk_len is a power of two (log_chunk < 32) or a multiple of kCarryTileWords
+    // (log_chunk == 0xFF).  nchunks <= 1 means an ordinary product.
     uint32_t nchunks, log_chunk, chunk_len;
     uint64_t chunk_stride;
+    // Unchunked products may use n = (words of A) + (words of B) - 2: the one coefficient past
+    // the transform, (top word of A) * (top word of B) at index n, wraps onto index 0.  Garner
+    // takes it back out of coefficient 0 and emits it at index n.  wrap_a == null: no wrap.
+    const uint32_t* wrap_a;
+    const uint32_t* wrap_b;
+    uint32_t wrap_la, wrap_lb;
 };
 
 struct GarnerArgs {

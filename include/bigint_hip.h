@@ -37,7 +37,7 @@ extern "C" {
 enum {
     BI_HIP_OK = 0,
     BI_HIP_EINVAL = 1,   /* bad argument: null pointer, zero length, top limb zero, limb >= 2^28 */
-    BI_HIP_ETOOBIG = 2,  /* la + lb - 1 > 2^26 (beyond the three primes' transform length) */
+    BI_HIP_ETOOBIG = 2,  /* product beyond a 2^26-point transform (see bi_hip_transform_log2) */
     BI_HIP_ENOMEM = 3,   /* device or host allocation failed */
     BI_HIP_EDEVICE = 4,  /* HIP runtime error */
     BI_HIP_ENODEV = 5,   /* no gfx950 device */
@@ -46,7 +46,7 @@ enum {
 
 /* Plan modes for bi_hip_mul_batch. */
 enum {
-    BI_HIP_PLAN_FIXED = 0,   /* one launch sequence per product, default transform plan */
+    BI_HIP_PLAN_FIXED = 0,   /* one launch sequence per product, plan 0 (no chunking) */
     BI_HIP_PLAN_TABLE = 1,   /* autotuned plan table */
     BI_HIP_PLAN_LEARNED = 2  /* learned cost model, table as fallback */
 };
@@ -75,7 +75,9 @@ int bi_hip_mul(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, int i
 
 /* Device-resident variant: a, b, out are device pointers; exactly la + lb limbs are written
  * (the top one may be zero).  Asynchronous on `stream` (a hipStream_t; NULL = the engine's
- * stream). */
+ * stream), except that a chunked plan waits for earlier work on the stream before it reuses the
+ * engine's descriptor buffer.  Both single-product calls use the engine's plan choice (the
+ * installed table, else the built-in cost heuristic). */
 int bi_hip_mul_device(const uint32_t* d_a, size_t la, const uint32_t* d_b, size_t lb,
                       int is_square, uint32_t* d_out, size_t out_cap, void* stream);
 
@@ -103,7 +105,12 @@ int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode);
 /* Wait for all work queued by the engine. */
 int bi_hip_synchronize(void);
 
-/* Transform length (log2) the engine uses for an la x lb product, or -1 if too large. */
+/* Transform length (log2) of an unchunked la x lb product, or -1 if too large.  The transforms
+ * run over the operands' 32-bit words, wa = ceil(7 la / 8) of them (8 limbs = 7 words), and the
+ * length is the smallest power of two >= max(wa + wb - 2, wa, wb), at least 16: when
+ * wa + wb - 1 exceeds it by one, the top coefficient wraps onto coefficient 0 and the Garner
+ * kernel separates the two.  The limit is 2^26 (the primes' 2-adic order, ntt_base.h:22-27):
+ * up to 2^25 + 1 words (about 1.07e9 bits) per operand. */
 int bi_hip_transform_log2(size_t la, size_t lb);
 
 /* Per-kernel-class timing, for benchmarks: while enabled, every engine launch is bracketed by

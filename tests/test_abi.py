@@ -32,10 +32,15 @@ def test_host_only_entry_points():
     if not os.path.exists(engine.LIB_PATH):
         pytest.skip("libbigint_hip.so not built")
     assert engine.strerror(0) == "ok"
+    # transforms run over 32-bit words (7 per 8 limbs); n >= max(wa + wb - 2, wa, wb), the top
+    # coefficient wrapping onto coefficient 0 when wa + wb - 1 = n + 1
     assert engine.transform_log2(1, 1) == 4
-    assert engine.transform_log2(4096, 1) == 12
-    assert engine.transform_log2(4096, 2) == 13
+    assert engine.transform_log2(4096, 1) == 12     # 3584 words
+    assert engine.transform_log2(4096, 2) == 12     # 3584 + 2 - 2
+    assert engine.transform_log2(4682, 4682) == 13  # 4097 + 4097 - 2 = 8192 words
+    assert engine.transform_log2(2341, 2341) == 12  # 2049 + 2049 - 2 = 4096: one wrapped coefficient
     assert engine.transform_log2(1 << 25, 1 << 25) == 26
-    assert engine.transform_log2(1 << 25, (1 << 25) + 2) == -1
-    # the 2^28 x 2^28 workload: 9,586,981 limbs each -> n = 2^25
-    assert engine.transform_log2(9586981, 9586981) == 25
+    assert engine.transform_log2(38347923, 38347923) == 26  # 2^25 + 1 words each: the largest
+    assert engine.transform_log2(38347923, 38347924) == -1
+    # the 2^28 x 2^28 workload: 9,586,981 limbs = 8,388,609 words each -> n = 2^24
+    assert engine.transform_log2(9586981, 9586981) == 24

@@ -112,11 +112,11 @@ def test_large_workloads_residues(a_bits, b_bits):
 
 
 def test_max_transform_length():
-    """la + lb - 1 = 2^26 exactly: the largest product the three primes allow."""
-    la = (1 << 25) + 1
-    lb = (1 << 25)
+    """2^25 + 1 words per operand on a 2^26-point transform (the top coefficient wraps onto
+    coefficient 0 and is separated again): the largest product the three primes allow."""
+    la = lb = 38347923
     a = L.random_operand(28 * la, 5)
-    b = L.random_operand(28 * lb, 6)
+    b = L.random_operand(28 * lb - 3, 6)
     assert engine.transform_log2(la, lb) == 26
     p = engine.mul(a, b)
     check_residues(a, b, p)
@@ -127,7 +127,7 @@ def test_error_paths():
     with pytest.raises(engine.EngineError) as e:
         engine.mul(np.array([1, 0], np.uint32), a)  # top limb zero
     assert e.value.code == engine.EINVAL
-    big = np.ones((1 << 25) + 2, np.uint32)
+    big = np.ones(38347924, np.uint32)
     with pytest.raises(engine.EngineError) as e:
         engine.mul(big, big)
     assert e.value.code == engine.ETOOBIG
@@ -189,3 +189,36 @@ def test_chunked_2e28_by_2e20_residues():
     (p,) = engine.mul_batch([(a, b)], engine.PLAN_TABLE)
     assert np.array_equal(p, engine.mul(a, b))
     check_residues(a, b, p)
+
+
+def _limbs_for_words(w: int) -> int:
+    """Largest limb count whose 32-bit word count is w."""
+    la = (8 * w) // 7
+    while (la * 7 + 7) // 8 > w:
+        la -= 1
+    while ((la + 1) * 7 + 7) // 8 == w:
+        la += 1
+    return la
+
+
+@pytest.mark.parametrize("wa,wb", [(2049, 2049), (4097, 4097), (3000, 1098), (65537, 65537),
+                                   (1 << 19, (1 << 19) + 2), ((1 << 21) + 1, (1 << 21) + 1)])
+def test_wrapped_top_coefficient(wa, wb):
+    """wa + wb - 1 = n + 1: the top coefficient wraps onto coefficient 0 of the cyclic
+    convolution and Garner separates them.  All-ones operands make both top words (and every
+    coefficient) maximal; random operands and the square cover the rest."""
+    la, lb = _limbs_for_words(wa), _limbs_for_words(wb)
+    n = 1 << engine.transform_log2(la, lb)
+    assert wa + wb - 1 == n + 1
+    x, y = (1 << (28 * la)) - 1, (1 << (28 * lb)) - 1
+    p = engine.mul(L.int_to_limbs(x), L.int_to_limbs(y))
+    assert L.limbs_to_int(p) == x * y
+    a = L.random_operand(28 * la, wa)
+    b = L.random_operand(28 * lb, wb)
+    p = engine.mul(a, b)
+    if la <= 70000:
+        assert np.array_equal(p, oracle.mul_limbs(a, b))
+    else:
+        check_residues(a, b, p)
+    if wa == wb:
+        check_residues(a, None, engine.mul(a, None))

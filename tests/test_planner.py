@@ -128,8 +128,8 @@ def test_plan_allowed_rules():
     assert {0, 1, 2, 3, 4, 5} <= set(al) | {1, 2}
     assert 3 in al and 4 in al
     assert not engine.lib().bi_hip_plan_allowed(9586981, 9586981, 1, 3)  # squares never chunk
-    # balanced 2^28-bit operands: two chunks on 2^24-point transforms instead of one 2^25 product
-    assert 6 in planner.allowed(9586981, 9586981)
+    # balanced 2^28-bit operands fit one 2^24-point transform: no chunk plan is shorter
+    assert planner.allowed(9586981, 9586981) == [0, 1, 2]
     assert set(planner.allowed(70000, 20000)) >= {0, 6, 7}
     assert set(planner.allowed(200000, 150000)) >= {0, 6}
 
@@ -163,7 +163,7 @@ def test_mixed_plan_batch_is_exact():
     from oracle import oracle
     engine.init(0)
     shapes = [(50000, 900), (900, 50000), (70000, 20000), (20000, 70000), (90000, 60000),
-              (3000, 3000), (9000, 130), (200000, 150000), (33000, 4096)]
+              (3000, 3000), (9000, 130), (200000, 150000), (33000, 4096), (114285, 68571)]
     jobs = []
     for i, (la, lb) in enumerate(shapes):
         for p in planner.allowed(la, lb):
@@ -175,12 +175,12 @@ def test_mixed_plan_batch_is_exact():
 
 @pytest.mark.gpu
 def test_three_chunk_overlap_all_ones():
-    """All-ones operands maximise every coefficient; with 45056-limb chunks on 2^17-point
-    transforms three chunk products overlap at the top of each chunk."""
+    """All-ones operands maximise every coefficient; 100000 x 60000 words with 50176-word chunks
+    on 2^17-point transforms: three chunk products overlap in words [100352, 110175)."""
     from bigint_amd import engine, planner
     from bigint_amd import limbs as L
     engine.init(0)
-    la, lb = 90000, 60000
+    la, lb = 114285, 68571
     assert 6 in planner.allowed(la, lb)
     x, y = (1 << (28 * la)) - 1, (1 << (28 * lb)) - 1
     (got,) = _planned([(L.int_to_limbs(x), L.int_to_limbs(y), 6)])
@@ -189,16 +189,16 @@ def test_three_chunk_overlap_all_ones():
 
 @pytest.mark.gpu
 def test_balanced_2e28_every_plan_agrees():
-    """2^28-bit x 2^28-bit: every allowed plan (plain 2^25-point transforms, re-split block
-    passes, two or three chunks on 2^24-point transforms with one shared B transform) gives the
-    same product, and that product satisfies the residue checks."""
+    """2^28-bit x 2^28-bit (8,388,609 words each, one coefficient wrapping on a 2^24-point
+    transform): every allowed plan (4096/2048/1024-point block passes) gives the same product,
+    and that product satisfies the residue checks."""
     from bigint_amd import engine, planner
     from bigint_amd import limbs as L
     engine.init(0)
     a = L.random_operand(1 << 28, L.SEED)
     b = L.random_operand(1 << 28, L.SEED + 1)
     plans = planner.allowed(a.size, b.size)
-    assert {0, 6, 7} <= set(plans)
+    assert plans == [0, 1, 2]
     outs = _planned([(a, b, p) for p in plans])
     for q in L.RESIDUE_PRIMES:
         assert L.residue_mod(outs[0], q) == L.residue_mod(a, q) * L.residue_mod(b, q) % q

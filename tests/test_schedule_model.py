@@ -118,6 +118,7 @@ def run_group(I, inv, EG, log_r, log_c, log_trlo, nthreads, X, tw):
                     k = (1 << (log_r - lt - 1)) + (r >> (lt + 1))
                     assert 1 <= k < (1 << log_r)
                     u, v = x[sg * EG + q], x[sg * EG + q + h]
+                    assert 0 <= u < P[I] and 0 <= v < P[I], "butterfly input out of range"
                     if inv:
                         x[sg * EG + q] = (u + v) % P[I]
                         x[sg * EG + q + h] = mont(I, (u - v) % P[I], tw[k])
@@ -152,7 +153,24 @@ def tile_transform(I, inv, X, log_r, log_c, tw, nthreads):
         prev = gi
 
 
-def gio(plane, n, X, EG, log_c, log_trlo, tile_base, log_s, nthreads, store, limbs=None):
+def word_residue(I, x):
+    """Kernel word_residue: conditional subtractions bringing a 32-bit word below P (below
+    4P for P0, whose forward butterflies are lazy)."""
+    def csub(v, m):
+        return v - m if v >= m else v
+    if I == 0:
+        x = csub(csub(x, 4 * P[0]), 4 * P[0])
+        assert x < 4 * P[0]
+    elif I == 1:
+        x = csub(csub(x, 2 * P[1]), P[1])
+        assert x < P[1]
+    else:
+        x = csub(x, P[2])
+        assert x < P[2]
+    return x % P[I]
+
+
+def gio(plane, n, X, EG, log_c, log_trlo, tile_base, log_s, nthreads, store, limbs=None, prime=None):
     for tid in range(nthreads):
         rc = rows_cols(EG, log_c, log_trlo, nthreads, tid)
         if not store:
@@ -163,9 +181,82 @@ def gio(plane, n, X, EG, log_c, log_trlo, tile_base, log_s, nthreads, store, lim
             if store:
                 plane[idx] = X[tid][slot]
             elif limbs is not None:
-                X[tid][slot] = limbs[idx] if idx < len(limbs) else 0
+                X[tid][slot] = word_residue(prime, limb_word(limbs, idx))
             else:
                 X[tid][slot] = plane[idx]
+
+
+def limb_word(limbs, g):
+    """Kernel limb_word: bits [32g, 32g+32) of the limb string, from limbs k and k+1."""
+    k = (g * 8) // 7
+    o = g * 32 - k * 28
+    assert o in (0, 4, 8, 12, 16, 20, 24)
+    lo = limbs[k] if k < len(limbs) else 0
+    hi = limbs[k + 1] if k + 1 < len(limbs) else 0
+    return ((lo >> o) | (hi << (28 - o))) & 0xFFFFFFFF
+
+
+def words_of(nlimbs):
+    return (nlimbs * 7 + 7) // 8
+
+
+KWT, KLT = 3584, 4096  # words / limbs per carry tile
+
+
+def map_apply(m, c):
+    return (m >> (2 * c)) & 3
+
+
+def map_compose(second, first):
+    return sum(map_apply(second, map_apply(first, c)) << (2 * c) for c in range(4))
+
+
+def garner_carry(coeff, out_len):
+    """Kernel k_garner_carry's tile/thread arithmetic on exact coefficients coeff(j) (zero
+    outside the product): 32-bit pieces, word sums with the two-word halo, per-thread carry maps
+    over 14 words, composition across threads and tiles, then 14 words -> 16 limbs."""
+    ntiles = (out_len + KLT - 1) // KLT
+    out = [None] * out_len
+    carry_in = 0
+    for tile in range(ntiles):
+        base = tile * KWT
+        def piece(j, k):
+            return (coeff(j) >> (32 * k)) & 0xFFFFFFFF if j >= 0 else 0
+        sums = []
+        for o in range(KWT):
+            s = piece(base + o, 0) + piece(base + o - 1, 1) + piece(base + o - 2, 2)
+            assert s < 3 << 32
+            sums.append(s)
+        maps = []
+        for t in range(256):
+            c = [0, 1, 2, 3]
+            for q in range(14):
+                s = sums[14 * t + q]
+                c = [(s + ck) >> 32 for ck in c]
+            assert max(c) <= 3
+            maps.append(c[0] | c[1] << 2 | c[2] << 4 | c[3] << 6)
+        excl = 0b11100100
+        for t in range(256):
+            carry = map_apply(excl, carry_in)
+            assert carry <= 2
+            words = []
+            for q in range(14):
+                v = sums[14 * t + q] + carry
+                words.append(v & 0xFFFFFFFF)
+                carry = v >> 32
+            for q in range(16):
+                bit = 28 * q
+                wi, bo = bit >> 5, bit & 31
+                v = words[wi] >> bo
+                if bo > 4:
+                    v |= words[wi + 1] << (32 - bo)
+                i = tile * KLT + 16 * t + q
+                if i < out_len:
+                    out[i] = v & MASK
+            excl = map_compose(maps[t], excl)
+        carry_in = map_apply(excl, carry_in)
+    assert carry_in == 0
+    return out
 
 
 def plan(log_n):
@@ -199,7 +290,7 @@ def model_product(a, b, log_n):
                     tw = build_tw(I, Wf, None if blk == 0 else bt[blk], log_r)
                     X = [None] * 256
                     gio(res[op][I], n, X, 1 << gs[0][0], log_c, gs[0][1], tile_base, log_s, 256,
-                        False, limbs if k == 0 else None)
+                        False, limbs if k == 0 else None, I)
                     tile_transform(I, False, X, log_r, log_c, tw, 256)
                     gio(res[op][I], n, X, 1 << gs[-1][0], log_c, 0, tile_base, log_s, 256, True)
         # middle pass
@@ -214,10 +305,10 @@ def model_product(a, b, log_n):
             twi = build_tw(I, Wi, None if blk == 0 else bi[blk], mid_log_r)
             XA, XB = [None] * nthreads, [None] * nthreads
             gio(res[0][I], n, XA, 1 << gs[0][0], 0, gs[0][1], tb, 0, nthreads, False,
-                a if not outer else None)
+                a if not outer else None, I)
             tile_transform(I, False, XA, mid_log_r, 0, twf, nthreads)
             gio(res[1][I], n, XB, 1 << gs[0][0], 0, gs[0][1], tb, 0, nthreads, False,
-                b if not outer else None)
+                b if not outer else None, I)
             tile_transform(I, False, XB, mid_log_r, 0, twf, nthreads)
             for t in range(nthreads):
                 XA[t] = [mont(I, u, v) for u, v in zip(XA[t], XB[t])]
@@ -237,10 +328,10 @@ def model_product(a, b, log_n):
                 gio(res[0][I], n, X, 1 << gs[-1][0], log_c, 0, tile_base, log_s, 256, False)
                 tile_transform(I, True, X, log_r, log_c, tw, 256)
                 gio(res[0][I], n, X, 1 << gs[0][0], log_c, gs[0][1], tile_base, log_s, 256, True)
-    # Garner with the n^-1 R factor, pieces, and the map carry scan done sequentially
+    # Garner with the n^-1 R factor, 32-bit pieces and the tile carry scan
     y = res[0]
     out_len = len(a) + len(b)
-    pieces = []
+    xs = []
     for j in range(n):
         c = [mont(I, y[I][j], pow(n, -1, P[I]) * R32 * R32 % P[I]) for I in range(3)]
         v0 = c[0]
@@ -248,29 +339,29 @@ def model_product(a, b, log_n):
         t = (c[2] - v0) * pow(P[0], -1, P[2]) % P[2]
         v2 = (t - v1) * pow(P[1], -1, P[2]) % P[2]
         x = v0 + P[0] * (v1 + P[1] * v2)
-        assert x < 1 << 84
-        pieces.append((x & MASK, (x >> 28) & MASK, (x >> 56) & MASK))
-    out, carry = [], 0
-    for i in range(out_len):
-        s = (pieces[i][0] if i < n else 0) + (pieces[i - 1][1] if 1 <= i <= n else 0) + \
-            (pieces[i - 2][2] if 2 <= i <= n + 1 else 0)
-        assert s < 3 << 28
-        v = s + carry
-        out.append(v & MASK)
-        carry = v >> 28
-        assert carry <= 3
-    assert carry == 0
-    return out
+        assert x < 1 << 89
+        xs.append(x)
+    wa, wb = words_of(len(a)), words_of(len(b))
+    wrap = limb_word(a, wa - 1) * limb_word(b, wb - 1) if wa + wb - 1 > n else 0
+
+    def coeff(j):  # kernel garner(): the wrapped top coefficient leaves index 0 for index n
+        if j >= n:
+            return wrap if j == n else 0
+        return xs[j] - wrap if j == 0 else xs[j]
+    return garner_carry(coeff, out_len)
 
 
 def tlog(la, lb):
+    wa, wb = words_of(la), words_of(lb)
+    need = max(wa + wb - 2, wa, wb)
     lg = 4
-    while (1 << lg) < la + lb - 1:
+    while (1 << lg) < need:
         lg += 1
     return lg
 
 
-@pytest.mark.parametrize("la,lb", [(1, 1), (3, 5), (8, 9), (100, 60), (1000, 1000), (2048, 2049)])
+@pytest.mark.parametrize("la,lb", [(1, 1), (3, 5), (8, 9), (100, 60), (1000, 1000), (2048, 2049),
+                                   (2341, 2341), (19, 1), (18, 1)])
 def test_single_block_schedule(la, lb):
     rng = random.Random(la * 7 + lb)
     a = [rng.randrange(1 << 28) for _ in range(la)]
@@ -306,3 +397,82 @@ def test_plan_shapes():
             assert log_r + log_c == 12 and log_c <= log_s
             assert nblk * (1 << (log_r + log_s)) == 1 << log_n
             assert 1 <= log_r <= 8 or (log_n - 12 > 8 and log_r <= 7)
+
+
+def _words(limbs):
+    return [limb_word(limbs, g) for g in range(words_of(len(limbs)))]
+
+
+@pytest.mark.parametrize("nlimbs", [1, 2, 7, 8, 9, 15, 16, 17, 100])
+def test_limb_words_reassemble_the_integer(nlimbs):
+    rng = random.Random(nlimbs)
+    limbs = [rng.randrange(1 << 28) for _ in range(nlimbs)]
+    limbs[-1] |= 1 << 27
+    w = _words(limbs)
+    assert sum(x << (32 * g) for g, x in enumerate(w)) == sum(x << (28 * i) for i, x in enumerate(limbs))
+    assert w[-1] != 0 and all(0 <= x < 1 << 32 for x in w)
+    # one word past the end reads zero
+    assert limb_word(limbs, len(w)) == 0
+
+
+@pytest.mark.parametrize("la,lb", [(1, 1), (8, 9), (4096, 3), (5000, 5000), (9000, 4)])
+def test_garner_tile_carry_all_ones(la, lb):
+    """Maximal coefficients: all-ones operands through the 32-bit-word tile carry."""
+    a, b = (1 << (28 * la)) - 1, (1 << (28 * lb)) - 1
+    wa = _words([MASK] * la)
+    wb = _words([MASK] * lb)
+    conv = [0] * (len(wa) + len(wb) - 1)
+    # all-ones words except the top: convolution by prefix sums
+    for i, x in enumerate(wa):
+        for j, y in enumerate(wb):
+            conv[i + j] += x * y
+    out = garner_carry(lambda j: conv[j] if 0 <= j < len(conv) else 0, la + lb)
+    got = sum(x << (28 * i) for i, x in enumerate(out))
+    assert got == a * b
+
+
+@pytest.mark.parametrize("la,lb,L,log_n", [
+    (9000, 700, 1024, 11),      # power-of-two chunks: primary chunk j >> log_chunk
+    (20000, 5000, 7168, 14),    # multiple of the 3584-word tile: one primary chunk per tile
+    (20000, 9000, 7168, 14),    # shorter operand longer than the chunk: three chunks overlap
+])
+def test_chunked_garner_index_logic(la, lb, L, log_n):
+    """The kernel's chunk loop (primary chunk, earlier chunks while off < n) summing exact chunk
+    convolutions reproduces the full product."""
+    rng = random.Random(L + la)
+    A = [rng.randrange(1 << 28) for _ in range(la)]
+    B = [rng.randrange(1 << 28) for _ in range(lb)]
+    wa, wb = _words(A), _words(B)
+    n = 1 << log_n
+    assert L + len(wb) - 1 <= n
+    nch = (len(wa) + L - 1) // L
+    conv = []
+    for c in range(nch):
+        ch = wa[c * L:(c + 1) * L]
+        v = [0] * n
+        for i, x in enumerate(ch):
+            if x:
+                for j, y in enumerate(wb):
+                    v[i + j] += x * y
+        conv.append(v)
+    pow2 = L & (L - 1) == 0
+    log_chunk = L.bit_length() - 1
+
+    def coeff(j):
+        if j < 0:
+            return 0
+        c = (j >> log_chunk) if pow2 else (j // KWT * KWT) // L
+        x = 0
+        for d in range(c + 1):
+            cc = c - d
+            off = j - cc * L
+            if off >= n:
+                break
+            if cc < nch:
+                x += conv[cc][off]
+        assert x < 1 << 89
+        return x
+
+    out = garner_carry(coeff, la + lb)
+    got = sum(x << (28 * i) for i, x in enumerate(out))
+    assert got == sum(x << (28 * i) for i, x in enumerate(A)) * sum(x << (28 * i) for i, x in enumerate(B))
