@@ -112,6 +112,17 @@ def batch_leg(hip, engine, L, pairs_count: int, steps: int) -> dict:
         stream.synchronize()
         dt = (time.perf_counter() - t0) / steps
         res[name] = {"ms": round(dt * 1e3, 3), "Gbit_s": round(sum(a + b for a, b in bits) / dt / 1e9, 2)}
+    # per-kernel device time of one learned-mode batch (engine's event timing)
+    engine.profile(True)
+    base, ob = pool.ptr, outs[engine.PLAN_LEARNED].ptr
+    engine.mul_batch_device([(base + 4 * int(offa[i]), int(la[i]), base + 4 * int(offb[i]), int(lb[i]),
+                              ob + 4 * int(offa[i]), int(la[i] + lb[i])) for i in range(pairs_count)],
+                            engine.PLAN_LEARNED, stream.handle)
+    stream.synchronize()
+    prof = engine.profile_read()
+    engine.profile(False)
+    res["learned_kernels"] = {k: {"ms": round(v["ms"], 3), "launches": v["launches"]}
+                              for k, v in prof.items() if v["launches"]}
     o = {m: outs[m].download(np.uint32, tot_in) for m in (0, 1, 2)}
     same = bool(np.array_equal(o[0], o[1]) and np.array_equal(o[0], o[2]))
     ok = True

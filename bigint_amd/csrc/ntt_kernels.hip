@@ -172,8 +172,12 @@ __device__ __forceinline__ uint32_t limb_word(const uint32_t* __restrict__ limbs
                                               uint32_t g) {
     const uint32_t k = (g * 8u) / 7u;
     const uint32_t o = g * 32u - k * 28u;
-    const uint32_t lo = k < nlimbs ? limbs[k] : 0u;
-    const uint32_t hi = k + 1u < nlimbs ? limbs[k + 1u] : 0u;
+    // loads at clamped (always valid) indices, then selects: no branch around the loads
+    const uint32_t last = nlimbs - 1u;
+    uint32_t lo = limbs[k < last ? k : last];
+    uint32_t hi = limbs[k + 1u < last ? k + 1u : last];
+    lo = k < nlimbs ? lo : 0u;
+    hi = k + 1u < nlimbs ? hi : 0u;
     return (lo >> o) | (hi << (28u - o));
 }
 
@@ -226,8 +230,11 @@ __device__ __forceinline__ void global_io(uint32_t* __restrict__ plane, const Li
         for (int q = 0; q < Ly::EG; ++q) {
             const uint32_t idx = tile_base + c + (Ly::row(sg, q) << log_s);
             if (STORE) plane[idx] = x[sg * Ly::EG + q];
-            else if (src.limbs)
-                x[sg * Ly::EG + q] = idx < src.cnt ? limb_word(src.limbs, src.nlimbs, src.off + idx) : 0u;
+            else if (src.limbs) {
+                const bool in = idx < src.cnt;
+                const uint32_t w = limb_word(src.limbs, src.nlimbs, src.off + (in ? idx : 0u));
+                x[sg * Ly::EG + q] = in ? w : 0u;
+            }
             else x[sg * Ly::EG + q] = plane[idx];
         }
     }
@@ -294,6 +301,53 @@ __device__ __forceinline__ void outer_body(const PassArgs& a, const PassJob& job
         tile_transform<I, true, LR, LC>(x, lds, tw);
         global_io<LR, LC, S::stages(G0), S::trlo(G0), true>(plane, kPlane, x, tile_base, a.log_s);
     }
+}
+
+// First forward pass, reading limbs: one block transforms its tile for all three primes, so the
+// 32-bit words are cut from the limbs once and reduced per prime.  grid.z = operand.
+template <int LR>
+__device__ __forceinline__ void outer_first_body(const PassArgs& a, const PassJob& job, uint32_t op,
+                                                 uint32_t* lds, uint32_t* tw, uint32_t* bpow) {
+    constexpr int LC = 12 - LR;
+    using S = Sched<LR>;
+    const uint32_t n = 1u << a.log_n;
+    const uint32_t ntiles = gridDim.x;
+    uint32_t t = blockIdx.x;
+    if ((ntiles & 7u) == 0u) t = (t & 7u) * (ntiles >> 3) + (t >> 3);  // neighbours on one XCD
+    const uint32_t log_ncg = a.log_s - LC;
+    const uint32_t blk = t >> log_ncg;
+    const uint32_t cg = t & ((1u << log_ncg) - 1u);
+    const uint32_t tile_base = (blk << (LR + a.log_s)) + (cg << LC);
+    const LimbSrc src = op == 0 ? LimbSrc{job.limbs_a, job.la, job.a_off, job.a_cnt}
+                                : LimbSrc{job.limbs_b, job.lb, 0u, ~0u};
+    constexpr LimbSrc kPlane{nullptr, 0u, 0u, 0u};
+    constexpr int G0 = 0, GL = S::NG - 1;
+    uint32_t words[kE];
+    global_io<LR, LC, S::stages(G0), S::trlo(G0), false>(nullptr, src, words, tile_base, a.log_s);
+    static_for<kNumPrimes>([&](auto pi) {
+        constexpr int I = (int)pi;
+        if constexpr (I > 0) __syncthreads();  // the previous prime is done with tw and lds
+        const uint32_t beta = (a.beta && blk != 0) ? a.beta[I * a.beta_stride + blk] : 0u;
+        build_twiddles<I, LR>(tw, bpow, a.wtab + I * kTile, beta, kThreads);
+        uint32_t x[kE];
+#pragma unroll
+        for (int k = 0; k < kE; ++k) x[k] = word_residue<I>(words[k]);
+        __syncthreads();
+        tile_transform<I, false, LR, LC>(x, lds, tw);
+        uint32_t* plane = (op == 0 ? job.res_a : job.res_b) + (size_t)I * n;
+        global_io<LR, LC, S::stages(GL), S::trlo(GL), true>(plane, kPlane, x, tile_base, a.log_s);
+    });
+}
+
+// Its own kernel, so the register demand of the three-prime loop does not lower the occupancy of
+// the other strided passes.
+template <int LR>
+__global__ __launch_bounds__(kThreads) void k_outer_first(PassArgs a) {
+    __shared__ uint32_t lds[kLdsWords];
+    __shared__ uint32_t tw[1 << LR];
+    __shared__ uint32_t bpow[16];
+    const PassJob& job = a.jobs ? a.jobs[blockIdx.y] : a.job;
+    outer_first_body<LR>(a, job, blockIdx.z, lds, tw, bpow);
 }
 
 template <bool INV, int LR>
@@ -508,11 +562,16 @@ __device__ __forceinline__ void status_store(uint32_t* p, uint32_t v) {
 }
 constexpr uint32_t kAgg = 1u, kInc = 2u;
 
+// Pieces of an unchunked product's coefficient from its residues (no wrap at this index).
+__device__ __forceinline__ Pieces split_coeff(uint32_t r0, uint32_t r1, uint32_t r2,
+                                              const GarnerJob& jb, const GarnerConsts& k) {
+    const unsigned __int128 x = crt_coeff(r0, r1, r2, jb.kscale, k);
+    const uint64_t lo = (uint64_t)x;
+    return Pieces{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)(x >> 64)};
+}
+
 __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     const GarnerJob& jb = a.jobs ? a.jobs[blockIdx.y] : a.job;
-    // pieces of the tile's coefficients, padded one word per 16 against bank conflicts
-    __shared__ uint32_t s_d[3][kWordTile + kWordTile / 16];
-    __shared__ uint32_t s_halo[4];  // w1, w2 of coefficient base-1; w2 of base-2
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave[kThreads / 64];
     __shared__ uint32_t s_carry_in;
@@ -523,46 +582,78 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     const uint32_t ntiles = (jb.out_len + kCarryTile - 1u) / kCarryTile;
     if (tile >= ntiles) return;
     const uint32_t base = tile * kWordTile;  // first word (coefficient) of the tile
-    // chunk lengths that are multiples of the tile: the whole tile has one primary chunk
-    const uint32_t tile_chunk = (jb.nchunks > 1 && jb.log_chunk == 0xFFu) ? base / jb.chunk_len : 0u;
+    const uint32_t j0 = base + tid * kCarryWordsPerThread;  // this thread's first word
 
-    // 1. Garner for the tile's coefficients, coalesced: thread tid takes base + tid + 256 q
+    // 1. pieces of coefficients j0-2 .. j0+13: the two before the thread's words feed its
+    //    first two sums (recomputed here rather than exchanged).
+    constexpr uint32_t NQ = kCarryWordsPerThread + 2;
+    uint32_t w0[NQ], w1[NQ], w2[NQ];
+    const uint32_t n = jb.n;
+    const bool wrap_here = jb.wrap_a && j0 - 2u <= n && n < j0 + kCarryWordsPerThread;
+    if (jb.nchunks <= 1 && j0 >= 2u && !wrap_here) {
+        // all residues first (16 per plane, as 8 aligned pairs), then the CRT: many loads in
+        // flight per thread.  j0 - 2 is even and n is a power of two, so pairs never straddle n.
+        uint32_t r0[NQ], r1[NQ], r2[NQ];
+        const uint32_t jl = j0 - 2u;
+#pragma unroll
+        for (uint32_t i = 0; i < NQ; i += 2) {
+            if (jl + i < n) {
+                const uint2 v0 = *reinterpret_cast<const uint2*>(jb.y + jl + i);
+                const uint2 v1 = *reinterpret_cast<const uint2*>(jb.y + n + jl + i);
+                const uint2 v2 = *reinterpret_cast<const uint2*>(jb.y + 2u * n + jl + i);
+                r0[i] = v0.x; r0[i + 1] = v0.y;
+                r1[i] = v1.x; r1[i + 1] = v1.y;
+                r2[i] = v2.x; r2[i + 1] = v2.y;
+            } else {
+                r0[i] = r0[i + 1] = r1[i] = r1[i + 1] = r2[i] = r2[i + 1] = 0u;
+            }
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < NQ; ++i) {
+            const Pieces p = split_coeff(r0[i], r1[i], r2[i], jb, a.k);
+            w0[i] = p.w0;
+            w1[i] = p.w1;
+            w2[i] = p.w2;
+        }
+    } else {
+        // chunked products, the wrapped coefficient, and the product's first two words
+        const uint32_t tile_chunk =
+            (jb.nchunks > 1 && jb.log_chunk == 0xFFu) ? base / jb.chunk_len : 0u;
 #pragma unroll 2
-    for (uint32_t q = 0; q < kCarryWordsPerThread; ++q) {
-        const uint32_t o = tid + kThreads * q;
-        const Pieces p = garner(jb, a.k, base + o, tile_chunk);
-        const uint32_t li = o + (o >> 4);
-        s_d[0][li] = p.w0;
-        s_d[1][li] = p.w1;
-        s_d[2][li] = p.w2;
-    }
-    if (tid < 2) {  // the previous tile's last two coefficients feed words base and base+1
-        // the halo lies in the previous tile, hence possibly in the previous chunk
-        const uint32_t j = base - 2 + tid;
-        const uint32_t hc = (jb.log_chunk == 0xFFu && jb.nchunks > 1) ? j / jb.chunk_len : tile_chunk;
-        const Pieces p = base >= 2 - tid ? garner(jb, a.k, j, hc) : Pieces{0u, 0u, 0u};
-        if (tid == 1) {
-            s_halo[0] = p.w1;
-            s_halo[1] = p.w2;
-        } else {
-            s_halo[2] = p.w2;
+        for (uint32_t i = 0; i < NQ; ++i) {
+            Pieces p{0u, 0u, 0u};
+            if (j0 + i >= 2u) {
+                const uint32_t j = j0 + i - 2u;
+                // a chunk length that is a multiple of the tile gives the tile one primary
+                // chunk; the two coefficients before the tile may lie in the previous one
+                const uint32_t c = (jb.nchunks > 1 && jb.log_chunk == 0xFFu && j < base)
+                                       ? j / jb.chunk_len : tile_chunk;
+                p = garner(jb, a.k, j, c);
+            }
+            w0[i] = p.w0;
+            w1[i] = p.w1;
+            w2[i] = p.w2;
         }
     }
-    __syncthreads();
+
+    // The carry into the tile, resolved locally when possible: the previous tile's last word has
+    // the sum s = w0(base-1) + w1(base-2) + w2(base-3) and a carry-in of at most 2, so its
+    // carry-out is s >> 32 whatever that carry-in, unless s mod 2^32 > 2^32 - 3.  Then the tile
+    // needs nothing from its predecessors (0xFF: not resolved, use the look-back).
+    uint32_t local = 0xFFu;
+    if (tid == 0 && tile != 0) {
+        const uint32_t j = base - 3u;
+        const uint32_t c = (jb.nchunks > 1 && jb.log_chunk == 0xFFu) ? j / jb.chunk_len : 0u;
+        const Pieces p3 = garner(jb, a.k, j, c);
+        const uint64_t sl = (uint64_t)w0[1] + w1[0] + p3.w2;
+        if ((uint32_t)sl <= 0xFFFFFFFDu) local = (uint32_t)(sl >> 32);
+    }
 
     // 2. word sums s_i = w0_i + w1_{i-1} + w2_{i-2} < 3 * 2^32, 14 consecutive words per thread
     uint32_t s_lo[kCarryWordsPerThread], s_hi[kCarryWordsPerThread];
-    const uint32_t o0 = tid * kCarryWordsPerThread;
 #pragma unroll
     for (uint32_t q = 0; q < kCarryWordsPerThread; ++q) {
-        const uint32_t o = o0 + q;
-        const uint32_t w0 = s_d[0][o + (o >> 4)];
-        uint32_t w1, w2;
-        if (o >= 1) w1 = s_d[1][(o - 1) + ((o - 1) >> 4)];
-        else w1 = s_halo[0];
-        if (o >= 2) w2 = s_d[2][(o - 2) + ((o - 2) >> 4)];
-        else w2 = (o == 1) ? s_halo[1] : s_halo[2];
-        const uint64_t sum = (uint64_t)w0 + w1 + w2;
+        const uint64_t sum = (uint64_t)w0[q + 2] + w1[q + 1] + w2[q];
         s_lo[q] = (uint32_t)sum;
         s_hi[q] = (uint32_t)(sum >> 32);
     }
@@ -596,15 +687,25 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     if (lane == 0) excl = kMapIdentity;
     excl = map_compose(excl, wave_prefix);
 
-    // 4. decoupled look-back by the first wave: 64 predecessor tiles are read at once
+    // 4. carry into the tile: resolved locally, or by a decoupled look-back of the first wave
+    //    (64 predecessor tiles read at once).  Every tile publishes its status either way.
     if (wave == 0) {
         uint32_t agg = kMapIdentity;
         for (uint32_t w = 0; w < kThreads / 64; ++w) agg = map_compose(s_wave[w], agg);
         uint32_t carry_in = 0;
+        const uint32_t loc = __shfl(local, 0, 64);
         if (tile == 0) {
             if (lane == 0) status_store(&jb.status[0], (kInc << 8) | map_apply(agg, 0u));
+        } else if (loc != 0xFFu) {
+            carry_in = loc;
+            if (lane == 0) status_store(&jb.status[tile], (kInc << 8) | map_apply(agg, loc));
         } else {
-            if (lane == 0) status_store(&jb.status[tile], (kAgg << 8) | agg);
+            // A tile whose carry-out does not depend on its carry-in (every tile but those with a
+            // run of all-ones words at the start) publishes that carry as inclusive at once, so
+            // successors end their look-back here without waiting for this one's.
+            const bool fixed = agg == (agg & 3u) * 0x55u;
+            if (lane == 0)
+                status_store(&jb.status[tile], fixed ? ((kInc << 8) | (agg & 3u)) : ((kAgg << 8) | agg));
             uint32_t acc = kMapIdentity;  // composition of the nearer tiles already visited
             int32_t top = (int32_t)tile - 1;
             while (true) {
@@ -630,7 +731,7 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
                 acc = map_compose(acc, mm);
                 top -= 64;
             }
-            if (lane == 0) status_store(&jb.status[tile], (kInc << 8) | map_apply(agg, carry_in));
+            if (lane == 0 && !fixed) status_store(&jb.status[tile], (kInc << 8) | map_apply(agg, carry_in));
         }
         if (lane == 0) s_carry_in = carry_in;
     }
@@ -684,6 +785,10 @@ constexpr auto outer_table(std::integer_sequence<int, LRs...>) {
     return std::array<KernelFn, sizeof...(LRs)>{&k_outer_pass<INV, LRs + 1>...};
 }
 template <int... LRs>
+constexpr auto first_table(std::integer_sequence<int, LRs...>) {
+    return std::array<KernelFn, sizeof...(LRs)>{&k_outer_first<LRs + 1>...};
+}
+template <int... LRs>
 constexpr auto mid_table(std::integer_sequence<int, LRs...>) {
     return std::array<KernelFn, sizeof...(LRs)>{&k_mid_pass<LRs + 4>...};
 }
@@ -692,10 +797,13 @@ hipError_t launch_outer_pass(const PassArgs& a, bool inv, uint32_t njobs, uint32
                              hipStream_t st) {
     static constexpr auto fwd = outer_table<false>(std::make_integer_sequence<int, 8>{});
     static constexpr auto bwd = outer_table<true>(std::make_integer_sequence<int, 8>{});
+    static constexpr auto first = first_table(std::make_integer_sequence<int, 8>{});
     if (a.log_r < 1 || a.log_r > 8 || a.log_r + a.log_c != 12) return hipErrorInvalidValue;
     const uint32_t ntiles = 1u << (a.log_n - a.log_r - a.log_c);
-    dim3 grid(ntiles, njobs, kNumPrimes * nops);
-    hipLaunchKernelGGL((inv ? bwd : fwd)[a.log_r - 1], grid, dim3(kThreads), 0, st, a);
+    // a first forward pass (from limbs) handles the three primes inside each block
+    const bool fused = !inv && (a.flags & kFromLimbs);
+    dim3 grid(ntiles, njobs, fused ? nops : kNumPrimes * nops);
+    hipLaunchKernelGGL((fused ? first : inv ? bwd : fwd)[a.log_r - 1], grid, dim3(kThreads), 0, st, a);
     return hipGetLastError();
 }
 

@@ -222,3 +222,19 @@ def test_wrapped_top_coefficient(wa, wb):
         check_residues(a, b, p)
     if wa == wb:
         check_residues(a, None, engine.mul(a, None))
+
+
+def test_carry_runs_across_tiles():
+    """a * (2^32 + 1) is a + (a << 32): with a's words alternating so that a_j + a_(j-1) =
+    2^32 - 1, one carry from word 2 runs through three whole carry tiles (3584 words each).
+    Those tiles' carry maps depend on their carry-in, so the tile look-back has to compose
+    them back to the tile where the carry was born."""
+    W = 3 * 3584 + 100
+    words = [(1 << 32) - 1, 5] + [((1 << 32) - 6) if j % 2 == 0 else 5 for j in range(2, W)]
+    a = sum(w << (32 * j) for j, w in enumerate(words))
+    b = (1 << 32) + 1
+    p = engine.mul(L.int_to_limbs(a), L.int_to_limbs(b))
+    assert L.limbs_to_int(p) == a * b
+    # and the square of such a run, for carries born inside the product
+    p2 = engine.mul(L.int_to_limbs(a), None)
+    assert L.limbs_to_int(p2) == a * a

@@ -227,6 +227,10 @@ def garner_carry(coeff, out_len):
             s = piece(base + o, 0) + piece(base + o - 1, 1) + piece(base + o - 2, 2)
             assert s < 3 << 32
             sums.append(s)
+        if tile:  # kernel's local resolution of the carry into the tile, checked against the scan
+            sl = piece(base - 1, 0) + piece(base - 2, 1) + piece(base - 3, 2)
+            if sl & 0xFFFFFFFF <= 0xFFFFFFFD:
+                assert sl >> 32 == carry_in
         maps = []
         for t in range(256):
             c = [0, 1, 2, 3]
@@ -476,3 +480,15 @@ def test_chunked_garner_index_logic(la, lb, L, log_n):
     out = garner_carry(coeff, la + lb)
     got = sum(x << (28 * i) for i, x in enumerate(out))
     assert got == sum(x << (28 * i) for i, x in enumerate(A)) * sum(x << (28 * i) for i, x in enumerate(B))
+
+
+def test_garner_carry_run_across_tiles():
+    """A carry born in word 2 runs through three whole tiles (a * (2^32 + 1) with a_j + a_(j-1)
+    = 2^32 - 1): the tile maps depend on the carry-in and must compose back to tile 0."""
+    W = 3 * KWT + 100
+    words = [(1 << 32) - 1, 5] + [((1 << 32) - 6) if j % 2 == 0 else 5 for j in range(2, W)]
+    conv = [words[j] + (words[j - 1] if j else 0) for j in range(W)] + [words[-1]]
+    a = sum(w << (32 * j) for j, w in enumerate(words))
+    la = (32 * W + 27) // 28
+    out = garner_carry(lambda j: conv[j] if 0 <= j < len(conv) else 0, la + 2)
+    assert sum(x << (28 * i) for i, x in enumerate(out)) == a * ((1 << 32) + 1)
