@@ -325,7 +325,7 @@ public:
         g.jobs = d_gjobs;
         g.k = gk;
         if (timed(BI_HIP_K_GARNER, 12.0 * n * nj + 4.0 * out_limbs, st,
-                  [&] { return launch_garner_carry(g, max_out_len, ngjobs, st); }) != hipSuccess)
+                  [&] { return launch_garner_carry(g, max_out_len, ngjobs, kind == kChunk, st); }) != hipSuccess)
             return BI_HIP_EDEVICE;
         return BI_HIP_OK;
     }

@@ -570,6 +570,9 @@ __device__ __forceinline__ Pieces split_coeff(uint32_t r0, uint32_t r1, uint32_t
     return Pieces{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)(x >> 64)};
 }
 
+// CHUNKED: the jobs are chunked products (nchunks > 1); otherwise every job is unchunked.  Two
+// instantiations keep the chunk loop's register demand out of the common kernel.
+template <bool CHUNKED>
 __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     const GarnerJob& jb = a.jobs ? a.jobs[blockIdx.y] : a.job;
     __shared__ uint32_t s_tile;
@@ -590,33 +593,72 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     uint32_t w0[NQ], w1[NQ], w2[NQ];
     const uint32_t n = jb.n;
     const bool wrap_here = jb.wrap_a && j0 - 2u <= n && n < j0 + kCarryWordsPerThread;
-    if (jb.nchunks <= 1 && j0 >= 2u && !wrap_here) {
+    if (j0 >= 2u && !wrap_here) {
         // all residues first (16 per plane, as 8 aligned pairs), then the CRT: many loads in
-        // flight per thread.  j0 - 2 is even and n is a power of two, so pairs never straddle n.
+        // flight per thread.  j0 - 2, n and chunk lengths are even, so pairs never straddle a
+        // transform's end.
         uint32_t r0[NQ], r1[NQ], r2[NQ];
         const uint32_t jl = j0 - 2u;
+        bool any;  // some coefficient of the run lies inside a transform
+        if constexpr (!CHUNKED) {
+            any = jl < n;
 #pragma unroll
-        for (uint32_t i = 0; i < NQ; i += 2) {
-            if (jl + i < n) {
-                const uint2 v0 = *reinterpret_cast<const uint2*>(jb.y + jl + i);
-                const uint2 v1 = *reinterpret_cast<const uint2*>(jb.y + n + jl + i);
-                const uint2 v2 = *reinterpret_cast<const uint2*>(jb.y + 2u * n + jl + i);
-                r0[i] = v0.x; r0[i + 1] = v0.y;
-                r1[i] = v1.x; r1[i + 1] = v1.y;
-                r2[i] = v2.x; r2[i + 1] = v2.y;
-            } else {
-                r0[i] = r0[i + 1] = r1[i] = r1[i + 1] = r2[i] = r2[i + 1] = 0u;
+            for (uint32_t i = 0; i < NQ; i += 2) {
+                if (jl + i < n) {
+                    const uint2 v0 = *reinterpret_cast<const uint2*>(jb.y + jl + i);
+                    const uint2 v1 = *reinterpret_cast<const uint2*>(jb.y + n + jl + i);
+                    const uint2 v2 = *reinterpret_cast<const uint2*>(jb.y + 2u * n + jl + i);
+                    r0[i] = v0.x; r0[i + 1] = v0.y;
+                    r1[i] = v1.x; r1[i + 1] = v1.y;
+                    r2[i] = v2.x; r2[i + 1] = v2.y;
+                } else {
+                    r0[i] = r0[i + 1] = r1[i] = r1[i + 1] = r2[i] = r2[i + 1] = 0u;
+                }
+            }
+        } else {
+            // chunk cc covers words [cc L, cc L + n); the chunks reaching the run are
+            // (jl - n) / L < cc <= (jl + NQ - 1) / L.  Their residues add mod each prime.
+            const uint32_t L = jb.chunk_len;
+            const bool p2 = jb.log_chunk < 32u;
+            const uint32_t top = jl + NQ - 1u;
+            uint32_t c_hi = p2 ? (top >> jb.log_chunk) : top / L;
+            c_hi = c_hi < jb.nchunks - 1u ? c_hi : jb.nchunks - 1u;
+            const uint32_t c_lo = jl >= n ? (p2 ? ((jl - n) >> jb.log_chunk) : (jl - n) / L) + 1u : 0u;
+            any = c_lo <= c_hi;
+#pragma unroll
+            for (uint32_t i = 0; i < NQ; ++i) r0[i] = r1[i] = r2[i] = 0u;
+            for (uint32_t cc = c_lo; cc <= c_hi; ++cc) {
+                const uint32_t* y = jb.y + (size_t)cc * jb.chunk_stride;
+                const int32_t off0 = (int32_t)jl - (int32_t)(cc * L);
+#pragma unroll
+                for (uint32_t i = 0; i < NQ; i += 2) {
+                    const int32_t off = off0 + (int32_t)i;
+                    if (off >= 0 && off < (int32_t)n) {
+                        const uint2 v0 = *reinterpret_cast<const uint2*>(y + off);
+                        const uint2 v1 = *reinterpret_cast<const uint2*>(y + n + off);
+                        const uint2 v2 = *reinterpret_cast<const uint2*>(y + 2u * n + off);
+                        r0[i] = add_mod<0>(r0[i], fold1p(v0.x));  // P0 outputs lie in [0, 2P)
+                        r0[i + 1] = add_mod<0>(r0[i + 1], fold1p(v0.y));
+                        r1[i] = add_mod<1>(r1[i], v1.x);
+                        r1[i + 1] = add_mod<1>(r1[i + 1], v1.y);
+                        r2[i] = add_mod<2>(r2[i], v2.x);
+                        r2[i + 1] = add_mod<2>(r2[i + 1], v2.y);
+                    }
+                }
             }
         }
 #pragma unroll
         for (uint32_t i = 0; i < NQ; ++i) {
-            const Pieces p = split_coeff(r0[i], r1[i], r2[i], jb, a.k);
+            // coefficients outside every transform are zero: no CRT (whole threads skip it in
+            // the tiles of short products)
+            const bool in = CHUNKED ? any : jl + i < n;
+            const Pieces p = in ? split_coeff(r0[i], r1[i], r2[i], jb, a.k) : Pieces{0u, 0u, 0u};
             w0[i] = p.w0;
             w1[i] = p.w1;
             w2[i] = p.w2;
         }
     } else {
-        // chunked products, the wrapped coefficient, and the product's first two words
+        // the wrapped coefficient and the product's first two words
         const uint32_t tile_chunk =
             (jb.nchunks > 1 && jb.log_chunk == 0xFFu) ? base / jb.chunk_len : 0u;
 #pragma unroll 2
@@ -818,10 +860,11 @@ hipError_t launch_mid_pass(const PassArgs& a, uint32_t njobs, hipStream_t st) {
 }
 
 hipError_t launch_garner_carry(const GarnerArgs& a, uint32_t max_out_len, uint32_t njobs,
-                               hipStream_t st) {
+                               bool chunked, hipStream_t st) {
     const uint32_t ntiles = (max_out_len + kCarryTile - 1u) / kCarryTile;
     dim3 grid(ntiles, njobs, 1);
-    hipLaunchKernelGGL(k_garner_carry, grid, dim3(kThreads), 0, st, a);
+    hipLaunchKernelGGL(chunked ? k_garner_carry<true> : k_garner_carry<false>, grid, dim3(kThreads), 0,
+                       st, a);
     return hipGetLastError();
 }
 

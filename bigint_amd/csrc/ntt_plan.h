@@ -82,7 +82,7 @@ hipError_t launch_outer_pass(const PassArgs& a, bool inv, uint32_t njobs, uint32
                              hipStream_t st);
 hipError_t launch_mid_pass(const PassArgs& a, uint32_t njobs, hipStream_t st);
 hipError_t launch_garner_carry(const GarnerArgs& a, uint32_t max_out_len, uint32_t njobs,
-                               hipStream_t st);
+                               bool chunked, hipStream_t st);
 hipError_t launch_fill(uint32_t* p, uint32_t v, uint32_t count, hipStream_t st);
 
 }  // namespace bi

@@ -481,6 +481,23 @@ def test_chunked_garner_index_logic(la, lb, L, log_n):
     got = sum(x << (28 * i) for i, x in enumerate(out))
     assert got == sum(x << (28 * i) for i, x in enumerate(A)) * sum(x << (28 * i) for i, x in enumerate(B))
 
+    # the kernel's run form: for a thread's 16 coefficients from jl (even), chunks
+    # (jl - n) / L < cc <= (jl + 15) / L, each adding at offsets 0 <= off < n
+    total_words = len(wa) + len(wb)
+    for jl in range(0, total_words + 64, 2):
+        top = jl + 15
+        c_hi = min(top // L, nch - 1)
+        c_lo = (jl - n) // L + 1 if jl >= n else 0
+        run = [0] * 16
+        for cc in range(c_lo, c_hi + 1):
+            off0 = jl - cc * L
+            for i in range(0, 16, 2):
+                off = off0 + i
+                if 0 <= off < n:
+                    run[i] += conv[cc][off]
+                    run[i + 1] += conv[cc][off + 1]
+        assert run == [coeff(jl + i) for i in range(16)], jl
+
 
 def test_garner_carry_run_across_tiles():
     """A carry born in word 2 runs through three whole tiles (a * (2^32 + 1) with a_j + a_(j-1)
