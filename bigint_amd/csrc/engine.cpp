@@ -7,6 +7,7 @@
 #include <cmath>
 #include <tuple>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -1028,6 +1029,30 @@ int bi_hip_set_planner(const int8_t* table, const float* mlp_params, int K, cons
         for (int i = 0; i < 6; ++i) e.mlp_norm[i] = norm6[i];
     }
     return BI_HIP_OK;
+}
+
+int bi_hip_load_planner(const char* path) {
+    if (!path) return BI_HIP_EINVAL;
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) return BI_HIP_EINVAL;
+    char magic[4];
+    int32_t K = 0, np = 0;
+    std::vector<int8_t> table(19 * 19 * 13);
+    std::vector<float> params, norm(6);
+    bool ok = std::fread(magic, 1, 4, f) == 4 && std::memcmp(magic, "BIPL", 4) == 0 &&
+              std::fread(&K, 4, 1, f) == 1 && K > 0 && K <= bi::kMaxPlans &&
+              std::fread(table.data(), 1, table.size(), f) == table.size() &&
+              std::fread(&np, 4, 1, f) == 1 && np == bi::mlp_param_count(K);
+    if (ok) {
+        params.resize((size_t)np);
+        ok = std::fread(params.data(), 4, params.size(), f) == params.size() &&
+             std::fread(norm.data(), 4, 6, f) == 6;
+    }
+    std::fclose(f);
+    if (!ok) return BI_HIP_EINVAL;
+    for (int8_t t : table)
+        if (t < 0 || t >= K) return BI_HIP_EINVAL;
+    return bi_hip_set_planner(table.data(), params.data(), K, norm.data());
 }
 
 int bi_hip_mlp_param_count(int K) { return (K > 0 && K <= bi::kMaxPlans) ? bi::mlp_param_count(K) : -1; }

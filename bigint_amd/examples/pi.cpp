@@ -8,7 +8,7 @@
 // the engine together (BigInt::mul_batch).  The merge (P, Q, R) o (P', Q', R') =
 // (P Q' + R P', Q Q', R R') is associative, so any bracketing gives the same P(1, r) and Q(1, r).
 //
-// usage: pi <decimal digits> [plan mode 0|1|2] [output file]
+// usage: pi <decimal digits> [plan mode 0|1|2] [output file] [planner file]
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -85,6 +85,7 @@ int main(int argc, char** argv) {
     size_type dec_digits = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 1000;
     const int plan_mode = argc > 2 ? std::atoi(argv[2]) : 1;
     const char* out_path = argc > 3 ? argv[3] : "pi.txt";
+    const bool planner = argc > 4 && mul::load_planner(argv[4]);
     if (dec_digits < 2) dec_digits = 2;
 
     auto inner = static_cast<size_type>(
@@ -121,6 +122,7 @@ int main(int argc, char** argv) {
     const auto t3 = clk::now();
     auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
     std::cout << "{\"digits\": " << dec_digits << ", \"plan_mode\": " << plan_mode
+              << ", \"planner_loaded\": " << (planner ? "true" : "false")
               << ", \"compute_ms\": " << ms(t1, t2) << ", \"split_ms\": " << ms(t1, t_split)
               << ", \"split_product_ms\": " << levels_ms << ", \"output_ms\": " << ms(t2, t3)
               << ", \"gpu_products\": " << mul::gpu_product_count()

@@ -59,6 +59,8 @@ auto schoolbook(std::span<const uint32_t> a, std::span<const uint32_t> b) -> Dig
 auto gpu_product_count() noexcept -> uint64_t { return g_gpu_products.load(); }
 auto host_product_count() noexcept -> uint64_t { return g_host_products.load(); }
 
+auto load_planner(const char* path) -> bool { return backend::load_planner(path); }
+
 auto mul_limbs(std::span<const uint32_t> a, std::span<const uint32_t> b, bool square) -> Digits {
     if (square) b = a;
     if ((words64(a) + words64(b)) * 64 <= HOST_MAX_TOTAL_BITS) {

@@ -20,4 +20,7 @@ struct Job {
 // Each job's product into its out buffer; throws on failure.
 void multiply(std::vector<Job>& jobs, int plan_mode);
 
+// Install a plan table / cost model file; false if unsupported or unusable.
+bool load_planner(const char* path);
+
 }  // namespace bigint::mul::backend

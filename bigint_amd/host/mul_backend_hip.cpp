@@ -30,4 +30,6 @@ void multiply(std::vector<Job>& jobs, int plan_mode) {
     for (size_t k = 0; k < jobs.size(); ++k) jobs[k].out_len = hp[k].out_len;
 }
 
+bool load_planner(const char* path) { return bi_hip_load_planner(path) == BI_HIP_OK; }
+
 }  // namespace bigint::mul::backend

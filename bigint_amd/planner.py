@@ -245,10 +245,23 @@ def choose(shapes, mode: int) -> list[int]:
 
 
 def save(path: str, table: np.ndarray, params: np.ndarray, norm6: np.ndarray, meta: dict) -> None:
+    """JSON (with the training metadata) plus the binary form bi_hip_load_planner reads:
+    b"BIPL", int32 K, int8 table[19*19*13], int32 parameter count, float32 params, float32 norm[6]."""
     os.makedirs(os.path.dirname(path), exist_ok=True)
     with open(path, "w") as f:
         json.dump({"K": K, "table": table.reshape(-1).tolist(), "params": params.tolist(),
                    "norm": norm6.tolist(), "meta": meta}, f)
+    write_binary(os.path.splitext(path)[0] + ".bin", table, params, norm6)
+
+
+def write_binary(path: str, table: np.ndarray, params: np.ndarray, norm6: np.ndarray) -> None:
+    with open(path, "wb") as f:
+        f.write(b"BIPL")
+        f.write(np.int32(K).tobytes())
+        f.write(np.ascontiguousarray(table, np.int8).reshape(-1).tobytes())
+        f.write(np.int32(params.size).tobytes())
+        f.write(np.ascontiguousarray(params, np.float32).tobytes())
+        f.write(np.ascontiguousarray(norm6, np.float32).tobytes())
 
 
 def load(path: str = DEFAULT_PATH, do_install: bool = True):

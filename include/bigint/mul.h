@@ -41,6 +41,11 @@ auto mul_digits(const Digits& a, const Digits& b, size_type output_precision = 0
 auto gpu_product_count() noexcept -> uint64_t;
 auto host_product_count() noexcept -> uint64_t;
 
+// Install a tuned plan table and cost model (the binary file bigint_amd/planner.py writes) for
+// the engine's plan modes 1 (table) and 2 (learned); false if the backend has no planner or the
+// file is unusable.  Without one the engine plans by its built-in heuristic.
+auto load_planner(const char* path) -> bool;
+
 }  // namespace bigint::mul
 
 namespace bigint::mul {

@@ -157,6 +157,11 @@ int bi_hip_choose_plans(const uint32_t* shapes, const int* is_square, size_t cou
  * round(log2 bits of b) - 10, round(log2 batch); NULL keeps the current one) and/or the MLP
  * (fp32 parameters for K outputs, input normalisation {mean[3], std[3]}; NULL keeps it). */
 int bi_hip_set_planner(const int8_t* table, const float* mlp_params, int K, const float* norm6);
+/* Install table and MLP from the binary file bigint_amd/planner.py writes
+ * (b"BIPL", int32 K, int8 table[19*19*13], int32 count, float32 params[count], float32 norm[6]);
+ * BI_HIP_EINVAL if the file is missing or malformed.  No reference counterpart (the reference's
+ * thresholds are compiled in, include/bigint/mul.h:80, 153, 192). */
+int bi_hip_load_planner(const char* path);
 
 /* MLP cost model kernels (training and inference on the GPU; device pointers). */
 int bi_hip_mlp_param_count(int K);
