@@ -140,6 +140,26 @@ def batch_leg(hip, engine, L, pairs_count: int, steps: int) -> dict:
     return res
 
 
+def profiled_traffic(kernel_class: str):
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled) and VALU issue fraction of a kernel
+    class at the 2^28-bit workload, from the committed rocprofv3 summary; None if absent."""
+    path = os.path.join(ROOT, "profiles", "ntt_mul_2e28_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    rows = []
+    with open(path) as f:
+        for line in f:
+            if line.startswith("#") or not line.strip():
+                continue
+            cls, kern, fb, wb, vi, frac, us = line.split()
+            if cls == kernel_class:
+                rows.append((kern, int(fb) + int(wb), float(frac), float(us)))
+    if not rows:
+        return None
+    kern, traffic, frac, us = max(rows, key=lambda r: r[3])
+    return {"kernel": kern, "traffic": traffic, "valu_issue_frac": frac}
+
+
 def copy_calibration(hip) -> int:
     """A plain 1 GiB device-to-device copy, three times: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
     this command gives the counters' reading for a known byte count (1 GiB read + 1 GiB written)."""
@@ -220,6 +240,7 @@ def main() -> int:
     avg_ms = d["ms"] / d["launches"]
     bytes_per_launch = d["bytes"] / d["launches"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    prof_t = profiled_traffic(dom) if args.bits == 28 else None
     n = 1 << engine.transform_log2(a.size, b.size)
     plan_id = planner.choose([(int(a.size), int(b.size), 1)], engine.PLAN_TABLE)[0]
     kernels = {k: {"avg_ms": v["ms"] / v["launches"], "launches_per_step": v["launches"] / args.steps,
@@ -242,7 +263,11 @@ def main() -> int:
                                f"device-resident", "limbs": [int(a.size), int(b.size)],
                    "plain_ntt_length": n, "plan": plan_id, "exact": bool(exact)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": prof_t["traffic"] if prof_t else None,
+                     "traffic_source": "profiles/ntt_mul_2e28_traffic.txt (rocprofv3 FETCH_SIZE/WRITE_SIZE)"
+                                       if prof_t else None,
+                     "valu_issue_frac": prof_t["valu_issue_frac"] if prof_t else None,
                      "alg_bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_ms, 4)},
         "kernels": kernels,
         "mul_ms": round(ms, 4),

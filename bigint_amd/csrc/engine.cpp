@@ -286,7 +286,7 @@ public:
             a.beta_stride = op.nblk;
             // the first pass reads the limbs once; every pass writes 3 planes per operand
             const double bytes = k == 0 ? 4.0 * in_limbs + 12.0 * n * nops_d * nj : 24.0 * n * nops_d * nj;
-            if (timed(BI_HIP_K_OUTER_FWD, bytes, st,
+            if (timed(k == 0 ? BI_HIP_K_OUTER_FIRST : BI_HIP_K_OUTER_FWD, bytes, st,
                       [&] { return launch_outer_pass(a, false, njobs, nops, st); }) != hipSuccess)
                 return BI_HIP_EDEVICE;
         }
@@ -435,20 +435,28 @@ public:
         return true;
     }
 
-    // Heuristic used until a tuned table or model is installed: a transform count model.  An
-    // unchunked product costs 3 transforms of length 2^lg (2 when squaring); a chunked one
-    // costs 1 + 2 * chunks transforms of its shorter length.  A transform is priced n * lg, and
-    // chunked plans must win by 10% to cover their extra launches and Garner reads.
+    // Transform-count cost of a plan: an unchunked product runs 3 transforms of 2^lg points (2
+    // when squaring), a chunked one 1 + 2 * chunks transforms of its shorter length; a transform
+    // is priced n * lg.  -1 for a plan that does not apply.  The heuristic picks by this cost
+    // (chunked plans must win by 10% to cover their extra launches and Garner reads); the learned
+    // model predicts each plan's time relative to it.
+    static double plan_cost(uint32_t la, uint32_t lb, bool sq, int id) {
+        PairPlan c{};
+        if (!plan_for_id(la, lb, sq, id, c)) return -1.0;
+        const double tr = c.chunk_len
+                              ? 1.0 + 2.0 * (double)((words_of(std::max(la, lb)) + c.chunk_len - 1) / c.chunk_len)
+                              : (sq ? 2.0 : 3.0);
+        return tr * std::ldexp((double)c.lg, (int)c.lg);
+    }
+
     static PairPlan heuristic_plan(uint32_t la, uint32_t lb, bool sq) {
         PairPlan best{};
         plan_for_id(la, lb, sq, 0, best);
-        double best_cost = (sq ? 2.0 : 3.0) * std::ldexp((double)best.lg, (int)best.lg);
-        const uint32_t hi = words_of(std::max(la, lb));
+        double best_cost = plan_cost(la, lb, sq, 0);
         for (int id = 3; id < kNumPlanIds; ++id) {
             PairPlan c{};
             if (!plan_for_id(la, lb, sq, id, c)) continue;
-            const double chunks = (double)((hi + c.chunk_len - 1) / c.chunk_len);
-            const double cost = 1.1 * (1.0 + 2.0 * chunks) * std::ldexp((double)c.lg, (int)c.lg);
+            const double cost = 1.1 * plan_cost(la, lb, sq, id);
             if (cost < best_cost) {
                 best_cost = cost;
                 best = c;
@@ -489,30 +497,37 @@ public:
     }
 
     // Learned choice for a list of shapes: MLP forward + masked argmin on the GPU.
-    int mlp_choose_ids(const std::vector<std::array<uint32_t, 3>>& shapes,
+    int mlp_choose_ids(const std::vector<std::array<uint32_t, 3>>& shapes, const std::vector<bool>& sq,
                        const std::vector<uint8_t>& allowed, std::vector<int32_t>& ids,
                        hipStream_t st) {
         const int B = (int)shapes.size(), K = mlp_k;
         ids.assign(B, 0);
         if (B == 0 || K == 0 || !d_mlp) return BI_HIP_EINVAL;
-        const size_t words = (size_t)B * 3 + (size_t)B * K + ((size_t)B * K + 3) / 4 + 4 + (size_t)B;
+        const size_t words = (size_t)B * 3 + 2 * (size_t)B * K + ((size_t)B * K + 3) / 4 + 4 + (size_t)B;
         float* w = scratch(words + 64);
         if (!w) return BI_HIP_ENOMEM;
-        std::vector<float> x((size_t)B * 3);
+        std::vector<float> x((size_t)B * 3), off((size_t)B * K, 0.f);
         for (int i = 0; i < B; ++i) {
             const double f[3] = {std::log2(28.0 * shapes[i][0]), std::log2(28.0 * shapes[i][1]),
                                  std::log2((double)std::max(1u, shapes[i][2]))};
             for (int j = 0; j < 3; ++j) x[(size_t)i * 3 + j] = (float)((f[j] - mlp_norm[j]) / mlp_norm[3 + j]);
+            // the model predicts log2(time / transform-count cost) per plan
+            for (int k = 0; k < K; ++k) {
+                const double c = plan_cost(shapes[i][0], shapes[i][1], sq[i], k);
+                off[(size_t)i * K + k] = c > 0 ? (float)std::log2(c) : 0.f;
+            }
         }
         float* dx = w;
         float* dy = dx + (size_t)B * 3;
-        uint8_t* dallow = reinterpret_cast<uint8_t*>(dy + (size_t)B * K);
-        int32_t* dchoice = reinterpret_cast<int32_t*>(dy + (size_t)B * K + ((size_t)B * K + 3) / 4 + 1);
+        float* doff = dy + (size_t)B * K;
+        uint8_t* dallow = reinterpret_cast<uint8_t*>(doff + (size_t)B * K);
+        int32_t* dchoice = reinterpret_cast<int32_t*>(doff + (size_t)B * K + ((size_t)B * K + 3) / 4 + 1);
         if (hipMemcpyAsync(dx, x.data(), x.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(doff, off.data(), off.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
             hipMemcpyAsync(dallow, allowed.data(), allowed.size(), hipMemcpyHostToDevice, st) != hipSuccess)
             return BI_HIP_EDEVICE;
         if (mlp_forward(dx, B, d_mlp, K, dy, nullptr, nullptr, st) != hipSuccess) return BI_HIP_EDEVICE;
-        if (mlp_choose(dy, dallow, B, K, dchoice, st) != hipSuccess) return BI_HIP_EDEVICE;
+        if (mlp_choose(dy, doff, dallow, B, K, dchoice, st) != hipSuccess) return BI_HIP_EDEVICE;
         if (hipMemcpyAsync(ids.data(), dchoice, (size_t)B * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
             return BI_HIP_EDEVICE;
         return hipStreamSynchronize(st) == hipSuccess ? BI_HIP_OK : BI_HIP_EDEVICE;
@@ -544,7 +559,7 @@ public:
                 for (int k = 0; k < mlp_k && k < kNumPlanIds; ++k)
                     allowed[i * mlp_k + k] = plan_for_id(shapes[i][0], shapes[i][1], sqv[i], k, tmp) ? 1 : 0;
             }
-            return mlp_choose_ids(shapes, allowed, ids, st);
+            return mlp_choose_ids(shapes, sqv, allowed, ids, st);
         }
         if (mode == BI_HIP_PLAN_TABLE || mode == BI_HIP_PLAN_LEARNED)
             for (size_t i = 0; i < count; ++i) ids[i] = table_plan(shapes[i][0], shapes[i][1], shapes[i][2]);
@@ -571,6 +586,19 @@ public:
             if (ids[i] < 0 || !plan_for_id(shapes[i][0], shapes[i][1], sqv[i], ids[i], pp))
                 pp = heuristic_plan(shapes[i][0], shapes[i][1], sqv[i]);
             out[i] = pp;
+        }
+        if (!explicit_ids && count > 1) {
+            // Unchunked products of one transform length share a launch set only if they share
+            // the block-pass size: give them the size most of them were assigned, so per-pair
+            // choices do not split a level into several under-filled launch sets.
+            std::map<uint32_t, std::array<uint32_t, 13>> votes;
+            for (const PairPlan& pp : out)
+                if (!pp.chunk_len) votes[pp.lg][std::min<uint32_t>(pp.mid_lr, 12)]++;
+            for (PairPlan& pp : out) {
+                if (pp.chunk_len) continue;
+                const auto& v = votes[pp.lg];
+                pp.mid_lr = (uint32_t)(std::max_element(v.begin(), v.end()) - v.begin());
+            }
         }
         return BI_HIP_OK;
     }
@@ -954,6 +982,11 @@ int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode) {
 }
 
 int bi_hip_plan_count(void) { return bi::Engine::kNumPlanIds; }
+
+double bi_hip_plan_cost(size_t la, size_t lb, int is_square, int plan_id) {
+    if (la == 0 || lb == 0 || la > 0xFFFFFFFFull || lb > 0xFFFFFFFFull) return -1.0;
+    return bi::Engine::plan_cost((uint32_t)la, (uint32_t)lb, is_square != 0, plan_id);
+}
 
 int bi_hip_plan_allowed(size_t la, size_t lb, int is_square, int plan_id) {
     if (la == 0 || lb == 0 || la > 0xFFFFFFFFull || lb > 0xFFFFFFFFull) return 0;

@@ -18,7 +18,7 @@ hipError_t mlp_grad(const float* x, const float* t, const float* mask, int B, co
                     float* part_loss, float* grad, hipStream_t st);
 hipError_t mlp_adam(float* p, const float* g, float* m, float* v, int NP, float lr, float b1,
                     float b2, float eps, int step, hipStream_t st);
-hipError_t mlp_choose(const float* y, const uint8_t* allowed, int B, int K, int32_t* choice,
-                      hipStream_t st);
+hipError_t mlp_choose(const float* y, const float* offset, const uint8_t* allowed, int B, int K,
+                      int32_t* choice, hipStream_t st);
 
 }  // namespace bi

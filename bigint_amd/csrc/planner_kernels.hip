@@ -211,15 +211,16 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 }
 
 // Plan choice: argmin of the predicted cost over the plans allowed for each sample.
-__global__ void k_argmin_plan(const float* __restrict__ y, const uint8_t* __restrict__ allowed,
-                              int B, int K, int32_t* __restrict__ choice) {
+__global__ void k_argmin_plan(const float* __restrict__ y, const float* __restrict__ offset,
+                              const uint8_t* __restrict__ allowed, int B, int K,
+                              int32_t* __restrict__ choice) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= B) return;
     int best = -1;
     float bv = 0.f;
     for (int k = 0; k < K; ++k) {
         if (!allowed[s * K + k]) continue;
-        const float c = y[s * K + k];
+        const float c = y[s * K + k] + (offset ? offset[s * K + k] : 0.f);
         if (best < 0 || c < bv) {
             best = k;
             bv = c;
@@ -261,10 +262,11 @@ hipError_t mlp_adam(float* p, const float* g, float* m, float* v, int NP, float 
     return hipGetLastError();
 }
 
-hipError_t mlp_choose(const float* y, const uint8_t* allowed, int B, int K, int32_t* choice,
-                      hipStream_t st) {
+hipError_t mlp_choose(const float* y, const float* offset, const uint8_t* allowed, int B, int K,
+                      int32_t* choice, hipStream_t st) {
     if (B <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_argmin_plan, dim3((B + 255) / 256), dim3(256), 0, st, y, allowed, B, K, choice);
+    hipLaunchKernelGGL(k_argmin_plan, dim3((B + 255) / 256), dim3(256), 0, st, y, offset, allowed, B, K,
+                       choice);
     return hipGetLastError();
 }
 

@@ -5,7 +5,9 @@
 #include <atomic>
 #include <cstring>
 #include <new>
+#include <thread>
 
+#include "bigint/bigint.h"
 #include "bigint_hip.h"
 #include "mul_backend.h"
 
@@ -119,13 +121,13 @@ namespace bigint::mul {
 auto mul_digits_batch(const std::vector<std::pair<const Digits*, const Digits*>>& pairs,
                       int plan_mode) -> std::vector<Digits> {
     std::vector<Digits> res(pairs.size());
-    struct Big {
+    struct Item {
         size_t idx;
         size_type za, zb;
         std::span<const uint32_t> a, b;
         bool same;
     };
-    std::vector<Big> big;
+    std::vector<Item> big, small;
     for (size_t i = 0; i < pairs.size(); ++i) {
         const Digits& a = *pairs[i].first;
         const Digits& b = *pairs[i].second;
@@ -141,26 +143,51 @@ auto mul_digits_batch(const std::vector<std::pair<const Digits*, const Digits*>>
         while (lb > 1 && sb[lb - 1] == 0) --lb;
         sa = sa.first(la);
         sb = sb.first(lb);
-        if ((words64(sa) + words64(sb)) * 64 <= HOST_MAX_TOTAL_BITS) {
-            Digits r = mul_limbs(sa, sb, same);
+        const Item it{i, za, zb, sa, sb, same};
+        if ((words64(sa) + words64(sb)) * 64 <= HOST_MAX_TOTAL_BITS) small.push_back(it);
+        else big.push_back(it);
+    }
+    // Host-sized products (the reference's brute range) run on the init_thread_pool threads while
+    // the calling thread drives the engine for the rest; every result slot is written by one
+    // thread only.
+    auto host_range = [&](size_t lo, size_t hi) {
+        for (size_t k = lo; k < hi; ++k) {
+            const Item& g = small[k];
+            Digits r = mul_limbs(g.a, g.b, g.same);
             while (r.size() > 1 && r.back() == 0) r.pop_back();
-            r.insert(r.begin(), za + zb, 0u);
-            res[i] = std::move(r);
-        } else {
-            big.push_back(Big{i, za, zb, sa, sb, same});
+            r.insert(r.begin(), g.za + g.zb, 0u);
+            res[g.idx] = std::move(r);
         }
+    };
+    const size_t nthreads =
+        small.size() < 4096 ? 1 : std::min<size_t>(thread_pool_size(), small.size() / 2048);
+    std::vector<std::thread> workers;
+    if (nthreads > 1) {
+        const size_t per = (small.size() + nthreads - 1) / nthreads;
+        for (size_t t = 0; t < nthreads; ++t)
+            workers.emplace_back(host_range, std::min(small.size(), t * per),
+                                 std::min(small.size(), (t + 1) * per));
     }
-    if (big.empty()) return res;
-    std::vector<backend::Job> jobs(big.size());
-    for (size_t k = 0; k < big.size(); ++k) {
-        Big& g = big[k];
-        Digits& r = res[g.idx];
-        r.assign(g.za + g.zb + g.a.size() + g.b.size(), 0u);
-        jobs[k] = backend::Job{g.a, g.same ? g.a : g.b, g.same, r.data() + g.za + g.zb, 0};
+    try {
+        if (nthreads <= 1) host_range(0, small.size());
+        if (!big.empty()) {
+            std::vector<backend::Job> jobs(big.size());
+            for (size_t k = 0; k < big.size(); ++k) {
+                Item& g = big[k];
+                Digits& r = res[g.idx];
+                r.assign(g.za + g.zb + g.a.size() + g.b.size(), 0u);
+                jobs[k] = backend::Job{g.a, g.same ? g.a : g.b, g.same, r.data() + g.za + g.zb, 0};
+            }
+            backend::multiply(jobs, plan_mode);
+            g_gpu_products.fetch_add(big.size(), std::memory_order_relaxed);
+            for (size_t k = 0; k < big.size(); ++k)
+                res[big[k].idx].resize(big[k].za + big[k].zb + jobs[k].out_len);
+        }
+    } catch (...) {
+        for (auto& w : workers) w.join();
+        throw;
     }
-    backend::multiply(jobs, plan_mode);
-    g_gpu_products.fetch_add(big.size(), std::memory_order_relaxed);
-    for (size_t k = 0; k < big.size(); ++k) res[big[k].idx].resize(big[k].za + big[k].zb + jobs[k].out_len);
+    for (auto& w : workers) w.join();
     return res;
 }
 

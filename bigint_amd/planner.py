@@ -82,10 +82,12 @@ class Timer:
         return float(np.median(times))
 
 
-def sample_shapes(count: int, seed: int, lo: float = 10.0, hi: float = 28.0, max_batch_log: int = 6,
+def sample_shapes(count: int, seed: int, lo: float = 10.0, hi: float = 28.0, max_batch_log: int = 10,
                   mem_limit_bytes: float = 24e9):
-    """Log-uniform operand sizes in [2^lo, 2^hi] bits and log-uniform batch sizes, with the batch
-    capped so the residues of one launch stay within mem_limit_bytes."""
+    """Log-uniform operand sizes in [2^lo, 2^hi] bits and log-uniform batch sizes up to
+    2^max_batch_log (product-tree levels put hundreds of products of one length in a launch, where
+    the GPU is saturated and the best plan differs from a lone product's), with the batch capped
+    so the residues of one launch stay within mem_limit_bytes."""
     rng = np.random.default_rng(seed)
     out = []
     while len(out) < count:
@@ -158,7 +160,18 @@ def build_table(recs) -> np.ndarray:
 
 # ---- MLP ----------------------------------------------------------------------------------------
 
+def plan_cost(la: int, lb: int, plan: int, square: bool = False) -> float:
+    """The engine's transform-count cost of a plan (bi_hip_plan_cost); -1 if it does not apply."""
+    L = engine.lib()
+    L.bi_hip_plan_cost.restype = ctypes.c_double
+    L.bi_hip_plan_cost.argtypes = [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]
+    return float(L.bi_hip_plan_cost(la, lb, int(square), plan))
+
+
 def dataset(recs):
+    """Inputs (log2 bits a, log2 bits b, log2 batch); targets log2(time / transform-count cost)
+    per plan, so the model learns the smooth part (efficiency by size, batch and plan shape)
+    while the engine adds back the cost's steps at powers of two when it chooses."""
     shapes = group_best(recs)
     X, T, M = [], [], []
     for (la, lb, k), plans in shapes.items():
@@ -166,7 +179,7 @@ def dataset(recs):
         t = np.zeros(K, np.float32)
         m = np.zeros(K, np.float32)
         for p, us in plans.items():
-            t[p] = math.log2(us)
+            t[p] = math.log2(us) - math.log2(plan_cost(la, lb, p))
             m[p] = 1.0
         T.append(t)
         M.append(m)

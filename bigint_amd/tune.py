@@ -42,8 +42,10 @@ def evaluate(recs_h, table, params, norm):
     P.install(table, params, norm)
     ch_table = P.choose([(la, lb, k) for la, lb, k in keys], engine.PLAN_TABLE)
     ch_mlp = P.choose([(la, lb, k) for la, lb, k in keys], engine.PLAN_LEARNED)
+    ch_heur = P.choose([(la, lb, k) for la, lb, k in keys], engine.PLAN_FIXED)  # built-in heuristic
     out = {}
-    for name, ch in [("plain", [0] * len(keys)), ("table", ch_table), ("mlp", ch_mlp)]:
+    for name, ch in [("plain", [0] * len(keys)), ("heuristic", ch_heur), ("table", ch_table),
+                     ("mlp", ch_mlp)]:
         reg, sp, tot_c, tot_p, hit = [], [], 0.0, 0.0, 0
         for key, c in zip(keys, ch):
             plans = shapes[key]
@@ -73,6 +75,9 @@ def main() -> int:
     ap.add_argument("--out", default=P.DEFAULT_PATH)
     ap.add_argument("--report", default=None)
     ap.add_argument("--records", default=None, help="save raw timing records (JSON lines)")
+    ap.add_argument("--from-records", default=None,
+                    help="train and evaluate on saved records (JSON lines with a 'split' field) "
+                         "instead of timing new shapes")
     args = ap.parse_args()
 
     engine.init(0)
@@ -89,8 +94,15 @@ def main() -> int:
         return res
 
     t0 = time.time()
-    train_shapes = shapes(args.train, args.seed)
-    recs = P.collect(train_shapes, timer, log)
+    if args.from_records:
+        with open(args.from_records) as f:
+            allrecs = [json.loads(line) for line in f if line.strip()]
+        recs = [r for r in allrecs if r.get("split") == "train"]
+        saved_heldout = [r for r in allrecs if r.get("split") == "heldout"]
+        train_shapes = sorted({(r["la"], r["lb"], r["k"]) for r in recs})
+    else:
+        train_shapes = shapes(args.train, args.seed)
+        recs = P.collect(train_shapes, timer, log)
     log(f"training timings: {len(recs)} runs over {len(train_shapes)} shapes in {time.time() - t0:.0f}s")
     table = P.build_table(recs)
     X, T, M = P.dataset(recs)
@@ -98,10 +110,13 @@ def main() -> int:
     params, norm, hist = P.train_gpu(X, T, M, epochs=args.epochs, seed=args.seed, log=log)
     log(f"MLP trained on the GPU in {time.time() - t1:.1f}s, final loss {hist[-1][1]:.5f}")
 
-    held = shapes(args.heldout, args.seed + 1)
     t2 = time.time()
-    recs_h = P.collect(held, timer, log)
-    log(f"held-out timings: {len(recs_h)} runs over {len(held)} shapes in {time.time() - t2:.0f}s")
+    if args.from_records:
+        recs_h = saved_heldout
+    else:
+        held = shapes(args.heldout, args.seed + 1)
+        recs_h = P.collect(held, timer, log)
+    log(f"held-out timings: {len(recs_h)} runs in {time.time() - t2:.0f}s")
     ev = evaluate(recs_h, table, params, norm)
     log(json.dumps(ev))
     meta = {"train_runs": len(recs), "train_shapes": len(train_shapes), "heldout": ev,

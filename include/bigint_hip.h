@@ -117,11 +117,12 @@ int bi_hip_transform_log2(size_t la, size_t lb);
  * HIP events on the stream it runs on, and its algorithmic HBM bytes are accumulated. */
 enum {
     BI_HIP_K_FILL = 0,      /* status/counter clearing */
-    BI_HIP_K_OUTER_FWD = 1, /* strided forward passes (first one reads limbs) */
+    BI_HIP_K_OUTER_FWD = 1, /* strided forward passes after the first */
     BI_HIP_K_MID = 2,       /* contiguous block pass: forward + pointwise + inverse */
     BI_HIP_K_OUTER_INV = 3, /* strided inverse passes */
     BI_HIP_K_GARNER = 4,    /* Garner CRT + carry look-back scan */
-    BI_HIP_K_COUNT = 5
+    BI_HIP_K_OUTER_FIRST = 5, /* first strided forward pass (reads limbs, all three primes) */
+    BI_HIP_K_COUNT = 6
 };
 typedef struct bi_hip_kstat {
     double ms;       /* summed event time */
@@ -147,6 +148,9 @@ int bi_hip_profile_read(bi_hip_kstat* stats, size_t cap);
 int bi_hip_plan_count(void);
 /* 1 if plan_id is valid for an la x lb product (is_square: b is a), else 0. */
 int bi_hip_plan_allowed(size_t la, size_t lb, int is_square, int plan_id);
+/* Transform-count cost of a plan (transforms x n log2 n; -1 if the plan does not apply): the
+ * heuristic's measure, and the baseline the learned model predicts time relative to. */
+double bi_hip_plan_cost(size_t la, size_t lb, int is_square, int plan_id);
 /* Device batch with an explicit plan id per pair (-1 = the engine's heuristic). */
 int bi_hip_mul_batch_device_planned(const bi_hip_dpair* pairs, size_t count,
                                     const int32_t* plan_ids, void* stream);

@@ -11,12 +11,14 @@
 //   frecip A pa prec | fprint A pa digits | ftodouble A pa | ffromdouble X (hex bits)
 //   fround A pa mode prec rel
 //   counts                   -> "<gpu products> <host products>"
+//   batch N SEED MAXBITS     -> checksum of BigInt::mul_batch over N SplitMix64 pairs (see below)
 #include <bit>
 #include <cstdio>
 #include <cstring>
 #include <iostream>
 #include <sstream>
 #include <string>
+#include <vector>
 
 #include "bigint/bigint.h"
 
@@ -147,6 +149,55 @@ int main() {
                 BigFloat f = F(a, pa);
                 f.round((RoundMode)mode, prec, (RoundRelativeTo)rel);
                 out << XF(f);
+            } else if (op == "batch") {
+                // pair i: a, b of 1 + next % MAXBITS bits (top bit set) from SplitMix64(SEED);
+                // every 7th pair squares a.  Checksum: sum (i+1) * (product mod 2^61-1).
+                size_t N;
+                uint64_t seed, maxbits;
+                in >> N >> seed >> maxbits;
+                uint64_t st = seed;
+                auto next = [&st] {
+                    uint64_t z = (st += 0x9e3779b97f4a7c15ull);
+                    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+                    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+                    return z ^ (z >> 31);
+                };
+                auto operand = [&] {
+                    const uint64_t bits = 1 + next() % maxbits;
+                    std::string hex;
+                    for (uint64_t w = 0; w < (bits + 63) / 64; ++w) {
+                        uint64_t v = next();
+                        const uint64_t top = bits - 64 * w;
+                        if (top <= 64) {
+                            if (top < 64) v &= (1ull << top) - 1;
+                            v |= 1ull << (top - 1);
+                        }
+                        char buf[17];
+                        std::snprintf(buf, sizeof buf, "%016llx", (unsigned long long)v);
+                        hex = std::string(buf) + hex;
+                    }
+                    return H(hex);
+                };
+                std::vector<BigInt> A, B;
+                A.reserve(N);
+                B.reserve(N);
+                for (size_t i = 0; i < N; ++i) {
+                    A.push_back(operand());
+                    B.push_back(i % 7 == 0 ? BigInt(0) : operand());
+                }
+                std::vector<std::pair<const BigInt*, const BigInt*>> pairs(N);
+                for (size_t i = 0; i < N; ++i) pairs[i] = {&A[i], i % 7 == 0 ? &A[i] : &B[i]};
+                const std::vector<BigInt> P = BigInt::mul_batch(pairs, 1);
+                const unsigned __int128 M = (((unsigned __int128)1) << 61) - 1;
+                unsigned __int128 sum = 0;
+                for (size_t i = 0; i < N; ++i) {
+                    // product mod 2^61 - 1 from its hex digits
+                    const std::string h = X(P[i]);
+                    unsigned __int128 r = 0;
+                    for (char c : h) r = (r * 16 + (uint64_t)(c <= '9' ? c - '0' : c - 'a' + 10)) % M;
+                    sum = (sum + r * (i + 1)) % M;
+                }
+                out << (uint64_t)sum;
             } else if (op == "counts") {
                 out << mul::gpu_product_count() << " " << mul::host_product_count();
             } else {

@@ -227,3 +227,48 @@ def test_large_reciprocal_matches_model(calc):
         a = rng.getrandbits(28 * 700) | 1
         f = BF.from_int(a, -28 * 650)
         assert calc("frecip", hx(a), 650, prec) == xf(BF.reciprocal(f, prec))
+
+
+def _splitmix(seed):
+    st = seed
+    M = (1 << 64) - 1
+    while True:
+        st = (st + 0x9e3779b97f4a7c15) & M
+        z = st
+        z = ((z ^ (z >> 30)) * 0xbf58476d1ce4e5b9) & M
+        z = ((z ^ (z >> 27)) * 0x94d049bb133111eb) & M
+        yield z ^ (z >> 31)
+
+
+def _batch_checksum(n, seed, maxbits):
+    g = _splitmix(seed)
+
+    def operand():
+        bits = 1 + next(g) % maxbits
+        v = 0
+        for w in range((bits + 63) // 64):
+            x = next(g)
+            top = bits - 64 * w
+            if top <= 64:
+                if top < 64:
+                    x &= (1 << top) - 1
+                x |= 1 << (top - 1)
+            v |= x << (64 * w)
+        return v
+
+    M = (1 << 61) - 1
+    s = 0
+    for i in range(n):
+        a = operand()
+        b = a if i % 7 == 0 else operand()
+        s = (s + (a * b % M) * (i + 1)) % M
+    return s
+
+
+def test_batch_of_host_products_on_pool_threads(calc):
+    """BigInt::mul_batch with 20,000 host-sized pairs (< 16,384 total bits): the products run on
+    the init_thread_pool threads; each slot must hold exactly its own product."""
+    g0, h0 = map(int, calc("counts").split())
+    assert int(calc("batch", 20000, 77, 8000)) == _batch_checksum(20000, 77, 8000)
+    g1, h1 = map(int, calc("counts").split())
+    assert g1 == g0 and h1 - h0 == 20000

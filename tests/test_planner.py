@@ -204,3 +204,30 @@ def test_balanced_2e28_every_plan_agrees():
         assert L.residue_mod(outs[0], q) == L.residue_mod(a, q) * L.residue_mod(b, q) % q
     for p, o in zip(plans[1:], outs[1:]):
         assert np.array_equal(o, outs[0]), p
+
+
+@pytest.mark.gpu
+def test_trained_planner_gains_over_plain():
+    """The committed planner (table + MLP trained on this engine's timings) against plan 0 on
+    fresh shapes it was not trained on: time every allowed plan, then compare the total time of
+    the learned choices with plan 0 and with the best plan."""
+    import os
+    from bigint_amd import engine, planner
+    engine.init(0)
+    if not os.path.exists(planner.DEFAULT_PATH):
+        pytest.skip("no tuned planner committed")
+    planner.load()
+    timer = planner.Timer(max_limbs=1 << 23)
+    shapes = [s for s in planner.sample_shapes(60, seed=424242, hi=24.0, max_batch_log=8)
+              if s[2] * (s[0] + s[1]) <= timer.out_words - 64]
+    choice = planner.choose(shapes, engine.PLAN_LEARNED)
+    t_plain = t_learned = t_best = 0.0
+    for (la, lb, k), c in zip(shapes, choice):
+        times = {p: timer.run(la, lb, k, p, reps=2) * k for p in planner.allowed(la, lb)}
+        assert c in times
+        t_plain += times[0]
+        t_learned += times[c]
+        t_best += min(times.values())
+    print(f"plain {t_plain:.0f} us, learned {t_learned:.0f} us, best {t_best:.0f} us")
+    assert t_plain / t_learned > 1.05
+    assert t_learned / t_best < 1.25
