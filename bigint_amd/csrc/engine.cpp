@@ -238,6 +238,8 @@ public:
     // device workspace of at least `words` u32
     uint32_t* workspace(size_t words) {
         if (words > ws_words) {
+            // queued kernels may still use the old buffer: let them finish before it goes
+            if (ws && hipDeviceSynchronize() != hipSuccess) return nullptr;
             if (ws) hipFree(ws);
             ws = nullptr;
             ws_words = 0;
@@ -247,7 +249,7 @@ public:
         return ws;
     }
 
-    enum Kind { kPlain = 0, kSquareKind = 1, kBForward = 2, kChunk = 3 };
+    enum Kind { kPlain = 0, kSquareKind = 1, kBForward = 2, kChunk = 3, kInverse = 4 };
 
     // Launch the pass sequence of one plan for njobs jobs of the same transform length.
     //   kPlain / kSquareKind: product of A and B (or A^2), Garner+carry into the output;
@@ -267,6 +269,7 @@ public:
             return BI_HIP_EDEVICE;
         const bool one_op = kind != kPlain;
         const double nops_d = one_op ? 1.0 : 2.0;
+        const bool fwd_passes = kind != kInverse;  // kInverse: res_a already holds a spectrum
         PassArgs a{};
         a.job = single;
         a.jobs = d_jobs;
@@ -275,7 +278,7 @@ public:
         const uint32_t base_flags = kind == kSquareKind ? (uint32_t)kSquare
                                   : (kind == kPlain ? 0u : (uint32_t)kSingle);
         const uint32_t nops = one_op ? 1u : 2u;
-        for (size_t k = 0; k < pl->outer.size(); ++k) {
+        for (size_t k = 0; fwd_passes && k < pl->outer.size(); ++k) {
             const OuterPass& op = pl->outer[k];
             a.log_r = op.log_r;
             a.log_c = op.log_c;
@@ -296,12 +299,13 @@ public:
         a.flags = (kind == kSquareKind ? (uint32_t)kSquare : 0u) |
                   (kind == kBForward ? (uint32_t)kFwdOnly : 0u) |
                   (kind == kChunk ? (uint32_t)kBPre : 0u) |
-                  (pl->outer.empty() ? (uint32_t)kFromLimbs : 0u);
+                  (kind == kInverse ? (uint32_t)kInvOnly : 0u) |
+                  (pl->outer.empty() && fwd_passes ? (uint32_t)kFromLimbs : 0u);
         a.beta = pl->mid_beta;
         a.beta_inv = pl->mid_beta_inv;
         a.beta_stride = pl->mid_nblk;
         {
-            double in_bytes = pl->outer.empty() ? 4.0 * in_limbs : 12.0 * n * nops_d * nj;
+            double in_bytes = pl->outer.empty() && fwd_passes ? 4.0 * in_limbs : 12.0 * n * nops_d * nj;
             if (kind == kChunk) in_bytes += 12.0 * n * nj;  // the shared transformed B
             if (timed(BI_HIP_K_MID, in_bytes + 12.0 * n * nj, st,
                       [&] { return launch_mid_pass(a, njobs, st); }) != hipSuccess)
@@ -326,7 +330,9 @@ public:
         g.jobs = d_gjobs;
         g.k = gk;
         if (timed(BI_HIP_K_GARNER, 12.0 * n * nj + 4.0 * out_limbs, st,
-                  [&] { return launch_garner_carry(g, max_out_len, ngjobs, kind == kChunk, st); }) != hipSuccess)
+                  [&] {
+                      return launch_garner_carry(g, max_out_len, ngjobs, kind == kChunk || kind == kInverse, st);
+                  }) != hipSuccess)
             return BI_HIP_EDEVICE;
         return BI_HIP_OK;
     }
@@ -336,9 +342,11 @@ public:
     int run(const uint32_t* d_a, uint32_t la, const uint32_t* d_b, uint32_t lb, bool square,
             uint32_t* d_out, hipStream_t st, bool planned = true) {
         const int lg0 = bi_hip_transform_log2(la, lb);
-        if (lg0 < 0) return BI_HIP_ETOOBIG;
-        // the table (or the heuristic) may pick a chunked or re-split plan: those run as a batch
+        if (!product_supported(la, lb)) return BI_HIP_ETOOBIG;
+        // the table (or the heuristic) may pick a chunked or re-split plan, and products beyond
+        // one transform need one: those run through the batch path
         const bi_hip_dpair p{d_a, la, square ? nullptr : d_b, lb, d_out, (size_t)la + lb};
+        if (lg0 < 0) return run_batch(&p, 1, BI_HIP_PLAN_TABLE, nullptr, st);
         std::vector<PairPlan> chosen;
         if (planned) {
             const int rc = choose_plans(&p, 1, BI_HIP_PLAN_TABLE, nullptr, chosen, st);
@@ -389,8 +397,24 @@ public:
         uint32_t lg;         // log2 of the transform length
         uint32_t chunk_len;  // 0: ordinary product; else words per chunk of the longer operand
         uint32_t mid_lr;     // log2 of the block-pass size
+        bool split_both = false;  // both operands cut into kSplitChunk-word chunks (see below)
     };
-    static constexpr int kNumPlanIds = 8;
+
+    // Products beyond one 2^26-point transform whose shorter operand is long too: both operands
+    // are cut into 2^25-word chunks, every chunk is transformed once, the pointwise products of
+    // chunk pairs are summed per diagonal (i + k = s) in the transform domain, and each diagonal
+    // is inverse-transformed once (the reference switches to its SSA tier there, mul.h:199-205).
+    // Coefficients stay below min(wa, wb) 2^64 <= 2^90 < P0 P1 P2 while min(wa, wb) <= 2^26.
+    static constexpr uint32_t kSplitLog = 25;
+    static constexpr uint32_t kSplitChunk = 1u << kSplitLog;
+    static bool product_supported(size_t la, size_t lb) {
+        if (la == 0 || lb == 0 || la > 0xFFFFFFFFull || lb > 0xFFFFFFFFull) return false;
+        if (la + lb > 0xFFFFFFFFull) return false;
+        if (bi_hip_transform_log2(la, lb) >= 0) return true;
+        return std::min(words_of((uint32_t)la), words_of((uint32_t)lb)) <= (1u << 26);
+    }
+    static constexpr int kNumPlanIds = 8;     // the plan space of the table and the MLP
+    static constexpr int kSplitPlanId = 8;    // both operands chunked (explicit, or beyond the rest)
 
     static uint32_t ceil_log2(uint64_t x) {
         uint32_t c = 0;
@@ -403,10 +427,15 @@ public:
     // 6/7 = the longer operand split into 2 / 3 chunks (multiples of 4096 limbs).  Chunk plans
     // are allowed only where their transform is shorter than the unchunked one.
     static bool plan_for_id(uint32_t la, uint32_t lb, bool sq, int id, PairPlan& out) {
-        const int lg0 = bi_hip_transform_log2(la, lb);
-        if (lg0 < 0 || id < 0 || id >= kNumPlanIds) return false;
+        const int lg0 = bi_hip_transform_log2(la, lb);  // -1: beyond one transform
+        if (id == kSplitPlanId && product_supported(la, lb)) {
+            out = PairPlan{26, kSplitChunk, 12, true};
+            return true;
+        }
+        if (id < 0 || id >= kNumPlanIds || !product_supported(la, lb)) return false;
         std::vector<uint32_t> rs;
         if (id <= 2) {
+            if (lg0 < 0) return false;
             const uint32_t mid = 12u - (uint32_t)id;
             if (id > 0 && (uint32_t)lg0 <= mid) return false;  // same as plan 0
             if (!split_outer((uint32_t)lg0, mid, rs)) return false;
@@ -428,7 +457,7 @@ public:
         }
         if (L >= hi) return false;
         const uint32_t lg = std::max<uint32_t>(4, ceil_log2((uint64_t)L + lo - 1));
-        if (lg >= (uint32_t)lg0 || lg > 26) return false;
+        if ((lg0 >= 0 && lg >= (uint32_t)lg0) || lg > 26) return false;
         const uint32_t mid = std::min<uint32_t>(12, lg);
         if (!split_outer(lg, mid, rs)) return false;
         out = PairPlan{lg, L, mid};
@@ -451,8 +480,9 @@ public:
 
     static PairPlan heuristic_plan(uint32_t la, uint32_t lb, bool sq) {
         PairPlan best{};
-        plan_for_id(la, lb, sq, 0, best);
-        double best_cost = plan_cost(la, lb, sq, 0);
+        double best_cost = 1e300;
+        if (plan_for_id(la, lb, sq, 0, best)) best_cost = plan_cost(la, lb, sq, 0);
+        else best = PairPlan{26, kSplitChunk, 12, true};  // beyond one transform
         for (int id = 3; id < kNumPlanIds; ++id) {
             PairPlan c{};
             if (!plan_for_id(la, lb, sq, id, c)) continue;
@@ -624,6 +654,7 @@ public:
         using Key = std::tuple<int, uint32_t, uint32_t, int>;  // phase, lg, mid_lr, kind
         std::map<Key, std::vector<uint32_t>> groups;           // -> item indices
         std::vector<Item> items(count);
+        std::vector<uint32_t> split;  // products run one by one with both operands chunked
         size_t bhat_words = 0;
         for (size_t i = 0; i < count; ++i) {
             const bi_hip_dpair& p = pairs[i];
@@ -632,10 +663,12 @@ public:
             if (!p.a || p.la == 0 || lb == 0 || !p.out || p.la > 0xFFFFFFFFull || lb > 0xFFFFFFFFull)
                 return BI_HIP_EINVAL;
             if (p.out_cap < p.la + lb) return BI_HIP_ECAP;
-            if (bi_hip_transform_log2(p.la, lb) < 0) return BI_HIP_ETOOBIG;
+            if (!product_supported(p.la, lb)) return BI_HIP_ETOOBIG;
             Item it{(uint32_t)i, (uint32_t)p.la, (uint32_t)lb, sq, false, chosen[i], 1, 0};
             const uint32_t lg = it.pp.lg, mid = it.pp.mid_lr;
-            if (it.pp.chunk_len) {
+            if (it.pp.split_both) {
+                split.push_back((uint32_t)i);
+            } else if (it.pp.chunk_len) {
                 it.swap = it.la < it.lb;
                 if (it.swap) std::swap(it.la, it.lb);
                 it.nchunks = (words_of(it.la) + it.pp.chunk_len - 1) / it.pp.chunk_len;
@@ -667,6 +700,7 @@ public:
         uint32_t* w = workspace(bhat_words + grp_words + 64);
         if (!w) return BI_HIP_ENOMEM;
         if (desc_need > desc_bytes) {
+            if (desc && hipDeviceSynchronize() != hipSuccess) return BI_HIP_EDEVICE;
             if (desc) hipFree(desc);
             desc = nullptr;
             desc_bytes = 0;
@@ -779,7 +813,82 @@ public:
                                            L.in_limbs, L.out_limbs, L.status, L.status_words, st);
             if (rc) return rc;
         }
+        for (uint32_t i : split) {
+            const bi_hip_dpair& p = pairs[i];
+            const bool sq = p.b == nullptr || (p.b == p.a && p.lb == p.la);
+            const int rc = run_split_both(p.a, (uint32_t)p.la, sq ? p.a : p.b, (uint32_t)(sq ? p.la : p.lb),
+                                          sq, p.out, st);
+            if (rc) return rc;
+        }
         return BI_HIP_OK;
+    }
+
+    // One product with both operands cut into kSplitChunk-word chunks (see PairPlan): forward
+    // transforms of every chunk (the transforms of a square's B are A's), diagonal sums in the
+    // transform domain, one inverse transform per diagonal, and Garner summing the diagonals,
+    // which overlap like the chunk products of a chunked plan.
+    int run_split_both(const uint32_t* A, uint32_t la, const uint32_t* B, uint32_t lb, bool sq,
+                       uint32_t* out, hipStream_t st) {
+        const uint32_t lg = 26, L = kSplitChunk;
+        const size_t n = (size_t)1 << lg;
+        const uint32_t ca = (words_of(la) + L - 1) / L, cb = sq ? ca : (words_of(lb) + L - 1) / L;
+        const uint32_t nd = ca + cb - 1;
+        Plan* pl = plan(lg, 12);
+        if (!pl) return BI_HIP_ENOMEM;
+        const uint32_t out_len = la + lb;
+        const uint32_t nt = (out_len + 4095u) / 4096u;
+        const size_t nspec = ca + (sq ? 0 : cb);
+        uint32_t* w = workspace((nspec + nd) * 3 * n + nt + 64);
+        if (!w) return BI_HIP_ENOMEM;
+        uint32_t* spec_a = w;
+        uint32_t* spec_b = sq ? spec_a : w + (size_t)ca * 3 * n;
+        uint32_t* diag = w + nspec * 3 * n;
+        uint32_t* status = diag + (size_t)nd * 3 * n;
+        std::vector<PassJob> fwd, inv(nd);
+        for (uint32_t c = 0; c < ca; ++c)
+            fwd.push_back(PassJob{A, nullptr, la, 0, spec_a + (size_t)c * 3 * n, nullptr, c * L, L});
+        if (!sq)
+            for (uint32_t c = 0; c < cb; ++c)
+                fwd.push_back(PassJob{B, nullptr, lb, 0, spec_b + (size_t)c * 3 * n, nullptr, c * L, L});
+        for (uint32_t s = 0; s < nd; ++s)
+            inv[s] = PassJob{nullptr, nullptr, 0, 0, diag + (size_t)s * 3 * n, nullptr, 0u, ~0u};
+        const size_t fwd_bytes = fwd.size() * sizeof(PassJob), inv_off = (fwd_bytes + 255) & ~(size_t)255;
+        const size_t need = inv_off + inv.size() * sizeof(PassJob);
+        if (need > desc_bytes) {
+            if (desc && hipDeviceSynchronize() != hipSuccess) return BI_HIP_EDEVICE;
+            if (desc) hipFree(desc);
+            desc = nullptr;
+            desc_bytes = 0;
+            if (hipMalloc((void**)&desc, need) != hipSuccess) return BI_HIP_ENOMEM;
+            desc_bytes = need;
+        }
+        if (hipStreamSynchronize(st) != hipSuccess ||
+            hipMemcpy(desc, fwd.data(), fwd_bytes, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(desc + inv_off, inv.data(), inv.size() * sizeof(PassJob), hipMemcpyHostToDevice) != hipSuccess)
+            return BI_HIP_EDEVICE;
+        const auto* d_fwd = reinterpret_cast<const PassJob*>(desc);
+        const auto* d_inv = reinterpret_cast<const PassJob*>(desc + inv_off);
+        GarnerJob g{};
+        int rc = launch_products(pl, kBForward, (uint32_t)fwd.size(), fwd[0], fwd.size() == 1 ? nullptr : d_fwd,
+                                 0, g, nullptr, 0, (double)la + (sq ? 0.0 : (double)lb), 0.0, nullptr, 0, st);
+        if (rc) return rc;
+        if (timed(BI_HIP_K_MID, 12.0 * n * (2.0 * ca * cb + nd), st, [&] {
+                return launch_diag_accumulate(spec_a, spec_b, ca, cb, (uint32_t)n, diag, st);
+            }) != hipSuccess)
+            return BI_HIP_EDEVICE;
+        g.y = diag;
+        g.out = out;
+        g.n = (uint32_t)n;
+        g.out_len = out_len;
+        for (int I = 0; I < 3; ++I) g.kscale[I] = pl->kscale[I];
+        g.status = status;
+        g.counter = status + nt;
+        g.nchunks = nd;
+        g.chunk_len = L;
+        g.log_chunk = kSplitLog;
+        g.chunk_stride = 3 * n;
+        return launch_products(pl, kInverse, nd, inv[0], nd == 1 ? nullptr : d_inv, 1, g, nullptr, out_len,
+                               0.0, (double)out_len, status, nt + 1, st);
     }
 };
 
@@ -843,7 +952,7 @@ int bi_hip_mul(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, int i
     }
     if (a[la - 1] == 0 || b[lb - 1] == 0) return BI_HIP_EINVAL;
     if (out_cap < la + lb) return BI_HIP_ECAP;
-    if (bi_hip_transform_log2(la, lb) < 0) return BI_HIP_ETOOBIG;
+    if (!bi::Engine::product_supported(la, lb)) return BI_HIP_ETOOBIG;
     std::lock_guard<std::mutex> lk(engine().mtx);
     bi::Engine& e = engine();
     int rc = e.init(0);
@@ -919,7 +1028,7 @@ int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode) {
             p.status = BI_HIP_ECAP;
             return BI_HIP_ECAP;
         }
-        if (bi_hip_transform_log2(p.la, p.lb) < 0) {
+        if (!bi::Engine::product_supported(p.la, p.lb)) {
             p.status = BI_HIP_ETOOBIG;
             return BI_HIP_ETOOBIG;
         }

@@ -392,6 +392,18 @@ __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, 
         bf = a.beta[I * a.beta_stride + blk];
         bi_ = a.beta_inv[I * a.beta_stride + blk];
     }
+    if (a.flags & kInvOnly) {  // a product spectrum from the diagonal sums: inverse stages only
+        build_twiddles<I, LR>(tw, bpow, wi, bi_, T);
+        using Sx = Sched<LR>;
+        constexpr LimbSrc kPl{nullptr, 0u, 0u, 0u};
+        uint32_t* p = job.res_a + (size_t)I * n;
+        uint32_t x[kE];
+        global_io<LR, 0, Sx::stages(Sx::NG - 1), Sx::trlo(Sx::NG - 1), false>(p, kPl, x, tile_base, 0);
+        __syncthreads();
+        tile_transform<I, true, LR, 0>(x, lds, tw);
+        global_io<LR, 0, Sx::stages(0), Sx::trlo(0), true>(p, kPl, x, tile_base, 0);
+        return;
+    }
     build_twiddles<I, LR>(tw, bpow, wf, bf, T);
 
     const bool from_limbs = (a.flags & kFromLimbs) != 0;
@@ -809,6 +821,35 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     }
 }
 
+// Diagonal sums of chunk spectra (see launch_diag_accumulate).  grid.x covers a plane,
+// grid.y = diagonal, grid.z = prime.  P0 spectra arrive in [0, 4P) from the lazy forward stages.
+template <int I>
+__device__ __forceinline__ void diag_body(const uint32_t* __restrict__ sa, const uint32_t* __restrict__ sb,
+                                          uint32_t ca, uint32_t cb, uint32_t n, uint32_t* __restrict__ d,
+                                          uint32_t s, uint32_t e) {
+    const size_t stride = (size_t)3 * n, off = (size_t)I * n + e;
+    const uint32_t i_lo = s + 1u > cb ? s + 1u - cb : 0u;
+    const uint32_t i_hi = s < ca - 1u ? s : ca - 1u;
+    uint32_t acc = 0;
+    for (uint32_t i = i_lo; i <= i_hi; ++i) {
+        const uint32_t x = sa[i * stride + off], y = sb[(s - i) * stride + off];
+        if constexpr (I == 0) acc = add_mod<0>(acc, fold1p(pointwise<0>(x, y)));
+        else acc = add_mod<I>(acc, mont_mul<I>(x, y));
+    }
+    d[s * stride + off] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_diag_accumulate(const uint32_t* sa, const uint32_t* sb, uint32_t ca,
+                                                         uint32_t cb, uint32_t n, uint32_t* d) {
+    const uint32_t e = blockIdx.x * 256u + threadIdx.x;
+    if (e >= n) return;
+    switch (blockIdx.z) {
+        case 0: diag_body<0>(sa, sb, ca, cb, n, d, blockIdx.y, e); break;
+        case 1: diag_body<1>(sa, sb, ca, cb, n, d, blockIdx.y, e); break;
+        default: diag_body<2>(sa, sb, ca, cb, n, d, blockIdx.y, e); break;
+    }
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t count) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) p[i] = v;
@@ -856,6 +897,14 @@ hipError_t launch_mid_pass(const PassArgs& a, uint32_t njobs, hipStream_t st) {
     const uint32_t threads = (1u << a.log_r) / kE;
     dim3 grid(nblk, njobs, kNumPrimes);
     hipLaunchKernelGGL(tab[a.log_r - 4], grid, dim3(threads), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_diag_accumulate(const uint32_t* spec_a, const uint32_t* spec_b, uint32_t ca,
+                                  uint32_t cb, uint32_t n, uint32_t* diag, hipStream_t st) {
+    if (ca == 0 || cb == 0 || n < 256) return hipErrorInvalidValue;
+    dim3 grid(n / 256u, ca + cb - 1u, kNumPrimes);
+    hipLaunchKernelGGL(k_diag_accumulate, grid, dim3(256), 0, st, spec_a, spec_b, ca, cb, n, diag);
     return hipGetLastError();
 }
 

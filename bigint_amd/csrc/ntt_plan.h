@@ -12,6 +12,7 @@ enum PassFlags : uint32_t {
     kSingle = 4u,     // strided forward passes transform operand A only
     kBPre = 8u,       // block pass: res_b already holds B's full forward transform
     kFwdOnly = 16u,   // block pass: forward-transform A and store it (no product, no inverse)
+    kInvOnly = 32u,   // block pass: res_a holds a product spectrum; inverse stages only
 };
 
 // Transform inputs are 32-bit words of the operands' bit strings (word g = bits [32g, 32g+32)),
@@ -84,5 +85,10 @@ hipError_t launch_mid_pass(const PassArgs& a, uint32_t njobs, hipStream_t st);
 hipError_t launch_garner_carry(const GarnerArgs& a, uint32_t max_out_len, uint32_t njobs,
                                bool chunked, hipStream_t st);
 hipError_t launch_fill(uint32_t* p, uint32_t v, uint32_t count, hipStream_t st);
+// Diagonal sums of chunk spectra: diag[s] = sum over i + k = s of spec_a[i] * spec_b[k]
+// (pointwise Montgomery products mod each prime), s = 0 .. ca + cb - 2.  Spectra and diagonals
+// are 3 planes of n words each, consecutive.
+hipError_t launch_diag_accumulate(const uint32_t* spec_a, const uint32_t* spec_b, uint32_t ca,
+                                  uint32_t cb, uint32_t n, uint32_t* diag, hipStream_t st);
 
 }  // namespace bi

@@ -141,7 +141,10 @@ int bi_hip_profile_read(bi_hip_kstat* stats, size_t cap);
  *   0, 1, 2  unchunked, 4096 / 2048 / 1024-point block pass
  *   3, 4, 5  longer operand cut into 2^c, 2^(c+1), 2^(c+2)-limb chunks (2^c = smallest power of two
  *            >= the shorter operand)
- *   6, 7     longer operand cut into 2 / 3 chunks (lengths rounded up to multiples of 4096 limbs)
+ *   6, 7     longer operand cut into 2 / 3 chunks (lengths multiples of the 3584-word carry tile)
+ *   8        both operands cut into 2^25-word chunks on 2^26-point transforms, chunk-pair spectra
+ *            summed per diagonal (outside the learned plan space; used for products beyond one
+ *            transform that no chunk plan fits, and accepted as an explicit id)
  * A chunked product transforms the shorter operand once and multiplies every chunk with that
  * transform; chunk plans are valid only where their transform is shorter than the unchunked one.
  * These replace the reference's hand-set size thresholds (include/bigint/mul.h:80, 153, 192). */

@@ -127,7 +127,8 @@ def test_error_paths():
     with pytest.raises(engine.EngineError) as e:
         engine.mul(np.array([1, 0], np.uint32), a)  # top limb zero
     assert e.value.code == engine.EINVAL
-    big = np.ones(38347924, np.uint32)
+    # beyond what three primes can reconstruct: both operands above 2^26 words
+    big = np.ones(76695851, np.uint32)
     with pytest.raises(engine.EngineError) as e:
         engine.mul(big, big)
     assert e.value.code == engine.ETOOBIG
@@ -238,3 +239,35 @@ def test_carry_runs_across_tiles():
     # and the square of such a run, for carries born inside the product
     p2 = engine.mul(L.int_to_limbs(a), None)
     assert L.limbs_to_int(p2) == a * a
+
+
+def check_residues3(a, b, p):
+    for q in L.RESIDUE_PRIMES[:3]:
+        ra = L.residue_mod(a, q)
+        rb = ra if b is None else L.residue_mod(b, q)
+        assert L.residue_mod(p, q) == ra * rb % q, q
+
+
+def test_beyond_one_transform_both_operands_chunked():
+    """50.75 M words (1.62 Gbit) per operand: beyond one 2^26-point transform, and too long for
+    any chunk plan of one operand, so both are cut into 2^25-word chunks (4 forward transforms, 3
+    diagonal sums, 3 inverse transforms); also the square, whose B chunks are A's."""
+    la = lb = 58_000_000
+    assert engine.transform_log2(la, lb) == -1
+    a = L.random_operand(28 * la - 5, 31)
+    b = L.random_operand(28 * lb - 9, 32)
+    p = engine.mul(a, b)
+    assert p.size in (la + lb, la + lb - 1)
+    check_residues3(a, b, p)
+    check_residues3(a, None, engine.mul(a, None))
+
+
+def test_beyond_one_transform_long_times_short():
+    """A 2.24-Gbit operand (70 M words, beyond one transform) times a 28-Mbit one: chunk plans
+    on 2^21-point transforms with one shared transform of the short operand."""
+    la, lb = 80_000_000, 1_000_000
+    assert engine.transform_log2(la, lb) == -1
+    a = L.random_operand(28 * la, 41)
+    b = L.random_operand(28 * lb, 42)
+    p = engine.mul(a, b)
+    check_residues3(a, b, p)

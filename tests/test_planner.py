@@ -132,6 +132,15 @@ def test_plan_allowed_rules():
     assert planner.allowed(9586981, 9586981) == [0, 1, 2]
     assert set(planner.allowed(70000, 20000)) >= {0, 6, 7}
     assert set(planner.allowed(200000, 150000)) >= {0, 6}
+    # beyond one 2^26-point transform: no unchunked plan; a short operand still allows chunks,
+    # two long ones run the engine's both-sides split (no plan id)
+    assert min(planner.allowed(38347925, 38347925)) >= 6  # A cut in 2 or 3, B whole
+    al = planner.allowed(80_000_000, 1_000_000)
+    assert al and min(al) >= 3
+    assert planner.allowed(58_000_000, 58_000_000) == []  # only the both-sides split fits
+    assert engine.lib().bi_hip_plan_allowed(58_000_000, 58_000_000, 0, 8)
+    assert engine.lib().bi_hip_plan_allowed(1000, 1000, 1, 8)  # explicit id: any product
+    assert not engine.lib().bi_hip_plan_allowed(76695851, 76695851, 0, 8)  # beyond 3 primes
 
 
 def _planned(pairs_ids):
@@ -231,3 +240,30 @@ def test_trained_planner_gains_over_plain():
     print(f"plain {t_plain:.0f} us, learned {t_learned:.0f} us, best {t_best:.0f} us")
     assert t_plain / t_learned > 1.05
     assert t_learned / t_best < 1.25
+
+
+@pytest.mark.gpu
+def test_split_both_plan_is_exact():
+    """Plan 8 (both operands in 2^25-word chunks, diagonal sums of chunk spectra) on products
+    that also have ordinary plans: 2^25 + 5 words times 2^25 + 3 words (two chunks each, the
+    second ones tiny), all-ones operands, against plan 0's product and the residue checks."""
+    from bigint_amd import engine
+    from bigint_amd import limbs as L
+    engine.init(0)
+    def limbs_for_words(w):
+        la = (8 * w) // 7
+        while (la * 7 + 7) // 8 > w:
+            la -= 1
+        while ((la + 1) * 7 + 7) // 8 == w:
+            la += 1
+        return la
+    la, lb = limbs_for_words((1 << 25) + 5), limbs_for_words((1 << 25) + 3)
+    a = L.random_operand(28 * la, 91)
+    b = L.random_operand(28 * lb - 11, 92)
+    p8, p6 = _planned([(a, b, 8), (a, b, 6)])
+    assert np.array_equal(p8, p6)
+    for q in L.RESIDUE_PRIMES[:3]:
+        assert L.residue_mod(p8, q) == L.residue_mod(a, q) * L.residue_mod(b, q) % q
+    x, y = (1 << (28 * 300000)) - 1, (1 << (28 * 200000)) - 1
+    (p,) = _planned([(L.int_to_limbs(x), L.int_to_limbs(y), 8)])
+    assert L.limbs_to_int(p) == x * y
