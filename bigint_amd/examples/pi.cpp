@@ -46,9 +46,13 @@ static std::vector<Term> leaves(size_type r) {
     return t;
 }
 
-// Level-wise binary splitting; returns (P, Q) of terms 1..r.
+// Level-wise binary splitting; returns (P, Q) of terms 1..r.  With PI_LEVELS set in the
+// environment, one line per level goes to stderr (merges, largest Q in limbs, product and
+// merge-addition milliseconds).
 static std::pair<BigInt, BigInt> split(size_type r, int plan_mode, double& gpu_levels_ms) {
+    const bool verbose = std::getenv("PI_LEVELS") != nullptr;
     std::vector<Term> cur = leaves(r);
+    int level = 0;
     while (cur.size() > 1) {
         const size_type m = cur.size() / 2;
         const bool top = cur.size() <= 2;
@@ -64,7 +68,9 @@ static std::pair<BigInt, BigInt> split(size_type r, int plan_mode, double& gpu_l
         }
         const auto t0 = clk::now();
         std::vector<BigInt> prod = BigInt::mul_batch(jobs, plan_mode);
-        gpu_levels_ms += std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+        const double pms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+        gpu_levels_ms += pms;
+        const auto t1 = clk::now();
         std::vector<Term> next;
         next.reserve(m + 1);
         const size_type per = top ? 3 : 4;
@@ -75,6 +81,11 @@ static std::pair<BigInt, BigInt> split(size_type r, int plan_mode, double& gpu_l
                                 top ? BigInt(0) : std::move(prod[per * k + 3])});
         }
         if (cur.size() % 2) next.push_back(std::move(cur.back()));
+        if (verbose)
+            std::cerr << "level " << level << ": merges " << m << ", largest Q limbs "
+                      << next.front().Q.get_data().size() << ", products ms " << pms << ", merge ms "
+                      << std::chrono::duration<double, std::milli>(clk::now() - t1).count() << "\n";
+        ++level;
         cur = std::move(next);
     }
     return {std::move(cur[0].P), std::move(cur[0].Q)};
@@ -96,6 +107,7 @@ int main(int argc, char** argv) {
     double levels_ms = 0;
     auto [P, Q] = split(dec_digits / 14 + 5, plan_mode, levels_ms);
     const auto t_split = clk::now();
+    const double eng_split = mul::engine_time_ms();
 
     BigFloat pi(Q);
     pi *= 4270934400;
@@ -114,6 +126,7 @@ int main(int argc, char** argv) {
     pi = BigFloat::mul(pi, s, inner);
     pi = BigFloat::mul(pi, BigFloat(13591409 * Q + P).reciprocal(inner), inner);
     const auto t2 = clk::now();
+    const double eng_compute = mul::engine_time_ms();
 
     {
         std::ofstream out(out_path, std::ios::out);
@@ -125,6 +138,8 @@ int main(int argc, char** argv) {
               << ", \"planner_loaded\": " << (planner ? "true" : "false")
               << ", \"compute_ms\": " << ms(t1, t2) << ", \"split_ms\": " << ms(t1, t_split)
               << ", \"split_product_ms\": " << levels_ms << ", \"output_ms\": " << ms(t2, t3)
+              << ", \"engine_ms\": {\"split\": " << eng_split << ", \"compute\": " << eng_compute
+              << ", \"output\": " << mul::engine_time_ms() - eng_compute << "}"
               << ", \"gpu_products\": " << mul::gpu_product_count()
               << ", \"host_products\": " << mul::host_product_count() << "}" << std::endl;
     return 0;

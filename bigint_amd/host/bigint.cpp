@@ -12,6 +12,7 @@
 #include <cstring>
 #include <iomanip>
 #include <iostream>
+#include <map>
 #include <mutex>
 #include <optional>
 #include <thread>
@@ -154,38 +155,91 @@ void BigInt::print(std::ostream& output, bool hex, bool direct) const {
     if (!direct) output << buf->rdbuf();
 }
 
+// Same digits as the reference's recursive split (src/bigint.cpp:126-265): x = q 10^k + r with
+// k half of x's decimal length, down to pieces below DEC_STRING_BRUTE_THRESHOLDS[1] limbs.
+// The decimal string of a number is unique, so three things here only change the speed:
+//   - 10^k and its reciprocal are computed once per distinct k (nodes of one depth share a few
+//     k values) instead of once per division, and the quotient is corrected until exact;
+//   - the brute-force conversions of the pieces run on the init_thread_pool threads;
+//   - digits are written in blocks.
 void BigInt::print_dec(std::ostream& output, size_type len) const {
-    auto emit = [&](const BigInt& x, size_type width) {
-        const std::string s = x.to_dec_string_brute();
-        size_type body = s.size();
-        auto it = s.begin();
-        if (x.is_neg_) {
-            output.put(*it++);
-            --body;
-        }
-        for (size_type i = body; i < width; ++i) output.put('0');
-        for (; it != s.end(); ++it) output.put(*it);
-    };
     const size_type thr = DEC_STRING_BRUTE_THRESHOLDS[1];
-    if (data_.size() == 1 || data_.size() < thr) {
-        emit(*this, len);
-        return;
+    auto small = [&](const BigInt& x) { return x.data_.size() == 1 || x.data_.size() < thr; };
+    std::vector<std::pair<BigInt, size_type>> pieces;  // in output order
+    if (small(*this)) {
+        pieces.emplace_back(*this, len);
+    } else {
+        struct Div {
+            BigInt d;
+            BigFloat recip;
+            size_type prec;
+        };
+        std::map<size_type, Div> divs;
+        std::vector<std::pair<BigInt, size_type>> stack;
+        auto split = [&](const BigInt& x, size_type width) {
+            const size_type est = (size_type)std::ceil((double)x.data_.size() * DIGIT_BITS * std::log10(2.0));
+            const size_type k = est / 2;
+            const size_type prec = x.data_.size() + 16;
+            auto it = divs.find(k);
+            if (it == divs.end()) {
+                BigInt d = get_pow_of_ten((uint32_t)k);
+                BigFloat rc = BigFloat(d).reciprocal(prec + 2);
+                it = divs.emplace(k, Div{std::move(d), std::move(rc), prec + 2}).first;
+            } else if (it->second.prec < prec) {
+                it->second.recip = BigFloat(it->second.d).reciprocal(prec + 2);
+                it->second.prec = prec + 2;
+            }
+            const BigInt& d = it->second.d;
+            BigInt q(BigFloat::mul(BigFloat(x), it->second.recip, prec), RoundMode::Truncate);
+            BigInt r(x - q * d);
+            // exact truncated division of |x| by d: r in [0, d) with the sign of x
+            const bool neg = x.is_neg_;
+            while (!r.is_zero() && r.is_neg_ != neg) {
+                r += neg ? -d : d;
+                neg ? ++q : --q;
+            }
+            while (compare_abs(r, d) >= 0) {
+                r -= neg ? -d : d;
+                neg ? --q : ++q;
+            }
+            r.remove_sign();
+            stack.emplace_back(std::move(r), k);
+            stack.emplace_back(std::move(q), width > k ? width - k : 0);
+        };
+        split(*this, len);
+        while (!stack.empty()) {
+            auto [x, width] = std::move(stack.back());
+            stack.pop_back();
+            if (small(x)) pieces.emplace_back(std::move(x), width);
+            else split(x, width);
+        }
     }
-    std::vector<std::pair<BigInt, size_type>> stack;
-    auto split = [&](const BigInt& x, size_type width) {
-        const size_type est = (size_type)std::ceil((double)x.data_.size() * DIGIT_BITS * std::log10(2.0));
-        const size_type k = est / 2;
-        auto [q, r] = x.divmod(get_pow_of_ten((uint32_t)k), RoundMode::Truncate);
-        r.remove_sign();
-        stack.emplace_back(std::move(r), k);
-        stack.emplace_back(std::move(q), width > k ? width - k : 0);
+    std::vector<std::string> strs(pieces.size());
+    std::atomic<size_t> next{0};
+    auto work = [&] {
+        for (size_t i; (i = next.fetch_add(1)) < pieces.size();) strs[i] = pieces[i].first.to_dec_string_brute();
     };
-    split(*this, len);
-    while (!stack.empty()) {
-        auto [x, width] = std::move(stack.back());
-        stack.pop_back();
-        if (x.data_.size() == 1 || x.data_.size() < thr) emit(x, width);
-        else split(x, width);
+    const size_t nt = std::min<size_t>(thread_pool_size(), pieces.size());
+    std::vector<std::thread> threads;
+    for (size_t t = 1; t < nt; ++t) threads.emplace_back(work);
+    work();
+    for (auto& t : threads) t.join();
+    const std::string zeros(4096, '0');
+    for (size_t i = 0; i < pieces.size(); ++i) {
+        const std::string& str = strs[i];
+        const size_type width = pieces[i].second;
+        size_type body = str.size(), start = 0;
+        if (pieces[i].first.is_neg_) {
+            output.put(str[0]);
+            --body;
+            start = 1;
+        }
+        for (size_type pad = body < width ? width - body : 0; pad;) {
+            const size_type c = std::min<size_type>(pad, zeros.size());
+            output.write(zeros.data(), (std::streamsize)c);
+            pad -= c;
+        }
+        output.write(str.data() + start, (std::streamsize)body);
     }
 }
 

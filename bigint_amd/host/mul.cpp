@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstring>
 #include <new>
 #include <thread>
@@ -16,6 +17,16 @@ namespace {
 
 std::atomic<uint64_t> g_gpu_products{0};
 std::atomic<uint64_t> g_host_products{0};
+std::atomic<uint64_t> g_backend_ns{0};
+
+// backend::multiply with its wall time added to g_backend_ns
+void timed_multiply(std::vector<backend::Job>& jobs, int plan_mode) {
+    const auto t0 = std::chrono::steady_clock::now();
+    backend::multiply(jobs, plan_mode);
+    g_backend_ns.fetch_add((uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                               std::chrono::steady_clock::now() - t0).count(),
+                           std::memory_order_relaxed);
+}
 
 // 64-bit words of the value held in limbs (what the reference's digits_to_vec64 produces after
 // dropping leading zero words, mul.h:245-268).
@@ -60,6 +71,7 @@ auto schoolbook(std::span<const uint32_t> a, std::span<const uint32_t> b) -> Dig
 
 auto gpu_product_count() noexcept -> uint64_t { return g_gpu_products.load(); }
 auto host_product_count() noexcept -> uint64_t { return g_host_products.load(); }
+auto engine_time_ms() noexcept -> double { return (double)g_backend_ns.load() * 1e-6; }
 
 auto load_planner(const char* path) -> bool { return backend::load_planner(path); }
 
@@ -77,7 +89,7 @@ auto mul_limbs(std::span<const uint32_t> a, std::span<const uint32_t> b, bool sq
     while (lb > 1 && b[lb - 1] == 0) --lb;
     Digits r(la + lb);
     std::vector<backend::Job> jobs{backend::Job{a.first(la), b.first(lb), square, r.data(), 0}};
-    backend::multiply(jobs, BI_HIP_PLAN_TABLE);
+    timed_multiply(jobs, BI_HIP_PLAN_TABLE);
     g_gpu_products.fetch_add(1, std::memory_order_relaxed);
     r.resize(jobs[0].out_len);
     return r;
@@ -178,7 +190,7 @@ auto mul_digits_batch(const std::vector<std::pair<const Digits*, const Digits*>>
                 r.assign(g.za + g.zb + g.a.size() + g.b.size(), 0u);
                 jobs[k] = backend::Job{g.a, g.same ? g.a : g.b, g.same, r.data() + g.za + g.zb, 0};
             }
-            backend::multiply(jobs, plan_mode);
+            timed_multiply(jobs, plan_mode);
             g_gpu_products.fetch_add(big.size(), std::memory_order_relaxed);
             for (size_t k = 0; k < big.size(); ++k)
                 res[big[k].idx].resize(big[k].za + big[k].zb + jobs[k].out_len);

@@ -40,6 +40,8 @@ auto mul_digits(const Digits& a, const Digits& b, size_type output_precision = 0
 // Number of products routed to the GPU engine / multiplied on the host since start-up.
 auto gpu_product_count() noexcept -> uint64_t;
 auto host_product_count() noexcept -> uint64_t;
+// Wall time spent inside the product backend (engine calls, including staging copies).
+auto engine_time_ms() noexcept -> double;
 
 // Install a tuned plan table and cost model (the binary file bigint_amd/planner.py writes) for
 // the engine's plan modes 1 (table) and 2 (learned); false if the backend has no planner or the

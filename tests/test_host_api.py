@@ -217,7 +217,12 @@ def test_large_ops_through_engine(calc):
     assert g1 > g0  # the big products went through the GPU engine
     a = rng.getrandbits(200000) | 1
     assert calc("todec", hx(a)) == str(a)
+    assert calc("todec", hx(-a)) == str(-a)
     assert unhx(calc("fromdec", str(a))) == a
+    # many pieces, with runs of zero digits that need padding inside the split
+    z = 10 ** 70000 + 12345 * 10 ** 30000 + 7
+    assert calc("todec", hx(z)) == str(z)
+    assert calc("todec", hx(-z)) == str(-z)
 
 
 @pytest.mark.gpu
