@@ -5,7 +5,8 @@
 // precision-truncated products, decimal printing), so the digits are the reference's.  The
 // splitting itself runs level by level: the reference merges terms with a streaming binary
 // counter (examples/pi.cpp:13-80), one `operator*` at a time; here every level's products go to
-// the engine together (BigInt::mul_batch).  The merge (P, Q, R) o (P', Q', R') =
+// the engine together (BigInt::mul_batch), and the leaves and each level's additions are spread
+// over the init_thread_pool threads.  The merge (P, Q, R) o (P', Q', R') =
 // (P Q' + R P', Q Q', R R') is associative, so any bracketing gives the same P(1, r) and Q(1, r).
 //
 // usage: pi <decimal digits> [plan mode 0|1|2] [output file] [planner file]
@@ -14,6 +15,7 @@
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
+#include <thread>
 #include <vector>
 
 #include "bigint/bigint.h"
@@ -25,25 +27,44 @@ struct Term {
     BigInt P, Q, R;
 };
 
+// Runs make(lo, hi) on the init_thread_pool threads over [0, count) split into contiguous ranges
+// and concatenates the results in order.
+template <class F>
+static std::vector<Term> parallel_terms(size_type count, F make) {
+    const size_type nt = std::max<size_type>(1, std::min<size_type>(thread_pool_size(), count / 4096));
+    std::vector<std::vector<Term>> parts(nt);
+    std::vector<std::thread> threads;
+    const size_type per = (count + nt - 1) / nt;
+    for (size_type t = 0; t < nt; ++t)
+        threads.emplace_back([&, t] { parts[t] = make(std::min(count, t * per), std::min(count, (t + 1) * per)); });
+    for (auto& th : threads) th.join();
+    std::vector<Term> out;
+    out.reserve(count);
+    for (auto& p : parts)
+        for (auto& x : p) out.push_back(std::move(x));
+    return out;
+}
+
 // Leaf i (examples/pi.cpp:26-44): Q = Q0 i^3, R = (2i-1)(6i-1)(6i-5), P = (-1)^i R (13591409 +
 // 545140134 i) with the same 128-bit intermediate arithmetic.
 static std::vector<Term> leaves(size_type r) {
     constexpr uint64_t Q0 = 10939058860032000ull;  // 640320^3 / 24
-    std::vector<Term> t;
-    t.reserve(r);
-    uint64_t p0 = 13591409;
-    for (size_type i = 1; i <= r; ++i) {
-        BigInt q(Q0);
-        q *= i;
-        q *= i;
-        q *= i;
-        const auto rr = static_cast<uint128_t>(2 * i - 1) * (6 * i - 1) * (6 * i - 5);
-        p0 += 545140134;
-        BigInt p(rr * p0);
-        if (i % 2) p.flip_sign();
-        t.push_back(Term{std::move(p), std::move(q), BigInt(rr)});
-    }
-    return t;
+    return parallel_terms(r, [](size_type lo, size_type hi) {
+        std::vector<Term> t;
+        t.reserve(hi - lo);
+        for (size_type i = lo + 1; i <= hi; ++i) {
+            BigInt q(Q0);
+            q *= i;
+            q *= i;
+            q *= i;
+            const auto rr = static_cast<uint128_t>(2 * i - 1) * (6 * i - 1) * (6 * i - 5);
+            const uint64_t p0 = 13591409 + 545140134 * (uint64_t)i;
+            BigInt p(rr * p0);
+            if (i % 2) p.flip_sign();
+            t.push_back(Term{std::move(p), std::move(q), BigInt(rr)});
+        }
+        return t;
+    });
 }
 
 // Level-wise binary splitting; returns (P, Q) of terms 1..r.  With PI_LEVELS set in the
@@ -71,15 +92,18 @@ static std::pair<BigInt, BigInt> split(size_type r, int plan_mode, double& gpu_l
         const double pms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
         gpu_levels_ms += pms;
         const auto t1 = clk::now();
-        std::vector<Term> next;
-        next.reserve(m + 1);
         const size_type per = top ? 3 : 4;
-        for (size_type k = 0; k < m; ++k) {
-            BigInt P = std::move(prod[per * k]);
-            P += prod[per * k + 1];
-            next.push_back(Term{std::move(P), std::move(prod[per * k + 2]),
-                                top ? BigInt(0) : std::move(prod[per * k + 3])});
-        }
+        std::vector<Term> next = parallel_terms(m, [&](size_type lo, size_type hi) {
+            std::vector<Term> t;
+            t.reserve(hi - lo);
+            for (size_type k = lo; k < hi; ++k) {
+                BigInt P = std::move(prod[per * k]);
+                P += prod[per * k + 1];
+                t.push_back(Term{std::move(P), std::move(prod[per * k + 2]),
+                                 top ? BigInt(0) : std::move(prod[per * k + 3])});
+            }
+            return t;
+        });
         if (cur.size() % 2) next.push_back(std::move(cur.back()));
         if (verbose)
             std::cerr << "level " << level << ": merges " << m << ", largest Q limbs "

@@ -38,32 +38,20 @@ auto words64(std::span<const uint32_t> x) -> size_type {
     return (bits + 63) / 64;
 }
 
-// Schoolbook on 28-bit limbs with 64-bit accumulation: each partial product is < 2^56, so up to
-// 255 of them can be summed before the accumulator needs folding; the host route only takes
-// products of <= 256 words (<= 586 limbs), and the carry is folded every row anyway.
+// Schoolbook on 28-bit limbs by columns: each column sums at most 586 partial products below
+// 2^56 (the host route only takes products of <= 16,384 bits), so a 128-bit accumulator holds it
+// with the carry from the column below.  No scratch array: the product is written as it goes.
 auto schoolbook(std::span<const uint32_t> a, std::span<const uint32_t> b) -> Digits {
-    Digits r(a.size() + b.size(), 0);
-    std::vector<uint64_t> acc(a.size() + b.size() + 1, 0);
-    for (size_type i = 0; i < a.size(); ++i) {
-        const uint64_t ai = a[i];
-        if (ai == 0) continue;
-        for (size_type j = 0; j < b.size(); ++j) acc[i + j] += ai * b[j];
-        // fold so every accumulator stays below 2^60
-        uint64_t carry = 0;
-        for (size_type k = i; k < i + b.size() + 1; ++k) {
-            const uint64_t v = acc[k] + carry;
-            acc[k] = v & DIGIT_MASK;
-            carry = v >> DIGIT_BITS;
-        }
-        size_type k = i + b.size() + 1;
-        while (carry) {
-            const uint64_t v = acc[k] + carry;
-            acc[k] = v & DIGIT_MASK;
-            carry = v >> DIGIT_BITS;
-            ++k;
-        }
+    const size_type na = a.size(), nb = b.size();
+    Digits r(na + nb, 0);
+    unsigned __int128 acc = 0;
+    for (size_type k = 0; k + 1 < na + nb; ++k) {
+        const size_type lo = k + 1 > nb ? k + 1 - nb : 0, hi = std::min(k, na - 1);
+        for (size_type i = lo; i <= hi; ++i) acc += (uint64_t)a[i] * b[k - i];
+        r[k] = (uint32_t)acc & DIGIT_MASK;
+        acc >>= DIGIT_BITS;
     }
-    for (size_type k = 0; k < r.size(); ++k) r[k] = (uint32_t)acc[k];
+    r[na + nb - 1] = (uint32_t)acc;
     return r;
 }
 

@@ -200,6 +200,12 @@ def test_products_stay_on_host_below_16384_bits(calc):
     calc("mul", "f" * 100, "e" * 100)
     g1, h1 = map(int, calc("counts").split())
     assert (g1, h1) == (g0, h0 + 1)
+    # the largest host products, all-ones (every column of the schoolbook at its maximum)
+    for bits_a, bits_b in [(8192, 8192), (16000, 320), (64, 16300)]:
+        x, y = (1 << bits_a) - 1, (1 << bits_b) - 1
+        assert unhx(calc("mul", hx(x), hx(y))) == x * y
+    g2, _ = map(int, calc("counts").split())
+    assert g2 == g1
 
 
 @pytest.mark.gpu
