@@ -493,24 +493,24 @@ auto BigInt::bitwise_not(size_type len) -> BigInt& {
 
 auto BigInt::mul_batch(const std::vector<std::pair<const BigInt*, const BigInt*>>& pairs,
                        int plan_mode) -> std::vector<BigInt> {
-    std::vector<BigInt> res(pairs.size(), BigInt(0));
     std::vector<std::pair<const Digits*, const Digits*>> dp;
-    std::vector<size_t> where;
-    for (size_t i = 0; i < pairs.size(); ++i) {
-        const BigInt& a = *pairs[i].first;
-        const BigInt& b = *pairs[i].second;
-        if (a.is_zero() || b.is_zero()) continue;
-        dp.emplace_back(&a.data_, &b.data_);
-        where.push_back(i);
-    }
+    dp.reserve(pairs.size());
+    for (const auto& [pa, pb] : pairs)
+        if (!pa->is_zero() && !pb->is_zero()) dp.emplace_back(&pa->data_, &pb->data_);
     auto prods = mul::mul_digits_batch(dp, plan_mode);
-    for (size_t k = 0; k < where.size(); ++k) {
-        const size_t i = where[k];
+    std::vector<BigInt> res;
+    res.reserve(pairs.size());
+    size_t k = 0;
+    for (const auto& [pa, pb] : pairs) {
+        if (pa->is_zero() || pb->is_zero()) {
+            res.push_back(BigInt(0));
+            continue;
+        }
         BigInt r;
-        r.data_ = std::move(prods[k]);
-        r.is_neg_ = pairs[i].first->is_neg_ != pairs[i].second->is_neg_;
+        r.data_ = std::move(prods[k++]);
+        r.is_neg_ = pa->is_neg_ != pb->is_neg_;
         r.remove_leading_zero();
-        res[i] = std::move(r);
+        res.push_back(std::move(r));
     }
     return res;
 }

@@ -127,67 +127,70 @@ auto mul_digits_batch(const std::vector<std::pair<const Digits*, const Digits*>>
         std::span<const uint32_t> a, b;
         bool same;
     };
-    std::vector<Item> big, small;
-    for (size_t i = 0; i < pairs.size(); ++i) {
-        const Digits& a = *pairs[i].first;
-        const Digits& b = *pairs[i].second;
-        const bool same = &a == &b;
-        size_type za = 0, zb = 0;
-        while (a[za] == 0) ++za;
-        if (same) zb = za;
-        else
-            while (b[zb] == 0) ++zb;
-        std::span<const uint32_t> sa(a.data() + za, a.size() - za), sb(b.data() + zb, b.size() - zb);
-        size_type la = sa.size(), lb = sb.size();
-        while (la > 1 && sa[la - 1] == 0) --la;
-        while (lb > 1 && sb[lb - 1] == 0) --lb;
-        sa = sa.first(la);
-        sb = sb.first(lb);
-        const Item it{i, za, zb, sa, sb, same};
-        if ((words64(sa) + words64(sb)) * 64 <= HOST_MAX_TOTAL_BITS) small.push_back(it);
-        else big.push_back(it);
-    }
-    // Host-sized products (the reference's brute range) run on the init_thread_pool threads while
-    // the calling thread drives the engine for the rest; every result slot is written by one
-    // thread only.
-    auto host_range = [&](size_t lo, size_t hi) {
-        for (size_t k = lo; k < hi; ++k) {
-            const Item& g = small[k];
-            Digits r = mul_limbs(g.a, g.b, g.same);
-            while (r.size() > 1 && r.back() == 0) r.pop_back();
-            r.insert(r.begin(), g.za + g.zb, 0u);
-            res[g.idx] = std::move(r);
+    // Each thread of the init_thread_pool takes a contiguous range of pairs: host-sized products
+    // (the reference's brute range) are multiplied at once, the rest are collected for one engine
+    // batch.  Every result slot is written by one thread only.
+    const size_t npairs = pairs.size();
+    const size_t nthreads =
+        npairs < 8192 ? 1 : std::max<size_t>(1, std::min<size_t>(thread_pool_size(), npairs / 4096));
+    std::vector<std::vector<Item>> big_parts(nthreads);
+    std::vector<size_t> host_counts(nthreads, 0);
+    auto range = [&](size_t t, size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; ++i) {
+            const Digits& a = *pairs[i].first;
+            const Digits& b = *pairs[i].second;
+            const bool same = &a == &b;
+            size_type za = 0, zb = 0;
+            while (a[za] == 0) ++za;
+            if (same) zb = za;
+            else
+                while (b[zb] == 0) ++zb;
+            std::span<const uint32_t> sa(a.data() + za, a.size() - za), sb(b.data() + zb, b.size() - zb);
+            size_type la = sa.size(), lb = sb.size();
+            while (la > 1 && sa[la - 1] == 0) --la;
+            while (lb > 1 && sb[lb - 1] == 0) --lb;
+            sa = sa.first(la);
+            sb = sb.first(lb);
+            if ((words64(sa) + words64(sb)) * 64 <= HOST_MAX_TOTAL_BITS) {
+                Digits r = schoolbook(sa, same ? sa : sb);
+                while (r.size() > 1 && r.back() == 0) r.pop_back();
+                if (za + zb) r.insert(r.begin(), za + zb, 0u);
+                res[i] = std::move(r);
+                ++host_counts[t];
+            } else {
+                big_parts[t].push_back(Item{i, za, zb, sa, sb, same});
+            }
         }
     };
-    const size_t nthreads =
-        small.size() < 4096 ? 1 : std::min<size_t>(thread_pool_size(), small.size() / 2048);
-    std::vector<std::thread> workers;
-    if (nthreads > 1) {
-        const size_t per = (small.size() + nthreads - 1) / nthreads;
+    if (nthreads == 1) {
+        range(0, 0, npairs);
+    } else {
+        std::vector<std::thread> workers;
+        const size_t per = (npairs + nthreads - 1) / nthreads;
         for (size_t t = 0; t < nthreads; ++t)
-            workers.emplace_back(host_range, std::min(small.size(), t * per),
-                                 std::min(small.size(), (t + 1) * per));
-    }
-    try {
-        if (nthreads <= 1) host_range(0, small.size());
-        if (!big.empty()) {
-            std::vector<backend::Job> jobs(big.size());
-            for (size_t k = 0; k < big.size(); ++k) {
-                Item& g = big[k];
-                Digits& r = res[g.idx];
-                r.assign(g.za + g.zb + g.a.size() + g.b.size(), 0u);
-                jobs[k] = backend::Job{g.a, g.same ? g.a : g.b, g.same, r.data() + g.za + g.zb, 0};
-            }
-            timed_multiply(jobs, plan_mode);
-            g_gpu_products.fetch_add(big.size(), std::memory_order_relaxed);
-            for (size_t k = 0; k < big.size(); ++k)
-                res[big[k].idx].resize(big[k].za + big[k].zb + jobs[k].out_len);
-        }
-    } catch (...) {
+            workers.emplace_back(range, t, std::min(npairs, t * per), std::min(npairs, (t + 1) * per));
         for (auto& w : workers) w.join();
-        throw;
     }
-    for (auto& w : workers) w.join();
+    size_t nhost = 0, nbig = 0;
+    for (size_t t = 0; t < nthreads; ++t) {
+        nhost += host_counts[t];
+        nbig += big_parts[t].size();
+    }
+    g_host_products.fetch_add(nhost, std::memory_order_relaxed);
+    if (nbig == 0) return res;
+    std::vector<Item> big;
+    big.reserve(nbig);
+    for (auto& part : big_parts) big.insert(big.end(), part.begin(), part.end());
+    std::vector<backend::Job> jobs(big.size());
+    for (size_t k = 0; k < big.size(); ++k) {
+        Item& g = big[k];
+        Digits& r = res[g.idx];
+        r.assign(g.za + g.zb + g.a.size() + g.b.size(), 0u);
+        jobs[k] = backend::Job{g.a, g.same ? g.a : g.b, g.same, r.data() + g.za + g.zb, 0};
+    }
+    timed_multiply(jobs, plan_mode);
+    g_gpu_products.fetch_add(big.size(), std::memory_order_relaxed);
+    for (size_t k = 0; k < big.size(); ++k) res[big[k].idx].resize(big[k].za + big[k].zb + jobs[k].out_len);
     return res;
 }
 
