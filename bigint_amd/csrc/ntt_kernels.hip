@@ -139,11 +139,23 @@ __device__ __forceinline__ void group_stages(uint32_t (&x)[kE], const uint32_t* 
                     }
                 } else if (!INV) {
                     const uint32_t vw = mont_mul<I>(v, w);
-                    x[sg * EG + q] = add_mod<I>(u, vw);
-                    x[sg * EG + q + h] = sub_mod<I>(u, vw);
+                    // P1 < 2^31: outputs that are the multiplied operand ("v") of the next stage
+                    // in this register group stay in [0, 2P) (Montgomery accepts any 32-bit
+                    // input); the rest, and every output of a group's last stage, are reduced.
+                    constexpr bool kLazy1 = I == 1 && (int)hi + 1 < ST;
+                    if (kLazy1 && (q & (h >> 1)) != 0) {
+                        x[sg * EG + q] = u + vw;
+                        x[sg * EG + q + h] = u + kP[1] - vw;
+                    } else {
+                        x[sg * EG + q] = add_mod<I>(u, vw);
+                        x[sg * EG + q + h] = sub_mod<I>(u, vw);
+                    }
                 } else {
                     x[sg * EG + q] = add_mod<I>(u, v);
-                    x[sg * EG + q + h] = mont_mul<I>(sub_mod<I>(u, v), w);
+                    if constexpr (I == 1)  // u - v + P < 2P < 2^32 goes to Montgomery unreduced
+                        x[sg * EG + q + h] = mont_mul<I>(u + kP[1] - v, w);
+                    else
+                        x[sg * EG + q + h] = mont_mul<I>(sub_mod<I>(u, v), w);
                 }
             }
         }
