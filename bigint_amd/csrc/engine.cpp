@@ -121,9 +121,14 @@ public:
         if (!profiling) return launch();
         ProfRec r{kind, get_event(), get_event(), bytes};
         if (!r.e0 || !r.e1) return hipErrorOutOfMemory;
-        hipEventRecord(r.e0, st);
-        hipError_t e = launch();
-        hipEventRecord(r.e1, st);
+        hipError_t e = hipEventRecord(r.e0, st);
+        if (e == hipSuccess) e = launch();
+        if (e == hipSuccess) e = hipEventRecord(r.e1, st);
+        if (e != hipSuccess) {
+            event_pool.push_back(r.e0);
+            event_pool.push_back(r.e1);
+            return e;
+        }
         prof.push_back(r);
         return e;
     }
@@ -240,7 +245,7 @@ public:
         if (words > ws_words) {
             // queued kernels may still use the old buffer: let them finish before it goes
             if (ws && hipDeviceSynchronize() != hipSuccess) return nullptr;
-            if (ws) hipFree(ws);
+            if (ws) (void)hipFree(ws);
             ws = nullptr;
             ws_words = 0;
             if (hipMalloc((void**)&ws, words * 4) != hipSuccess) return nullptr;
@@ -389,6 +394,10 @@ public:
     // descriptor buffer for batched launches (device), grow-only
     uint8_t* desc = nullptr;
     size_t desc_bytes = 0;
+    // device staging of bi_hip_mul_batch (host operands in, products out), grow-only
+    std::mutex stage_mtx;
+    uint32_t* stage = nullptr;
+    size_t stage_words = 0;
 
     // Plan of one product: transform length, optional chunking of the longer operand, and the
     // size of the contiguous block pass (which fixes the radix split of the outer stages).
@@ -517,7 +526,8 @@ public:
 
     float* scratch(size_t words) {
         if (words > d_scratch_words) {
-            if (d_scratch) hipFree(d_scratch);
+            if (d_scratch && hipDeviceSynchronize() != hipSuccess) return nullptr;
+            if (d_scratch) (void)hipFree(d_scratch);
             d_scratch = nullptr;
             d_scratch_words = 0;
             if (hipMalloc((void**)&d_scratch, words * 4) != hipSuccess) return nullptr;
@@ -701,7 +711,7 @@ public:
         if (!w) return BI_HIP_ENOMEM;
         if (desc_need > desc_bytes) {
             if (desc && hipDeviceSynchronize() != hipSuccess) return BI_HIP_EDEVICE;
-            if (desc) hipFree(desc);
+            if (desc) (void)hipFree(desc);
             desc = nullptr;
             desc_bytes = 0;
             if (hipMalloc((void**)&desc, desc_need) != hipSuccess) return BI_HIP_ENOMEM;
@@ -856,7 +866,7 @@ public:
         const size_t need = inv_off + inv.size() * sizeof(PassJob);
         if (need > desc_bytes) {
             if (desc && hipDeviceSynchronize() != hipSuccess) return BI_HIP_EDEVICE;
-            if (desc) hipFree(desc);
+            if (desc) (void)hipFree(desc);
             desc = nullptr;
             desc_bytes = 0;
             if (hipMalloc((void**)&desc, need) != hipSuccess) return BI_HIP_ENOMEM;
@@ -959,7 +969,8 @@ int bi_hip_mul(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, int i
     if (rc) return rc;
     const size_t words = la + (is_square ? 0 : lb) + la + lb;
     if (words > e.io_words) {
-        if (e.io) hipFree(e.io);
+        if (e.io && hipDeviceSynchronize() != hipSuccess) return BI_HIP_EDEVICE;
+        if (e.io) (void)hipFree(e.io);
         e.io = nullptr;
         e.io_words = 0;
         if (hipMalloc((void**)&e.io, words * 4) != hipSuccess) return BI_HIP_ENOMEM;
@@ -1035,13 +1046,23 @@ int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode) {
         in_words += p.la + (sq ? 0 : p.lb);
         out_words += p.la + p.lb;
     }
-    uint32_t* dbuf = nullptr;
+    bi::Engine& e = engine();
     {
-        std::lock_guard<std::mutex> lk(engine().mtx);
-        int rc = engine().init(0);
+        std::lock_guard<std::mutex> lk(e.mtx);
+        int rc = e.init(0);
         if (rc) return rc;
     }
-    if (hipMalloc((void**)&dbuf, (in_words + out_words) * 4) != hipSuccess) return BI_HIP_ENOMEM;
+    // one staging call at a time: the device staging buffer is the engine's, grow-only
+    std::lock_guard<std::mutex> stage_lk(e.stage_mtx);
+    if (in_words + out_words > e.stage_words) {
+        if (e.stage && hipDeviceSynchronize() != hipSuccess) return BI_HIP_EDEVICE;
+        if (e.stage) (void)hipFree(e.stage);
+        e.stage = nullptr;
+        e.stage_words = 0;
+        if (hipMalloc((void**)&e.stage, (in_words + out_words) * 4) != hipSuccess) return BI_HIP_ENOMEM;
+        e.stage_words = in_words + out_words;
+    }
+    uint32_t* dbuf = e.stage;
     std::vector<uint32_t> hin(in_words);
     std::vector<bi_hip_dpair> dp(count);
     size_t io = 0, oo = in_words;
@@ -1074,7 +1095,6 @@ int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode) {
         if (hipMemcpy(hout.data(), dbuf + in_words, out_words * 4, hipMemcpyDeviceToHost) != hipSuccess)
             rc = BI_HIP_EDEVICE;
     }
-    hipFree(dbuf);
     size_t o = 0;
     for (size_t i = 0; i < count; ++i) {
         bi_hip_pair& p = pairs[i];
@@ -1160,7 +1180,8 @@ int bi_hip_set_planner(const int8_t* table, const float* mlp_params, int K, cons
         if (K <= 0 || K > bi::kMaxPlans || !norm6) return BI_HIP_EINVAL;
         const size_t np = (size_t)bi::mlp_param_count(K);
         if (np > e.d_mlp_words) {
-            if (e.d_mlp) hipFree(e.d_mlp);
+            if (e.d_mlp && hipDeviceSynchronize() != hipSuccess) return BI_HIP_EDEVICE;
+            if (e.d_mlp) (void)hipFree(e.d_mlp);
             e.d_mlp = nullptr;
             e.d_mlp_words = 0;
             if (hipMalloc((void**)&e.d_mlp, np * 4) != hipSuccess) return BI_HIP_ENOMEM;
@@ -1246,7 +1267,7 @@ int bi_hip_profile(int enable) {
     std::lock_guard<std::mutex> lk(engine().mtx);
     bi::Engine& e = engine();
     if (enable) {
-        if (e.ready) hipStreamSynchronize(e.stream);
+        if (e.ready && hipStreamSynchronize(e.stream) != hipSuccess) return BI_HIP_EDEVICE;
         e.prof_clear();
     }
     e.profiling = enable != 0;
