@@ -112,6 +112,23 @@ def batch_leg(hip, engine, L, pairs_count: int, steps: int) -> dict:
         stream.synchronize()
         dt = (time.perf_counter() - t0) / steps
         res[name] = {"ms": round(dt * 1e3, 3), "Gbit_s": round(sum(a + b for a, b in bits) / dt / 1e9, 2)}
+    # the learned mode again with stream launches instead of one captured graph per batch
+    engine.lib().bi_hip_set_graphs(0)
+    try:
+        base, ob = pool.ptr, outs[engine.PLAN_LEARNED].ptr
+        lp = [(base + 4 * int(offa[i]), int(la[i]), base + 4 * int(offb[i]), int(lb[i]),
+               ob + 4 * int(offa[i]), int(la[i] + lb[i])) for i in range(pairs_count)]
+        engine.mul_batch_device(lp, engine.PLAN_LEARNED, stream.handle)
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            engine.mul_batch_device(lp, engine.PLAN_LEARNED, stream.handle)
+        stream.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        res["learned_stream_launches"] = {"ms": round(dt * 1e3, 3),
+                                          "Gbit_s": round(sum(a + b for a, b in bits) / dt / 1e9, 2)}
+    finally:
+        engine.lib().bi_hip_set_graphs(1)
     # per-kernel device time of one learned-mode batch (engine's event timing)
     engine.profile(True)
     base, ob = pool.ptr, outs[engine.PLAN_LEARNED].ptr

@@ -245,6 +245,7 @@ public:
         if (words > ws_words) {
             // queued kernels may still use the old buffer: let them finish before it goes
             if (ws && hipDeviceSynchronize() != hipSuccess) return nullptr;
+            graph_reset();
             if (ws) (void)hipFree(ws);
             ws = nullptr;
             ws_words = 0;
@@ -391,6 +392,59 @@ public:
                                ntiles + 1, st);
     }
 
+    // Batches as captured graphs: on by default (bi_hip_set_graphs); the last executable graph is
+    // kept and updated when the next batch has the same shape.
+    bool graphs = true;
+    hipGraphExec_t graph_exec = nullptr;
+    hipStream_t graph_stream = nullptr;  // where graph_exec was last launched
+
+    // Drop the cached executable graph: called before any buffer it may reference is freed.
+    void graph_reset() {
+        if (!graph_exec) return;
+        if (graph_stream) (void)hipStreamSynchronize(graph_stream);
+        (void)hipGraphExecDestroy(graph_exec);
+        graph_exec = nullptr;
+        graph_stream = nullptr;
+    }
+
+    template <class F>
+    int launch_as_graph(F&& launch_all, hipStream_t st) {
+        if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return BI_HIP_EDEVICE;
+        const int rc = launch_all();
+        hipGraph_t g = nullptr;
+        const hipError_t end = hipStreamEndCapture(st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (end != hipSuccess || !g) return BI_HIP_EDEVICE;
+        bool ok = false;
+        if (graph_exec) {
+            // the previous launch of this executable graph must be done before it changes
+            if (graph_stream && graph_stream != st && hipStreamSynchronize(graph_stream) != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                return BI_HIP_EDEVICE;
+            }
+            hipGraphNode_t err_node = nullptr;
+            hipGraphExecUpdateResult res{};
+            ok = hipGraphExecUpdate(graph_exec, g, &err_node, &res) == hipSuccess &&
+                 res == hipGraphExecUpdateSuccess;
+            if (!ok) {  // another shape: a new executable graph (and no stale error left behind)
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(graph_exec);
+                graph_exec = nullptr;
+            }
+        }
+        if (!ok && hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            graph_exec = nullptr;
+            (void)hipGraphDestroy(g);
+            return BI_HIP_EDEVICE;
+        }
+        (void)hipGraphDestroy(g);
+        graph_stream = st;
+        return hipGraphLaunch(graph_exec, st) == hipSuccess ? BI_HIP_OK : BI_HIP_EDEVICE;
+    }
+
     // descriptor buffer for batched launches (device), grow-only
     uint8_t* desc = nullptr;
     size_t desc_bytes = 0;
@@ -527,6 +581,7 @@ public:
     float* scratch(size_t words) {
         if (words > d_scratch_words) {
             if (d_scratch && hipDeviceSynchronize() != hipSuccess) return nullptr;
+            graph_reset();
             if (d_scratch) (void)hipFree(d_scratch);
             d_scratch = nullptr;
             d_scratch_words = 0;
@@ -711,6 +766,7 @@ public:
         if (!w) return BI_HIP_ENOMEM;
         if (desc_need > desc_bytes) {
             if (desc && hipDeviceSynchronize() != hipSuccess) return BI_HIP_EDEVICE;
+            graph_reset();
             if (desc) (void)hipFree(desc);
             desc = nullptr;
             desc_bytes = 0;
@@ -812,17 +868,26 @@ public:
         if (hipStreamSynchronize(st) != hipSuccess) return BI_HIP_EDEVICE;
         if (hipMemcpy(desc, hdesc.data(), off, hipMemcpyHostToDevice) != hipSuccess)
             return BI_HIP_EDEVICE;
-        for (auto& L : launches) {
-            const auto* pj = reinterpret_cast<const PassJob*>(desc + L.pj_off);
-            const auto* gj = reinterpret_cast<const GarnerJob*>(desc + L.gj_off);
-            const auto* hp = reinterpret_cast<const PassJob*>(hdesc.data() + L.pj_off);
-            const GarnerJob g0 = L.ngjobs ? reinterpret_cast<const GarnerJob*>(hdesc.data() + L.gj_off)[0]
-                                          : GarnerJob{};
-            const int rc = launch_products(L.pl, L.kind, L.njobs, hp[0], L.njobs == 1 ? nullptr : pj,
-                                           L.ngjobs, g0, L.ngjobs == 1 ? nullptr : gj, L.max_out,
-                                           L.in_limbs, L.out_limbs, L.status, L.status_words, st);
-            if (rc) return rc;
-        }
+        auto launch_all = [&]() -> int {
+            for (auto& L : launches) {
+                const auto* pj = reinterpret_cast<const PassJob*>(desc + L.pj_off);
+                const auto* gj = reinterpret_cast<const GarnerJob*>(desc + L.gj_off);
+                const auto* hp = reinterpret_cast<const PassJob*>(hdesc.data() + L.pj_off);
+                const GarnerJob g0 = L.ngjobs ? reinterpret_cast<const GarnerJob*>(hdesc.data() + L.gj_off)[0]
+                                              : GarnerJob{};
+                const int rc = launch_products(L.pl, L.kind, L.njobs, hp[0], L.njobs == 1 ? nullptr : pj,
+                                               L.ngjobs, g0, L.ngjobs == 1 ? nullptr : gj, L.max_out,
+                                               L.in_limbs, L.out_limbs, L.status, L.status_words, st);
+                if (rc) return rc;
+            }
+            return BI_HIP_OK;
+        };
+        // A level's launch sets go to the GPU as one captured graph (kernels only: the status
+        // words are cleared by a fill kernel, not a memset node).  A graph of the same shape as
+        // the previous one is updated in place instead of instantiated again.
+        const bool use_graph = graphs && !profiling && launches.size() > 1;
+        const int rc = use_graph ? launch_as_graph(launch_all, st) : launch_all();
+        if (rc) return rc;
         for (uint32_t i : split) {
             const bi_hip_dpair& p = pairs[i];
             const bool sq = p.b == nullptr || (p.b == p.a && p.lb == p.la);
@@ -866,6 +931,7 @@ public:
         const size_t need = inv_off + inv.size() * sizeof(PassJob);
         if (need > desc_bytes) {
             if (desc && hipDeviceSynchronize() != hipSuccess) return BI_HIP_EDEVICE;
+            graph_reset();
             if (desc) (void)hipFree(desc);
             desc = nullptr;
             desc_bytes = 0;
@@ -970,6 +1036,7 @@ int bi_hip_mul(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, int i
     const size_t words = la + (is_square ? 0 : lb) + la + lb;
     if (words > e.io_words) {
         if (e.io && hipDeviceSynchronize() != hipSuccess) return BI_HIP_EDEVICE;
+        e.graph_reset();
         if (e.io) (void)hipFree(e.io);
         e.io = nullptr;
         e.io_words = 0;
@@ -1056,6 +1123,7 @@ int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode) {
     std::lock_guard<std::mutex> stage_lk(e.stage_mtx);
     if (in_words + out_words > e.stage_words) {
         if (e.stage && hipDeviceSynchronize() != hipSuccess) return BI_HIP_EDEVICE;
+        e.graph_reset();
         if (e.stage) (void)hipFree(e.stage);
         e.stage = nullptr;
         e.stage_words = 0;
@@ -1108,6 +1176,12 @@ int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode) {
         p.out_len = n;
     }
     return rc;
+}
+
+int bi_hip_set_graphs(int enable) {
+    std::lock_guard<std::mutex> lk(engine().mtx);
+    engine().graphs = enable != 0;
+    return BI_HIP_OK;
 }
 
 int bi_hip_plan_count(void) { return bi::Engine::kNumPlanIds; }
@@ -1181,6 +1255,7 @@ int bi_hip_set_planner(const int8_t* table, const float* mlp_params, int K, cons
         const size_t np = (size_t)bi::mlp_param_count(K);
         if (np > e.d_mlp_words) {
             if (e.d_mlp && hipDeviceSynchronize() != hipSuccess) return BI_HIP_EDEVICE;
+            e.graph_reset();
             if (e.d_mlp) (void)hipFree(e.d_mlp);
             e.d_mlp = nullptr;
             e.d_mlp_words = 0;

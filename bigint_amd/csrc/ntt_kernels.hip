@@ -898,6 +898,7 @@ hipError_t launch_outer_pass(const PassArgs& a, bool inv, uint32_t njobs, uint32
     // a first forward pass (from limbs) handles the three primes inside each block
     const bool fused = !inv && (a.flags & kFromLimbs);
     dim3 grid(ntiles, njobs, fused ? nops : kNumPrimes * nops);
+    (void)hipGetLastError();  // report this launch's error, not an earlier ignored one
     hipLaunchKernelGGL((fused ? first : inv ? bwd : fwd)[a.log_r - 1], grid, dim3(kThreads), 0, st, a);
     return hipGetLastError();
 }
@@ -908,6 +909,7 @@ hipError_t launch_mid_pass(const PassArgs& a, uint32_t njobs, hipStream_t st) {
     const uint32_t nblk = 1u << (a.log_n - a.log_r);
     const uint32_t threads = (1u << a.log_r) / kE;
     dim3 grid(nblk, njobs, kNumPrimes);
+    (void)hipGetLastError();  // report this launch's error, not an earlier ignored one
     hipLaunchKernelGGL(tab[a.log_r - 4], grid, dim3(threads), 0, st, a);
     return hipGetLastError();
 }
@@ -916,6 +918,7 @@ hipError_t launch_diag_accumulate(const uint32_t* spec_a, const uint32_t* spec_b
                                   uint32_t cb, uint32_t n, uint32_t* diag, hipStream_t st) {
     if (ca == 0 || cb == 0 || n < 256) return hipErrorInvalidValue;
     dim3 grid(n / 256u, ca + cb - 1u, kNumPrimes);
+    (void)hipGetLastError();  // report this launch's error, not an earlier ignored one
     hipLaunchKernelGGL(k_diag_accumulate, grid, dim3(256), 0, st, spec_a, spec_b, ca, cb, n, diag);
     return hipGetLastError();
 }
@@ -924,6 +927,7 @@ hipError_t launch_garner_carry(const GarnerArgs& a, uint32_t max_out_len, uint32
                                bool chunked, hipStream_t st) {
     const uint32_t ntiles = (max_out_len + kCarryTile - 1u) / kCarryTile;
     dim3 grid(ntiles, njobs, 1);
+    (void)hipGetLastError();  // report this launch's error, not an earlier ignored one
     hipLaunchKernelGGL(chunked ? k_garner_carry<true> : k_garner_carry<false>, grid, dim3(kThreads), 0,
                        st, a);
     return hipGetLastError();
@@ -931,6 +935,7 @@ hipError_t launch_garner_carry(const GarnerArgs& a, uint32_t max_out_len, uint32
 
 hipError_t launch_fill(uint32_t* p, uint32_t v, uint32_t count, hipStream_t st) {
     if (count == 0) return hipSuccess;
+    (void)hipGetLastError();  // report this launch's error, not an earlier ignored one
     hipLaunchKernelGGL(k_fill_u32, dim3((count + 255u) / 256u), dim3(256), 0, st, p, v, count);
     return hipGetLastError();
 }

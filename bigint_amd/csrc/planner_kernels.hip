@@ -235,6 +235,7 @@ int mlp_param_count(int K) { return mlp_nparams(K); }
 hipError_t mlp_forward(const float* x, int B, const float* p, int K, float* y, float* h1,
                        float* h2, hipStream_t st) {
     if (B <= 0) return hipSuccess;
+    (void)hipGetLastError();  // report this launch's error, not an earlier ignored one
     hipLaunchKernelGGL(k_mlp_forward, dim3((B + 127) / 128), dim3(128), 0, st, x, B, p, K, y, h1, h2);
     return hipGetLastError();
 }
@@ -245,11 +246,14 @@ hipError_t mlp_grad(const float* x, const float* t, const float* mask, int B, co
     hipError_t e = mlp_forward(x, B, p, K, y, h1, h2, st);
     if (e != hipSuccess) return e;
     const int nb = (B + kGradSamples - 1) / kGradSamples;
+    (void)hipGetLastError();  // report this launch's error, not an earlier ignored one
     hipLaunchKernelGGL(k_mlp_grad_partial, dim3(nb), dim3(256), 0, st, x, t, mask, B, p, K, y, h1,
                        h2, inv_count, part, part_loss);
     const int NP = mlp_nparams(K);
+    (void)hipGetLastError();  // report this launch's error, not an earlier ignored one
     hipLaunchKernelGGL(k_reduce_partials, dim3((NP + 255) / 256), dim3(256), 0, st, part, nb, NP, grad);
     // loss: reduce part_loss into grad[NP] (caller allocates NP + 1)
+    (void)hipGetLastError();  // report this launch's error, not an earlier ignored one
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1), 0, st, part_loss, nb, 1, grad + NP);
     return hipGetLastError();
 }
@@ -257,6 +261,7 @@ hipError_t mlp_grad(const float* x, const float* t, const float* mask, int B, co
 hipError_t mlp_adam(float* p, const float* g, float* m, float* v, int NP, float lr, float b1,
                     float b2, float eps, int step, hipStream_t st) {
     const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    (void)hipGetLastError();  // report this launch's error, not an earlier ignored one
     hipLaunchKernelGGL(k_adam, dim3((NP + 255) / 256), dim3(256), 0, st, p, g, m, v, NP, lr, b1, b2,
                        eps, c1, c2);
     return hipGetLastError();
@@ -265,6 +270,7 @@ hipError_t mlp_adam(float* p, const float* g, float* m, float* v, int NP, float 
 hipError_t mlp_choose(const float* y, const float* offset, const uint8_t* allowed, int B, int K,
                       int32_t* choice, hipStream_t st) {
     if (B <= 0) return hipSuccess;
+    (void)hipGetLastError();  // report this launch's error, not an earlier ignored one
     hipLaunchKernelGGL(k_argmin_plan, dim3((B + 255) / 256), dim3(256), 0, st, y, offset, allowed, B, K,
                        choice);
     return hipGetLastError();

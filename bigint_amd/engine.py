@@ -21,7 +21,7 @@ EXPORTS = ["bi_hip_init", "bi_hip_mul", "bi_hip_mul_device", "bi_hip_mul_batch",
            "bi_hip_profile_read", "bi_hip_mul_batch_device", "bi_hip_plan_count",
            "bi_hip_plan_allowed", "bi_hip_plan_cost", "bi_hip_mul_batch_device_planned", "bi_hip_choose_plans",
            "bi_hip_set_planner", "bi_hip_load_planner", "bi_hip_mlp_param_count",
-           "bi_hip_mlp_forward", "bi_hip_mlp_grad", "bi_hip_mlp_adam"]
+           "bi_hip_mlp_forward", "bi_hip_mlp_grad", "bi_hip_mlp_adam", "bi_hip_set_graphs"]
 KERNEL_NAMES = ["fill", "outer_fwd", "mid", "outer_inv", "garner_carry", "outer_first"]
 
 u32p = ctypes.POINTER(ctypes.c_uint32)

@@ -98,6 +98,12 @@ typedef struct bi_hip_dpair {
  * on the stream before it reuses its descriptor buffer. */
 int bi_hip_mul_batch_device(const bi_hip_dpair* pairs, size_t count, int plan_mode, void* stream);
 
+/* Grouped batches go to the GPU as one captured hipGraph per call (kernels only), updated in
+ * place when the next batch has the same launch shape: enable != 0 (the default) or plain stream
+ * launches (0).  No reference counterpart (its product-tree levels are loops of operator*,
+ * examples/pi.cpp:45-58). */
+int bi_hip_set_graphs(int enable);
+
 /* Batch of independent products (host pointers).  Each pair gets its own status; the call
  * returns the first nonzero status or BI_HIP_OK. */
 int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode);
