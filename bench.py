@@ -177,6 +177,28 @@ def profiled_traffic(kernel_class: str):
     return {"kernel": kern, "traffic": traffic, "valu_issue_frac": frac}
 
 
+def cpu_sweep() -> int:
+    """CPU restatement timings, one process per thread count (the pool's first call wins)."""
+    info = cpu_info()
+    phys = info.get("sockets", 1) * info.get("cores_per_socket", os.cpu_count() or 1)
+    runs = []
+    def run(args):
+        out = subprocess.run([sys.executable, "-m", "oracle.cpu_time"] + args, cwd=ROOT,
+                             capture_output=True, text=True, timeout=900)
+        if out.returncode:
+            raise RuntimeError(out.stderr)
+        r = json.loads(out.stdout.strip().splitlines()[-1])
+        r["args"] = " ".join(args)
+        runs.append(r)
+        print(json.dumps(r), file=sys.stderr, flush=True)
+    for t in sorted({1, 8, min(phys, 16)}):
+        run(["--threads", str(t), "--bits", "28", "--max-s", "60"])
+    run(["--threads", str(min(phys, 16)), "--batch", "4096", "--stride", "16"])
+    run(["--threads", str(min(phys, 16)), "--batch", "4096", "--stride", "16", "--across"])
+    print(json.dumps({"cpu_sweep": runs, "host": info}), flush=True)
+    return 0
+
+
 def copy_calibration(hip) -> int:
     """A plain 1 GiB device-to-device copy, three times: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
     this command gives the counters' reading for a known byte count (1 GiB read + 1 GiB written)."""
@@ -204,11 +226,17 @@ def main() -> int:
                     help="products in the batched product-tree leg (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (reference point for rocprofv3 byte counters)")
+    ap.add_argument("--cpu-sweep", action="store_true",
+                    help="run only the CPU restatement: 2^28 x 2^28 at 1, 8 and all physical cores, "
+                         "and a 1/16 sample of the batched workload, sequential and across threads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
         return 2
+
+    if args.cpu_sweep:
+        return cpu_sweep()
 
     from bigint_amd import engine, hip
     from bigint_amd import limbs as L
