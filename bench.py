@@ -177,6 +177,55 @@ def profiled_traffic(kernel_class: str):
     return {"kernel": kern, "traffic": traffic, "valu_issue_frac": frac}
 
 
+def size_sweep(hip, engine, L) -> int:
+    """Per size: device-resident time (operands and product in HBM, bi_hip_mul_device) and
+    end-to-end time through bi_hip_mul (host limbs in, host limbs out, copies included); median
+    of 5 after 2 warm-ups; every product checked against residues."""
+    engine.init(0)
+    from bigint_amd import planner
+    if os.path.exists(planner.DEFAULT_PATH):
+        planner.load()
+    rows = []
+    for name, ba, bb in [("2^22 x 2^22", 22, 22), ("2^22 squared", 22, None), ("2^24 x 2^24", 24, 24),
+                         ("2^26 x 2^26", 26, 26), ("2^28 x 2^28", 28, 28), ("2^28 x 2^20", 28, 20)]:
+        a = L.random_operand(1 << ba, L.SEED)
+        b = a if bb is None else L.random_operand(1 << bb, L.SEED + 1)
+        da = hip.Buffer.from_array(a)
+        db = da if bb is None else hip.Buffer.from_array(b)
+        dout = hip.Buffer(4 * (a.size + b.size))
+        st = hip.Stream()
+        e0, e1 = hip.Event(), hip.Event()
+        dev = []
+        for r in range(7):
+            e0.record(st)
+            engine.mul_device(da.ptr, a.size, db.ptr, b.size, bb is None, dout.ptr, a.size + b.size, st.handle)
+            e1.record(st)
+            e1.synchronize()
+            if r >= 2:
+                dev.append(e0.elapsed_ms(e1))
+        p = dout.download(np.uint32, a.size + b.size)
+        ok = all(L.residue_mod(p, q) == L.residue_mod(a, q) * L.residue_mod(b, q) % q
+                 for q in L.RESIDUE_PRIMES[:3])
+        e2e = []
+        for r in range(7):
+            t0 = time.perf_counter()
+            h = engine.mul(a, None if bb is None else b)
+            if r >= 2:
+                e2e.append((time.perf_counter() - t0) * 1e3)
+        ok = ok and np.array_equal(L.strip(p), h)
+        bits = (1 << ba) + (1 << (ba if bb is None else bb))
+        dm, em = float(np.median(dev)), float(np.median(e2e))
+        row = {"workload": name, "transform_log2": engine.transform_log2(a.size, b.size),
+               "device_ms": round(dm, 4), "device_Gbit_s": round(bits / dm / 1e6, 2),
+               "end_to_end_ms": round(em, 4), "end_to_end_Gbit_s": round(bits / em / 1e6, 2), "exact": bool(ok)}
+        rows.append(row)
+        print(json.dumps(row), file=sys.stderr, flush=True)
+        for buf in (da, db, dout):
+            buf.free()
+    print(json.dumps({"size_sweep": rows}), flush=True)
+    return 0 if all(r["exact"] for r in rows) else 1
+
+
 def cpu_sweep() -> int:
     """CPU restatement timings, one process per thread count (the pool's first call wins)."""
     info = cpu_info()
@@ -226,6 +275,9 @@ def main() -> int:
                     help="products in the batched product-tree leg (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (reference point for rocprofv3 byte counters)")
+    ap.add_argument("--sweep", action="store_true",
+                    help="run only the size sweep: 2^22 (and its square), 2^24, 2^26, 2^28 and "
+                         "2^28 x 2^20, device-resident and end-to-end through the host API")
     ap.add_argument("--cpu-sweep", action="store_true",
                     help="run only the CPU restatement: 2^28 x 2^28 at 1, 8 and all physical cores, "
                          "and a 1/16 sample of the batched workload, sequential and across threads")
@@ -243,6 +295,8 @@ def main() -> int:
 
     if args.copy_calibration:
         return copy_calibration(hip)
+    if args.sweep:
+        return size_sweep(hip, engine, L)
 
     bits = 1 << args.bits
     engine.init(0)
