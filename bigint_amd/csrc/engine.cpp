@@ -12,6 +12,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "../../include/bigint_hip.h"
@@ -96,6 +97,91 @@ struct ProfRec {
     int kind;
     hipEvent_t e0, e1;
     double bytes;
+};
+
+// Host <-> device copies of caller (pageable) memory through two pinned chunks: the host copy of
+// one chunk overlaps the DMA of the other, and large host copies are split over a few threads.
+class PinnedPipe {
+  public:
+    static constexpr size_t kChunk = size_t(8) << 20;  // bytes per pinned buffer
+
+    ~PinnedPipe() = default;  // pinned buffers live until the process ends
+
+    hipError_t h2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
+        hipError_t e = ensure();
+        for (size_t off = 0, k = 0; e == hipSuccess && off < bytes; off += kChunk, ++k) {
+            const size_t n = std::min(kChunk, bytes - off), b = next_;
+            next_ ^= 1;
+            // the buffer may still feed a DMA of this call or of an earlier one
+            if (pending_[b]) e = hipEventSynchronize(ev_[b]);
+            if (e != hipSuccess) break;
+            pending_[b] = false;
+            copy(buf_[b], (const char*)src + off, n);
+            e = hipMemcpyAsync((char*)dst + off, buf_[b], n, hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) e = hipEventRecord(ev_[b], st);
+            pending_[b] = e == hipSuccess;
+        }
+        return e;
+    }
+
+    // Waits for earlier work on st; returns when dst holds the data.
+    hipError_t d2h(void* dst, const void* src, size_t bytes, hipStream_t st) {
+        hipError_t e = ensure();
+        const size_t nchunks = (bytes + kChunk - 1) / kChunk;
+        auto issue = [&](size_t k) {
+            const size_t off = k * kChunk, n = std::min(kChunk, bytes - off), b = k & 1;
+            hipError_t r = hipMemcpyAsync(buf_[b], (const char*)src + off, n, hipMemcpyDeviceToHost, st);
+            return r == hipSuccess ? hipEventRecord(ev_[b], st) : r;
+        };
+        // DMAs into a buffer are ordered after any pending DMA out of it on the same stream
+        if (e == hipSuccess && nchunks) e = issue(0);
+        for (size_t k = 0; e == hipSuccess && k < nchunks; ++k) {
+            if (k + 1 < nchunks) e = issue(k + 1);  // the other buffer was emptied in step k-1
+            if (e == hipSuccess) e = hipEventSynchronize(ev_[k & 1]);
+            if (e != hipSuccess) break;
+            const size_t off = k * kChunk;
+            copy((char*)dst + off, buf_[k & 1], std::min(kChunk, bytes - off));
+        }
+        pending_[0] = pending_[1] = false;  // every event recorded above has completed
+        return e;
+    }
+
+  private:
+    void* buf_[2] = {nullptr, nullptr};
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+    bool pending_[2] = {false, false};  // ev_[i] marks a DMA out of buf_[i] not yet waited for
+    size_t next_ = 0;                   // buffer for the next host-to-device chunk
+
+    hipError_t ensure() {
+        for (int i = 0; i < 2; ++i) {
+            if (!buf_[i]) {
+                const hipError_t e = hipHostMalloc(&buf_[i], kChunk, hipHostMallocDefault);
+                if (e != hipSuccess) return e;
+            }
+            if (!ev_[i]) {
+                const hipError_t e = hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming);
+                if (e != hipSuccess) return e;
+            }
+        }
+        return hipSuccess;
+    }
+
+    static void copy(void* dst, const void* src, size_t n) {
+        constexpr size_t kSplit = size_t(2) << 20;
+        const size_t nt = std::min<size_t>(4, std::max<size_t>(1, n / kSplit));
+        if (nt == 1) {
+            std::memcpy(dst, src, n);
+            return;
+        }
+        const size_t per = (n / nt + 63) & ~size_t(63);
+        std::thread th[3];
+        for (size_t t = 1; t < nt; ++t) {
+            const size_t lo = std::min(n, t * per), hi = std::min(n, (t + 1) * per);
+            th[t - 1] = std::thread([=] { std::memcpy((char*)dst + lo, (const char*)src + lo, hi - lo); });
+        }
+        std::memcpy(dst, src, std::min(n, per));
+        for (size_t t = 1; t < nt; ++t) th[t - 1].join();
+    }
 };
 
 class Engine {
@@ -448,6 +534,8 @@ public:
     // descriptor buffer for batched launches (device), grow-only
     uint8_t* desc = nullptr;
     size_t desc_bytes = 0;
+    PinnedPipe pipe;  // host <-> device copies of the host-pointer entry points
+    std::mutex pipe_mtx;  // taken after mtx when both are held
     // device staging of bi_hip_mul_batch (host operands in, products out), grow-only
     std::mutex stage_mtx;
     uint32_t* stage = nullptr;
@@ -1046,16 +1134,12 @@ int bi_hip_mul(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, int i
     uint32_t* da = e.io;
     uint32_t* db = is_square ? da : da + la;
     uint32_t* dout = da + la + (is_square ? 0 : lb);
-    if (hipMemcpyAsync(da, a, la * 4, hipMemcpyHostToDevice, e.stream) != hipSuccess)
-        return BI_HIP_EDEVICE;
-    if (!is_square &&
-        hipMemcpyAsync(db, b, lb * 4, hipMemcpyHostToDevice, e.stream) != hipSuccess)
-        return BI_HIP_EDEVICE;
+    std::lock_guard<std::mutex> plk(e.pipe_mtx);
+    if (e.pipe.h2d(da, a, la * 4, e.stream) != hipSuccess) return BI_HIP_EDEVICE;
+    if (!is_square && e.pipe.h2d(db, b, lb * 4, e.stream) != hipSuccess) return BI_HIP_EDEVICE;
     rc = e.run(da, (uint32_t)la, db, (uint32_t)lb, is_square != 0, dout, e.stream);
     if (rc) return rc;
-    if (hipMemcpyAsync(out, dout, (la + lb) * 4, hipMemcpyDeviceToHost, e.stream) != hipSuccess)
-        return BI_HIP_EDEVICE;
-    if (hipStreamSynchronize(e.stream) != hipSuccess) return BI_HIP_EDEVICE;
+    if (e.pipe.d2h(out, dout, (la + lb) * 4, e.stream) != hipSuccess) return BI_HIP_EDEVICE;
     size_t n = la + lb;
     while (n > 1 && out[n - 1] == 0) --n;
     *out_len = n;
@@ -1154,13 +1238,18 @@ int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode) {
         oo += p.la + p.lb;
     }
     int rc = BI_HIP_OK;
-    if (hipMemcpy(dbuf, hin.data(), in_words * 4, hipMemcpyHostToDevice) != hipSuccess) rc = BI_HIP_EDEVICE;
+    {
+        std::lock_guard<std::mutex> plk(e.pipe_mtx);
+        if (e.pipe.h2d(dbuf, hin.data(), in_words * 4, e.stream) != hipSuccess ||
+            hipStreamSynchronize(e.stream) != hipSuccess)
+            rc = BI_HIP_EDEVICE;
+    }
     if (!rc) rc = bi_hip_mul_batch_device(dp.data(), count, plan_mode, nullptr);
-    if (!rc) rc = bi_hip_synchronize();
     std::vector<uint32_t> hout;
     if (!rc) {
         hout.resize(out_words);
-        if (hipMemcpy(hout.data(), dbuf + in_words, out_words * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        std::lock_guard<std::mutex> plk(e.pipe_mtx);
+        if (e.pipe.d2h(hout.data(), dbuf + in_words, out_words * 4, e.stream) != hipSuccess)
             rc = BI_HIP_EDEVICE;
     }
     size_t o = 0;

@@ -227,8 +227,8 @@ def test_trained_planner_gains_over_plain():
         pytest.skip("no tuned planner committed")
     planner.load()
     timer = planner.Timer(max_limbs=1 << 23)
-    shapes = [s for s in planner.sample_shapes(60, seed=424242, hi=24.0, max_batch_log=8)
-              if s[2] * (s[0] + s[1]) <= timer.out_words - 64]
+    shapes = [s for s in planner.sample_shapes(60, seed=424242, hi=27.0, max_batch_log=8)
+              if s[2] * (s[0] + s[1]) <= timer.out_words - 64 and max(s[0], s[1]) <= timer.max_limbs]
     choice = planner.choose(shapes, engine.PLAN_LEARNED)
     t_plain = t_learned = t_best = 0.0
     for (la, lb, k), c in zip(shapes, choice):
@@ -238,8 +238,10 @@ def test_trained_planner_gains_over_plain():
         t_learned += times[c]
         t_best += min(times.values())
     print(f"plain {t_plain:.0f} us, learned {t_learned:.0f} us, best {t_best:.0f} us")
-    assert t_plain / t_learned > 1.05
-    assert t_learned / t_best < 1.25
+    # the learned choices keep at least half of what the best plans gain over plan 0, and stay
+    # within 15 % of the best in total
+    assert t_plain - t_learned >= 0.5 * (t_plain - t_best)
+    assert t_learned / t_best < 1.15
 
 
 @pytest.mark.gpu
