@@ -11,6 +11,33 @@
 
 namespace bi {
 
+// Global-memory views of device pointers.  Pointers read from job descriptors are generic to the
+// compiler, which then emits flat loads and stores: those also count against the LDS wait
+// counter, so every LDS wait would wait for outstanding global traffic too.  Accesses through
+// these views compile to global_load / global_store.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+
+// Native vectors for 8- and 16-byte global accesses (HIP's uint2/uint4 classes do not copy across
+// address spaces).
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Copy of a job descriptor read through a global view, one 32-bit word at a time; the index is
+// uniform across the block, so the loads are scalar.
+template <class T>
+__device__ __forceinline__ T load_desc(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "descriptor size");
+    T v;
+    const auto* src = gmem(reinterpret_cast<const uint32_t*>(p));
+    auto* dst = reinterpret_cast<uint32_t*>(&v);
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(T) / 4; ++i) dst[i] = src[i];
+    return v;
+}
+
 constexpr int kNumPrimes = 3;
 constexpr uint32_t kP[kNumPrimes] = {469762049u, 2013265921u, 2281701377u};
 constexpr uint32_t kG[kNumPrimes] = {3u, 31u, 3u};

@@ -87,7 +87,7 @@ __device__ __forceinline__ void build_twiddles(uint32_t* __restrict__ tw, uint32
     constexpr uint32_t R = 1u << LR;
     const uint32_t tid = threadIdx.x;
     if (beta == 0) {
-        for (uint32_t k = tid; k < R; k += nthreads) tw[k] = wtab[k];
+        for (uint32_t k = tid; k < R; k += nthreads) tw[k] = gmem(wtab)[k];
         return;
     }
     for (uint32_t e = tid; e < (uint32_t)LR; e += nthreads) {
@@ -99,7 +99,7 @@ __device__ __forceinline__ void build_twiddles(uint32_t* __restrict__ tw, uint32
     for (uint32_t k = tid; k < R; k += nthreads) {
         if (k == 0) continue;
         const uint32_t log_m = 31u - __builtin_clz(k);
-        tw[k] = mont_mul<I>(wtab[k], bpow[LR - 1 - log_m]);
+        tw[k] = mont_mul<I>(gmem(wtab)[k], bpow[LR - 1 - log_m]);
     }
 }
 
@@ -186,8 +186,8 @@ __device__ __forceinline__ uint32_t limb_word(const uint32_t* __restrict__ limbs
     const uint32_t o = g * 32u - k * 28u;
     // loads at clamped (always valid) indices, then selects: no branch around the loads
     const uint32_t last = nlimbs - 1u;
-    uint32_t lo = limbs[k < last ? k : last];
-    uint32_t hi = limbs[k + 1u < last ? k + 1u : last];
+    uint32_t lo = gmem(limbs)[k < last ? k : last];
+    uint32_t hi = gmem(limbs)[k + 1u < last ? k + 1u : last];
     lo = k < nlimbs ? lo : 0u;
     hi = k + 1u < nlimbs ? hi : 0u;
     return (lo >> o) | (hi << (28u - o));
@@ -241,13 +241,13 @@ __device__ __forceinline__ void global_io(uint32_t* __restrict__ plane, const Li
 #pragma unroll
         for (int q = 0; q < Ly::EG; ++q) {
             const uint32_t idx = tile_base + c + (Ly::row(sg, q) << log_s);
-            if (STORE) plane[idx] = x[sg * Ly::EG + q];
+            if (STORE) gmem(plane)[idx] = x[sg * Ly::EG + q];
             else if (src.limbs) {
                 const bool in = idx < src.cnt;
                 const uint32_t w = limb_word(src.limbs, src.nlimbs, src.off + (in ? idx : 0u));
                 x[sg * Ly::EG + q] = in ? w : 0u;
             }
-            else x[sg * Ly::EG + q] = plane[idx];
+            else x[sg * Ly::EG + q] = gmem(plane)[idx];
         }
     }
 }
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(kThreads) void k_outer_first(PassArgs a) {
     __shared__ uint32_t lds[kLdsWords];
     __shared__ uint32_t tw[1 << LR];
     __shared__ uint32_t bpow[16];
-    const PassJob& job = a.jobs ? a.jobs[blockIdx.y] : a.job;
+    const PassJob job = a.jobs ? load_desc(a.jobs + blockIdx.y) : a.job;
     outer_first_body<LR>(a, job, blockIdx.z, lds, tw, bpow);
 }
 
@@ -367,7 +367,7 @@ __global__ __launch_bounds__(kThreads) void k_outer_pass(PassArgs a) {
     __shared__ uint32_t lds[kLdsWords];
     __shared__ uint32_t tw[1 << LR];
     __shared__ uint32_t bpow[16];
-    const PassJob& job = a.jobs ? a.jobs[blockIdx.y] : a.job;
+    const PassJob job = a.jobs ? load_desc(a.jobs + blockIdx.y) : a.job;
     const uint32_t nops = INV ? 1u : ((a.flags & (kSquare | kSingle)) ? 1u : 2u);
     const uint32_t prime = blockIdx.z / nops, op = blockIdx.z % nops;
     switch (prime) {
@@ -464,7 +464,7 @@ __global__ __launch_bounds__(1 << (LR - 4)) void k_mid_pass(PassArgs a) {
     __shared__ uint32_t lds[(1 << LR) + (1 << LR) / 16];
     __shared__ uint32_t tw[1 << LR];
     __shared__ uint32_t bpow[16];
-    const PassJob& job = a.jobs ? a.jobs[blockIdx.y] : a.job;
+    const PassJob job = a.jobs ? load_desc(a.jobs + blockIdx.y) : a.job;
     switch (blockIdx.z) {
         case 0: mid_body<0, LR>(a, job, lds, tw, bpow); break;
         case 1: mid_body<1, LR>(a, job, lds, tw, bpow); break;
@@ -540,9 +540,9 @@ __device__ __forceinline__ Pieces garner(const GarnerJob& jb, const GarnerConsts
             const uint64_t w = wrap_product(jb);
             return {(uint32_t)w, (uint32_t)(w >> 32), 0u};
         }
-        r0 = jb.y[j];
-        r1 = jb.y[n + j];
-        r2 = jb.y[2u * n + j];
+        r0 = gmem(jb.y)[j];
+        r1 = gmem(jb.y)[n + j];
+        r2 = gmem(jb.y)[2u * n + j];
         if (j == 0 && jb.wrap_a) {
             const unsigned __int128 x = crt_coeff(r0, r1, r2, jb.kscale, k) - wrap_product(jb);
             const uint64_t lo = (uint64_t)x;
@@ -558,9 +558,9 @@ __device__ __forceinline__ Pieces garner(const GarnerJob& jb, const GarnerConsts
             if (off >= n) break;
             if (cc < jb.nchunks) {
                 const uint32_t* y = jb.y + (size_t)cc * jb.chunk_stride;
-                r0 = add_mod<0>(r0, fold1p(y[off]));  // P0 outputs lie in [0, 2P)
-                r1 = add_mod<1>(r1, y[n + off]);
-                r2 = add_mod<2>(r2, y[2u * n + off]);
+                r0 = add_mod<0>(r0, fold1p(gmem(y)[off]));  // P0 outputs lie in [0, 2P)
+                r1 = add_mod<1>(r1, gmem(y)[n + off]);
+                r2 = add_mod<2>(r2, gmem(y)[2u * n + off]);
             }
         }
     }
@@ -579,10 +579,10 @@ static_assert(kCarryTile == kCarryTileLimbs && kWordTile == kCarryTileWords, "ti
 // polled with relaxed agent-scope atomics: no acquire/release fence, hence no L2 write-back or
 // invalidation per poll (the L2 is private to each XCD).
 __device__ __forceinline__ uint32_t status_load(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(gmem(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void status_store(uint32_t* p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(gmem(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 constexpr uint32_t kAgg = 1u, kInc = 2u;
 
@@ -598,12 +598,12 @@ __device__ __forceinline__ Pieces split_coeff(uint32_t r0, uint32_t r1, uint32_t
 // instantiations keep the chunk loop's register demand out of the common kernel.
 template <bool CHUNKED>
 __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
-    const GarnerJob& jb = a.jobs ? a.jobs[blockIdx.y] : a.job;
+    const GarnerJob jb = a.jobs ? load_desc(a.jobs + blockIdx.y) : a.job;
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave[kThreads / 64];
     __shared__ uint32_t s_carry_in;
     const uint32_t tid = threadIdx.x;
-    if (tid == 0) s_tile = atomicAdd(jb.counter, 1u);
+    if (tid == 0) s_tile = __hip_atomic_fetch_add(gmem(jb.counter), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint32_t ntiles = (jb.out_len + kCarryTile - 1u) / kCarryTile;
@@ -629,9 +629,9 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
 #pragma unroll
             for (uint32_t i = 0; i < NQ; i += 2) {
                 if (jl + i < n) {
-                    const uint2 v0 = *reinterpret_cast<const uint2*>(jb.y + jl + i);
-                    const uint2 v1 = *reinterpret_cast<const uint2*>(jb.y + n + jl + i);
-                    const uint2 v2 = *reinterpret_cast<const uint2*>(jb.y + 2u * n + jl + i);
+                    const u32x2 v0 = *gmem(reinterpret_cast<const u32x2*>(jb.y + jl + i));
+                    const u32x2 v1 = *gmem(reinterpret_cast<const u32x2*>(jb.y + n + jl + i));
+                    const u32x2 v2 = *gmem(reinterpret_cast<const u32x2*>(jb.y + 2u * n + jl + i));
                     r0[i] = v0.x; r0[i + 1] = v0.y;
                     r1[i] = v1.x; r1[i + 1] = v1.y;
                     r2[i] = v2.x; r2[i + 1] = v2.y;
@@ -658,9 +658,9 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
                 for (uint32_t i = 0; i < NQ; i += 2) {
                     const int32_t off = off0 + (int32_t)i;
                     if (off >= 0 && off < (int32_t)n) {
-                        const uint2 v0 = *reinterpret_cast<const uint2*>(y + off);
-                        const uint2 v1 = *reinterpret_cast<const uint2*>(y + n + off);
-                        const uint2 v2 = *reinterpret_cast<const uint2*>(y + 2u * n + off);
+                        const u32x2 v0 = *gmem(reinterpret_cast<const u32x2*>(y + off));
+                        const u32x2 v1 = *gmem(reinterpret_cast<const u32x2*>(y + n + off));
+                        const u32x2 v2 = *gmem(reinterpret_cast<const u32x2*>(y + 2u * n + off));
                         r0[i] = add_mod<0>(r0[i], fold1p(v0.x));  // P0 outputs lie in [0, 2P)
                         r0[i + 1] = add_mod<0>(r0[i + 1], fold1p(v0.y));
                         r1[i] = add_mod<1>(r1[i], v1.x);
@@ -822,14 +822,14 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     }
     const uint32_t i0 = tile * kCarryTile + tid * kCarryLimbsPerThread;
     if (i0 + kCarryLimbsPerThread <= jb.out_len && ((reinterpret_cast<uintptr_t>(jb.out + i0) & 15u) == 0)) {
-        uint4* dst = reinterpret_cast<uint4*>(jb.out + i0);
+        auto dst = gmem(reinterpret_cast<u32x4*>(jb.out + i0));
 #pragma unroll
         for (uint32_t q = 0; q < kCarryLimbsPerThread / 4; ++q)
-            dst[q] = make_uint4(outv[4 * q], outv[4 * q + 1], outv[4 * q + 2], outv[4 * q + 3]);
+            dst[q] = u32x4{outv[4 * q], outv[4 * q + 1], outv[4 * q + 2], outv[4 * q + 3]};
     } else {
 #pragma unroll
         for (uint32_t q = 0; q < kCarryLimbsPerThread; ++q)
-            if (i0 + q < jb.out_len) jb.out[i0 + q] = outv[q];
+            if (i0 + q < jb.out_len) gmem(jb.out)[i0 + q] = outv[q];
     }
 }
 
@@ -844,11 +844,11 @@ __device__ __forceinline__ void diag_body(const uint32_t* __restrict__ sa, const
     const uint32_t i_hi = s < ca - 1u ? s : ca - 1u;
     uint32_t acc = 0;
     for (uint32_t i = i_lo; i <= i_hi; ++i) {
-        const uint32_t x = sa[i * stride + off], y = sb[(s - i) * stride + off];
+        const uint32_t x = gmem(sa)[i * stride + off], y = gmem(sb)[(s - i) * stride + off];
         if constexpr (I == 0) acc = add_mod<0>(acc, fold1p(pointwise<0>(x, y)));
         else acc = add_mod<I>(acc, mont_mul<I>(x, y));
     }
-    d[s * stride + off] = acc;
+    gmem(d)[s * stride + off] = acc;
 }
 
 __global__ __launch_bounds__(256) void k_diag_accumulate(const uint32_t* sa, const uint32_t* sb, uint32_t ca,
