@@ -17,13 +17,14 @@
 
 #include "../../include/bigint_hip.h"
 #include "ntt_plan.h"
+#include "ntt_primes.h"
 #include "planner.h"
 
 namespace bi {
 namespace {
 
-constexpr uint32_t kPh[3] = {469762049u, 2013265921u, 2281701377u};
-constexpr uint32_t kGh[3] = {3u, 31u, 3u};
+constexpr uint32_t kPh[3] = {kP[0], kP[1], kP[2]};  // the device primes and generators
+constexpr uint32_t kGh[3] = {kG[0], kG[1], kG[2]};
 constexpr int kTileH = 4096;
 constexpr uint32_t kMaxLog = 26;
 

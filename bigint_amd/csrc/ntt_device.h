@@ -1,13 +1,15 @@
 // Device-side modular arithmetic for the three NTT primes (gfx950 / CDNA4).
 //
-// Primes, generators and the 28-bit limb width are the reference's
-// (include/bigint/ntt_base.h:14-20); Montgomery reduction uses R = 2^32 as in
-// include/bigint/ntt_base.h:40-71.  The prime index is a template parameter so every modulus and
-// its -P^-1 mod 2^32 are immediates in the generated code.
+// The 28-bit limb width is the reference's (include/bigint/ntt_base.h:14-20), the primes are in
+// ntt_primes.h; Montgomery reduction uses R = 2^32 as in include/bigint/ntt_base.h:40-71.  The
+// prime index is a template parameter so every modulus and its -P^-1 mod 2^32 are immediates in
+// the generated code.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "ntt_primes.h"
 
 namespace bi {
 
@@ -38,14 +40,11 @@ __device__ __forceinline__ T load_desc(const T* p) {
     return v;
 }
 
-constexpr int kNumPrimes = 3;
-constexpr uint32_t kP[kNumPrimes] = {469762049u, 2013265921u, 2281701377u};
-constexpr uint32_t kG[kNumPrimes] = {3u, 31u, 3u};
 // -P^-1 mod 2^32.  Each P = k*2^m + 1 with 2m >= 32, so P*(P-2) = (P-1)^2 - 1 == -1 mod 2^32.
 constexpr uint32_t kNegInvP[kNumPrimes] = {kP[0] - 2u, kP[1] - 2u, kP[2] - 2u};
 constexpr uint32_t kDigitBits = 28;
 constexpr uint32_t kDigitMask = (1u << kDigitBits) - 1u;
-constexpr int kMaxLogN = 26;  // smallest 2-adic valuation of P-1 (ntt_base.h:22-27)
+constexpr int kMaxLogN = 26;  // smallest 2-adic valuation of P-1 (P0 and P2: 26)
 
 static_assert(kP[0] * (kP[0] - 2u) == 0xFFFFFFFFu, "P0 Montgomery constant");
 static_assert(kP[1] * (kP[1] - 2u) == 0xFFFFFFFFu, "P1 Montgomery constant");
@@ -87,14 +86,9 @@ __device__ __forceinline__ uint32_t fold2p(uint32_t x) {
 template <int I>
 __device__ __forceinline__ uint32_t add_mod(uint32_t a, uint32_t b) {
     constexpr uint32_t p = kP[I];
-    if constexpr (p < 0x80000000u) {
-        const uint32_t s = a + b;  // < 2p < 2^32
-        const uint32_t s2 = s - p;
-        return s2 < s ? s2 : s;
-    } else {
-        const uint32_t d = a - (p - b);  // a + b - p, wraps when a + b < p
-        return a >= p - b ? d : d + p;
-    }
+    const uint32_t s = a + b;  // < 2p < 2^32
+    const uint32_t s2 = s - p;
+    return s2 < s ? s2 : s;
 }
 
 // (a - b) mod P for a, b < P.
@@ -102,12 +96,8 @@ template <int I>
 __device__ __forceinline__ uint32_t sub_mod(uint32_t a, uint32_t b) {
     constexpr uint32_t p = kP[I];
     const uint32_t d = a - b;
-    if constexpr (p < 0x80000000u) {
-        const uint32_t d2 = d + p;  // < 2p < 2^32 when no borrow
-        return d2 < d ? d2 : d;
-    } else {
-        return a >= b ? d : d + p;
-    }
+    const uint32_t d2 = d + p;  // < 2p < 2^32 when no borrow
+    return d2 < d ? d2 : d;
 }
 
 }  // namespace bi

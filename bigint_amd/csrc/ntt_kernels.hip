@@ -139,23 +139,21 @@ __device__ __forceinline__ void group_stages(uint32_t (&x)[kE], const uint32_t* 
                     }
                 } else if (!INV) {
                     const uint32_t vw = mont_mul<I>(v, w);
-                    // P1 < 2^31: outputs that are the multiplied operand ("v") of the next stage
-                    // in this register group stay in [0, 2P) (Montgomery accepts any 32-bit
+                    // P1, P2 < 2^31: outputs that are the multiplied operand ("v") of the next
+                    // stage in this register group stay in [0, 2P) (Montgomery accepts any 32-bit
                     // input); the rest, and every output of a group's last stage, are reduced.
-                    constexpr bool kLazy1 = I == 1 && (int)hi + 1 < ST;
-                    if (kLazy1 && (q & (h >> 1)) != 0) {
+                    constexpr bool kLazy = (int)hi + 1 < ST;
+                    if (kLazy && (q & (h >> 1)) != 0) {
                         x[sg * EG + q] = u + vw;
-                        x[sg * EG + q + h] = u + kP[1] - vw;
+                        x[sg * EG + q + h] = u + kP[I] - vw;
                     } else {
                         x[sg * EG + q] = add_mod<I>(u, vw);
                         x[sg * EG + q + h] = sub_mod<I>(u, vw);
                     }
                 } else {
                     x[sg * EG + q] = add_mod<I>(u, v);
-                    if constexpr (I == 1)  // u - v + P < 2P < 2^32 goes to Montgomery unreduced
-                        x[sg * EG + q + h] = mont_mul<I>(u + kP[1] - v, w);
-                    else
-                        x[sg * EG + q + h] = mont_mul<I>(sub_mod<I>(u, v), w);
+                    // u - v + P < 2P < 2^32 goes to Montgomery unreduced
+                    x[sg * EG + q + h] = mont_mul<I>(u + kP[I] - v, w);
                 }
             }
         }
@@ -194,7 +192,7 @@ __device__ __forceinline__ uint32_t limb_word(const uint32_t* __restrict__ limbs
 }
 
 // A 32-bit input word as a residue: below P for P1 and P2, below 4P for P0 (the range of the
-// lazy forward butterflies).  2^32 < 9.2 P0 < 2.2 P1 < 1.9 P2.
+// lazy forward butterflies).  2^32 < 9.2 P0 < 2.2 P1 < 2.4 P2.
 template <int I>
 __device__ __forceinline__ uint32_t word_residue(uint32_t x) {
     if constexpr (I == 0) {
@@ -203,14 +201,11 @@ __device__ __forceinline__ uint32_t word_residue(uint32_t x) {
         x = y < x ? y : x;  // < 5.2 P0
         y = x - p4;
         return y < x ? y : x;  // < 4 P0
-    } else if constexpr (I == 1) {
-        constexpr uint32_t p2 = 2u * kP[1];
-        uint32_t y = x - p2;
-        x = y < x ? y : x;  // < 2 P1
-        y = x - kP[1];
-        return y < x ? y : x;
     } else {
-        const uint32_t y = x - kP[2];
+        constexpr uint32_t p2 = 2u * kP[I];
+        uint32_t y = x - p2;
+        x = y < x ? y : x;  // < 2 P
+        y = x - kP[I];
         return y < x ? y : x;
     }
 }
@@ -515,7 +510,9 @@ __device__ __forceinline__ unsigned __int128 crt_coeff(uint32_t r0, uint32_t r1,
     const uint32_t v0 = c0;
     const uint32_t v1 = mont_mul<1>(sub_mod<1>(c1, v0), k.inv01);
     const uint32_t t = mont_mul<2>(sub_mod<2>(c2, v0), k.inv02);
-    const uint32_t v2 = mont_mul<2>(sub_mod<2>(t, v1), k.inv12);
+    static_assert(kP[1] < 2u * kP[2] && kP[0] < kP[1] && kP[0] < kP[2], "CRT folds");
+    const uint32_t v1r = v1 >= kP[2] ? v1 - kP[2] : v1;  // v1 < P1 < 2 P2, as a residue mod P2
+    const uint32_t v2 = mont_mul<2>(sub_mod<2>(t, v1r), k.inv12);
     const uint64_t inner = (uint64_t)v1 + (uint64_t)kP[1] * v2;  // < 2^62.1
     return (unsigned __int128)inner * kP[0] + v0;
 }

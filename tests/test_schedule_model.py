@@ -13,8 +13,10 @@ import pytest
 
 from bigint_amd import limbs as L
 
-P = [469762049, 2013265921, 2281701377]
-G = [3, 31, 3]
+# The device primes: the reference's first two, and 1811939329 = 27 * 2^26 + 1 in place of its
+# 2281701377 so that every prime is below 2^31.
+P = [469762049, 2013265921, 1811939329]
+G = [3, 31, 13]
 R32 = 1 << 32
 KE, KTILE = 16, 4096
 KLDS = KTILE + KTILE // 16
@@ -161,12 +163,9 @@ def word_residue(I, x):
     if I == 0:
         x = csub(csub(x, 4 * P[0]), 4 * P[0])
         assert x < 4 * P[0]
-    elif I == 1:
-        x = csub(csub(x, 2 * P[1]), P[1])
-        assert x < P[1]
     else:
-        x = csub(x, P[2])
-        assert x < P[2]
+        x = csub(csub(x, 2 * P[I]), P[I])
+        assert x < P[I]
     return x % P[I]
 
 
@@ -337,12 +336,7 @@ def model_product(a, b, log_n):
     out_len = len(a) + len(b)
     xs = []
     for j in range(n):
-        c = [mont(I, y[I][j], pow(n, -1, P[I]) * R32 * R32 % P[I]) for I in range(3)]
-        v0 = c[0]
-        v1 = (c[1] - v0) * pow(P[0], -1, P[1]) % P[1]
-        t = (c[2] - v0) * pow(P[0], -1, P[2]) % P[2]
-        v2 = (t - v1) * pow(P[1], -1, P[2]) % P[2]
-        x = v0 + P[0] * (v1 + P[1] * v2)
+        x = kernel_crt(y[0][j], y[1][j], y[2][j], n)
         assert x < 1 << 89
         xs.append(x)
     wa, wb = words_of(len(a)), words_of(len(b))
@@ -509,3 +503,155 @@ def test_garner_carry_run_across_tiles():
     la = (32 * W + 27) // 28
     out = garner_carry(lambda j: conv[j] if 0 <= j < len(conv) else 0, la + 2)
     assert sum(x << (28 * i) for i, x in enumerate(out)) == a * ((1 << 32) + 1)
+
+
+M32 = (1 << 32) - 1
+
+
+def k_mont(I, a, b, lazy=False):
+    """Kernel mont_mul (lazy: mont_mul_lazy) on 32-bit words, asserting the ranges it relies on:
+    the 64-bit sum t + u*P must not overflow, and the shifted value must lie below 2P."""
+    assert 0 <= a <= M32 and 0 <= b < P[I]
+    t = a * b
+    u = (t * (P[I] - 2)) & M32  # -P^-1 mod 2^32
+    s = t + u * P[I]
+    assert s < (1 << 64) and s % R32 == 0
+    r = s >> 32
+    assert r < 2 * P[I]
+    return r if lazy or r < P[I] else r - P[I]
+
+
+def k_add_mod(I, a, b):
+    assert 0 <= a < P[I] and 0 <= b < P[I]
+    s = a + b
+    assert s <= M32
+    return s - P[I] if s >= P[I] else s
+
+
+def k_sub_mod(I, a, b):
+    assert 0 <= a < P[I] and 0 <= b < P[I]
+    return a - b if a >= b else a - b + P[I]
+
+
+def k_fold2p(x):
+    assert 0 <= x < 4 * P[0]
+    return x - 2 * P[0] if x >= 2 * P[0] else x
+
+
+def lazy_group(I, inv, ST, x, rng):
+    """Kernel group_stages on one register group of 2^ST values with its lazy reductions
+    (P0: Harvey butterflies, forward values in [0, 4P), inverse in [0, 2P); P1 and P2: forward
+    outputs that feed the next stage's Montgomery operand stay in [0, 2P), the inverse difference
+    goes to Montgomery as u - v + P).  Every intermediate must fit 32 bits; the values are checked
+    against the exact butterflies mod P."""
+    p = P[I]
+    rinv = pow(R32, -1, p)
+    exact = [v % p for v in x]
+    for hi in range(ST):
+        lh = hi if inv else ST - 1 - hi
+        h = 1 << lh
+        for q in range(1 << ST):
+            if q & h:
+                continue
+            w = rng.choice([p - 1, p - 2, 1, rng.randrange(p)])
+            u, v = x[q], x[q + h]
+            eu, ev = exact[q], exact[q + h]
+            if inv:
+                exact[q], exact[q + h] = (eu + ev) % p, (eu - ev) * w * rinv % p
+            else:
+                exact[q], exact[q + h] = (eu + ev * w * rinv) % p, (eu - ev * w * rinv) % p
+            if I == 0:
+                if not inv:
+                    assert u < 4 * p and v < 4 * p
+                    u2 = k_fold2p(u)
+                    vw = k_mont(0, v, w, lazy=True)
+                    x[q], x[q + h] = u2 + vw, u2 + 2 * p - vw
+                    assert x[q] < 4 * p and 0 < x[q + h] < 4 * p
+                else:
+                    assert u < 2 * p and v < 2 * p
+                    x[q] = k_fold2p(u + v)
+                    x[q + h] = k_mont(0, u + 2 * p - v, w, lazy=True)
+            elif not inv:
+                vw = k_mont(I, v, w)
+                if hi + 1 < ST and (q & (h >> 1)):
+                    assert u < p
+                    x[q], x[q + h] = u + vw, u + p - vw
+                    assert x[q] < 2 * p <= M32 and x[q + h] < 2 * p
+                else:
+                    x[q], x[q + h] = k_add_mod(I, u, vw), k_sub_mod(I, u, vw)
+            else:
+                x[q] = k_add_mod(I, u, v)
+                x[q + h] = k_mont(I, u + p - v, w)
+            assert x[q] % p == exact[q] and x[q + h] % p == exact[q + h]
+    return x
+
+
+@pytest.mark.parametrize("I", [0, 1, 2])
+@pytest.mark.parametrize("inv", [False, True])
+@pytest.mark.parametrize("ST", [1, 2, 3, 4])
+def test_lazy_butterfly_ranges(I, inv, ST):
+    rng = random.Random(1000 * I + 10 * ST + inv)
+    p = P[I]
+    top = (4 * p if not inv else 2 * p) if I == 0 else p  # input range of a group
+    for _ in range(150):
+        x = [rng.choice([top - 1, top - 2, 0, rng.randrange(top)]) for _ in range(1 << ST)]
+        out = lazy_group(I, inv, ST, x, rng)
+        assert all(v < top for v in out)
+
+
+@pytest.mark.parametrize("I", [0, 1, 2])
+def test_word_residue_and_pointwise_ranges(I):
+    """Input words reduce into a forward group's input range, and the pointwise product of two
+    forward outputs lands in an inverse group's input range."""
+    p = P[I]
+    for x in [0, 1, p - 1, p, 2 * p - 1, 2 * p, 4 * p - 1 if I == 0 else M32, M32]:
+        r = word_residue(I, x)
+        assert r == x % p
+    fwd_top = 4 * p if I == 0 else p
+    for a, b in [(fwd_top - 1, fwd_top - 1), (0, fwd_top - 1), (fwd_top // 3, fwd_top - 7)]:
+        if I == 0:
+            fa, fb = k_fold2p(a), k_fold2p(b)
+            t = fa * fb  # mont_mul_lazy(fold2p(a), fold2p(b)): both below 2P
+            u = (t * (p - 2)) & M32
+            s = t + u * p
+            assert s < (1 << 64)
+            r = s >> 32
+            assert r < 2 * p and r % p == a * b * pow(R32, -1, p) % p
+        else:
+            r = k_mont(I, a, b)
+            assert r < p
+
+
+def kernel_crt(r0, r1, r2, n):
+    """Kernel crt_coeff: the residues carry the inverse transform's factor n * R^-1, removed by
+    the Garner scale constants n^-1 * R^2; every modular step is checked for its input ranges."""
+    ks = [pow(n, -1, P[I]) * R32 * R32 % P[I] for I in range(3)]
+    inv01 = to_mont(1, pow(P[0], -1, P[1]))
+    inv02 = to_mont(2, pow(P[0], -1, P[2]))
+    inv12 = to_mont(2, pow(P[1], -1, P[2]))
+    assert r0 < 2 * P[0] and r1 < P[1] and r2 < P[2]
+    v0 = k_mont(0, r0, ks[0])
+    v1 = k_mont(1, k_sub_mod(1, k_mont(1, r1, ks[1]), v0), inv01)
+    t = k_mont(2, k_sub_mod(2, k_mont(2, r2, ks[2]), v0), inv02)
+    v1r = v1 - P[2] if v1 >= P[2] else v1
+    v2 = k_mont(2, k_sub_mod(2, t, v1r), inv12)
+    x = v0 + P[0] * (v1 + P[1] * v2)
+    assert x < P[0] * P[1] * P[2]
+    return x
+
+
+def test_crt_recovers_every_coefficient():
+    """Coefficients up to the largest possible (2^26 terms below 2^64: under 2^90) come back from
+    their three residues, including P0 residues given in [P0, 2 P0) as the lazy inverse leaves
+    them."""
+    rng = random.Random(7)
+    assert P[0] * P[1] * P[2] > (1 << 26) * ((1 << 32) - 1) ** 2
+    xs = [0, 1, P[0], P[0] * P[1], P[2] * P[1] - 1, (1 << 26) * ((1 << 32) - 1) ** 2]
+    xs += [rng.randrange(1 << 90) for _ in range(3000)] + [rng.randrange(1 << 64) for _ in range(500)]
+    for n in (2, 1 << 11, 1 << 24):
+        rinv = [pow(R32, -1, P[I]) for I in range(3)]
+        for x in xs:
+            r = [x * n * rinv[I] % P[I] for I in range(3)]
+            assert kernel_crt(*r, n) == x
+            if r[0] < P[0]:
+                assert kernel_crt(r[0] + P[0], r[1], r[2], n) == x
