@@ -68,21 +68,32 @@ struct Plan {
     uint32_t kscale[3] = {0, 0, 0};
 };
 
+// Per-block roots of a pass whose tiles have 2^log_r rows: for block b,
+// beta_b = w_{n/S}^{bitrev_{log nblk}(b)}, stored as beta_b^(2^e) for e < log_r (Montgomery form,
+// kBetaPowStride words per block), so a tile's twiddles tw[k] = W[k] * beta^tr need one product
+// each and no squaring chain on the device.
 hipError_t upload_beta(uint32_t log_n, uint32_t log_r, uint32_t log_s, uint32_t nblk,
                        uint32_t** fwd, uint32_t** inv) {
     *fwd = *inv = nullptr;
     if (nblk <= 1) return hipSuccess;
-    std::vector<uint32_t> hf(3 * (size_t)nblk), hi(3 * (size_t)nblk);
+    if (log_r > kBetaPowStride) return hipErrorInvalidValue;
+    const size_t stride = kBetaPowStride;
+    std::vector<uint32_t> hf(3 * (size_t)nblk * stride, 0u), hi(3 * (size_t)nblk * stride, 0u);
     const uint32_t lb = 31u - (uint32_t)__builtin_clz(nblk);
-    (void)log_r;
     for (int I = 0; I < 3; ++I) {
         const uint32_t p = kPh[I];
-        // beta_blk = w_{n/S}^{bitrev_{log nblk}(blk)}
         const uint32_t wf = root(I, log_n - log_s, false), wi = root(I, log_n - log_s, true);
         for (uint32_t b = 0; b < nblk; ++b) {
             const uint32_t e = bitrev(b, lb);
-            hf[I * (size_t)nblk + b] = to_mont(powmod(wf, e, p), p);
-            hi[I * (size_t)nblk + b] = to_mont(powmod(wi, e, p), p);
+            uint32_t bf = powmod(wf, e, p), bi = powmod(wi, e, p);
+            uint32_t* of = &hf[((size_t)I * nblk + b) * stride];
+            uint32_t* oi = &hi[((size_t)I * nblk + b) * stride];
+            for (uint32_t k = 0; k < log_r; ++k) {
+                of[k] = to_mont(bf, p);
+                oi[k] = to_mont(bi, p);
+                bf = mulmod(bf, bf, p);
+                bi = mulmod(bi, bi, p);
+            }
         }
     }
     hipError_t e = hipMalloc((void**)fwd, hf.size() * 4);

@@ -79,28 +79,33 @@ struct Layout {
 };
 
 // Twiddle table of one tile: tw[k], k in [1, 2^LR), for the stage with half-span
-// tr = 2^LR / (2 * 2^floor(log2 k)) rows:  tw[k] = W[k] * beta^tr  (beta == 0 encodes beta = 1).
+// tr = 2^LR / (2 * 2^floor(log2 k)) rows:  tw[k] = W[k] * beta^tr.  bpow holds the block's
+// beta^(2^e), e < LR (precomputed per block, kBetaPowStride words); null encodes beta = 1.
 template <int I, int LR>
-__device__ __forceinline__ void build_twiddles(uint32_t* __restrict__ tw, uint32_t* __restrict__ bpow,
-                                               const uint32_t* __restrict__ wtab, uint32_t beta,
+__device__ __forceinline__ void build_twiddles(uint32_t* __restrict__ tw,
+                                               const uint32_t* __restrict__ wtab,
+                                               const uint32_t* __restrict__ bpow,
                                                uint32_t nthreads) {
+    static_assert(LR <= (int)kBetaPowStride, "beta power table too short");
     constexpr uint32_t R = 1u << LR;
     const uint32_t tid = threadIdx.x;
-    if (beta == 0) {
+    if (!bpow) {
         for (uint32_t k = tid; k < R; k += nthreads) tw[k] = gmem(wtab)[k];
         return;
     }
-    for (uint32_t e = tid; e < (uint32_t)LR; e += nthreads) {
-        uint32_t b = beta;
-        for (uint32_t j = 0; j < e; ++j) b = mont_mul<I>(b, b);
-        bpow[e] = b;  // beta^(2^e)
-    }
-    __syncthreads();
     for (uint32_t k = tid; k < R; k += nthreads) {
         if (k == 0) continue;
         const uint32_t log_m = 31u - __builtin_clz(k);
-        tw[k] = mont_mul<I>(gmem(wtab)[k], bpow[LR - 1 - log_m]);
+        tw[k] = mont_mul<I>(gmem(wtab)[k], gmem(bpow)[LR - 1 - log_m]);
     }
+}
+
+// The block's beta powers in a pass's table ([prime][block][kBetaPowStride]), or null for
+// block 0 (beta = 1) and for passes without per-block roots.
+template <int I>
+__device__ __forceinline__ const uint32_t* block_pows(const uint32_t* table, uint32_t stride,
+                                                      uint32_t blk) {
+    return (table && blk != 0) ? table + ((size_t)I * stride + blk) * kBetaPowStride : nullptr;
 }
 
 // Forward (Cooley-Tukey) or inverse (Gentleman-Sande) stages of one register group.
@@ -271,7 +276,7 @@ __device__ __forceinline__ void tile_transform(uint32_t (&x)[kE], uint32_t* lds,
 // grid.x = tiles, grid.y = job, grid.z = prime * nops + operand.
 template <int I, bool INV, int LR>
 __device__ __forceinline__ void outer_body(const PassArgs& a, const PassJob& job, uint32_t op,
-                                           uint32_t* lds, uint32_t* tw, uint32_t* bpow) {
+                                           uint32_t* lds, uint32_t* tw) {
     constexpr int LC = 12 - LR;
     using S = Sched<LR>;
     const uint32_t n = 1u << a.log_n;
@@ -284,8 +289,7 @@ __device__ __forceinline__ void outer_body(const PassArgs& a, const PassJob& job
     const uint32_t tile_base = (blk << (LR + a.log_s)) + (cg << LC);
 
     const uint32_t* wt = a.wtab + ((INV ? kNumPrimes : 0) + I) * kTile;
-    const uint32_t beta = (a.beta && blk != 0) ? a.beta[I * a.beta_stride + blk] : 0u;
-    build_twiddles<I, LR>(tw, bpow, wt, beta, kThreads);
+    build_twiddles<I, LR>(tw, wt, block_pows<I>(a.beta, a.beta_stride, blk), kThreads);
 
     uint32_t* plane = (op == 0 ? job.res_a : job.res_b) + (size_t)I * n;
     LimbSrc src{nullptr, 0u, 0u, 0u};
@@ -314,7 +318,7 @@ __device__ __forceinline__ void outer_body(const PassArgs& a, const PassJob& job
 // 32-bit words are cut from the limbs once and reduced per prime.  grid.z = operand.
 template <int LR>
 __device__ __forceinline__ void outer_first_body(const PassArgs& a, const PassJob& job, uint32_t op,
-                                                 uint32_t* lds, uint32_t* tw, uint32_t* bpow) {
+                                                 uint32_t* lds, uint32_t* tw) {
     constexpr int LC = 12 - LR;
     using S = Sched<LR>;
     const uint32_t n = 1u << a.log_n;
@@ -331,16 +335,22 @@ __device__ __forceinline__ void outer_first_body(const PassArgs& a, const PassJo
     constexpr int G0 = 0, GL = S::NG - 1;
     uint32_t words[kE];
     global_io<LR, LC, S::stages(G0), S::trlo(G0), false>(nullptr, src, words, tile_base, a.log_s);
+    // The three primes' twiddle tables (3 x 2^LR words) are built at once, overlapping the limb
+    // loads, so no prime waits for its table behind a barrier.
     static_for<kNumPrimes>([&](auto pi) {
         constexpr int I = (int)pi;
-        if constexpr (I > 0) __syncthreads();  // the previous prime is done with tw and lds
-        const uint32_t beta = (a.beta && blk != 0) ? a.beta[I * a.beta_stride + blk] : 0u;
-        build_twiddles<I, LR>(tw, bpow, a.wtab + I * kTile, beta, kThreads);
+        build_twiddles<I, LR>(tw + (I << LR), a.wtab + I * kTile,
+                              block_pows<I>(a.beta, a.beta_stride, blk), kThreads);
+    });
+    __syncthreads();
+    static_for<kNumPrimes>([&](auto pi) {
+        constexpr int I = (int)pi;
+        // lds is reused across primes: the barrier in front of each exchange write orders it
+        // after the previous prime's reads
         uint32_t x[kE];
 #pragma unroll
         for (int k = 0; k < kE; ++k) x[k] = word_residue<I>(words[k]);
-        __syncthreads();
-        tile_transform<I, false, LR, LC>(x, lds, tw);
+        tile_transform<I, false, LR, LC>(x, lds, tw + (I << LR));
         uint32_t* plane = (op == 0 ? job.res_a : job.res_b) + (size_t)I * n;
         global_io<LR, LC, S::stages(GL), S::trlo(GL), true>(plane, kPlane, x, tile_base, a.log_s);
     });
@@ -351,24 +361,22 @@ __device__ __forceinline__ void outer_first_body(const PassArgs& a, const PassJo
 template <int LR>
 __global__ __launch_bounds__(kThreads) void k_outer_first(PassArgs a) {
     __shared__ uint32_t lds[kLdsWords];
-    __shared__ uint32_t tw[1 << LR];
-    __shared__ uint32_t bpow[16];
+    __shared__ uint32_t tw[kNumPrimes << LR];
     const PassJob job = a.jobs ? load_desc(a.jobs + blockIdx.y) : a.job;
-    outer_first_body<LR>(a, job, blockIdx.z, lds, tw, bpow);
+    outer_first_body<LR>(a, job, blockIdx.z, lds, tw);
 }
 
 template <bool INV, int LR>
 __global__ __launch_bounds__(kThreads) void k_outer_pass(PassArgs a) {
     __shared__ uint32_t lds[kLdsWords];
     __shared__ uint32_t tw[1 << LR];
-    __shared__ uint32_t bpow[16];
     const PassJob job = a.jobs ? load_desc(a.jobs + blockIdx.y) : a.job;
     const uint32_t nops = INV ? 1u : ((a.flags & (kSquare | kSingle)) ? 1u : 2u);
     const uint32_t prime = blockIdx.z / nops, op = blockIdx.z % nops;
     switch (prime) {
-        case 0: outer_body<0, INV, LR>(a, job, op, lds, tw, bpow); break;
-        case 1: outer_body<1, INV, LR>(a, job, op, lds, tw, bpow); break;
-        default: outer_body<2, INV, LR>(a, job, op, lds, tw, bpow); break;
+        case 0: outer_body<0, INV, LR>(a, job, op, lds, tw); break;
+        case 1: outer_body<1, INV, LR>(a, job, op, lds, tw); break;
+        default: outer_body<2, INV, LR>(a, job, op, lds, tw); break;
     }
 }
 
@@ -386,7 +394,7 @@ __device__ __forceinline__ uint32_t pointwise(uint32_t a, uint32_t b) {
 
 template <int I, int LR>
 __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, uint32_t* lds,
-                                         uint32_t* tw, uint32_t* bpow) {
+                                         uint32_t* tw) {
     using S = Sched<LR>;
     constexpr uint32_t T = 1u << (LR - 4);
     const uint32_t n = 1u << a.log_n;
@@ -394,13 +402,10 @@ __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, 
     const uint32_t tile_base = blk << LR;
     const uint32_t* wf = a.wtab + I * kTile;
     const uint32_t* wi = a.wtab + (kNumPrimes + I) * kTile;
-    uint32_t bf = 0, bi_ = 0;
-    if (a.beta && blk != 0) {
-        bf = a.beta[I * a.beta_stride + blk];
-        bi_ = a.beta_inv[I * a.beta_stride + blk];
-    }
+    const uint32_t* bf = block_pows<I>(a.beta, a.beta_stride, blk);
+    const uint32_t* bi_ = block_pows<I>(a.beta_inv, a.beta_stride, blk);
     if (a.flags & kInvOnly) {  // a product spectrum from the diagonal sums: inverse stages only
-        build_twiddles<I, LR>(tw, bpow, wi, bi_, T);
+        build_twiddles<I, LR>(tw, wi, bi_, T);
         using Sx = Sched<LR>;
         constexpr LimbSrc kPl{nullptr, 0u, 0u, 0u};
         uint32_t* p = job.res_a + (size_t)I * n;
@@ -411,7 +416,7 @@ __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, 
         global_io<LR, 0, Sx::stages(0), Sx::trlo(0), true>(p, kPl, x, tile_base, 0);
         return;
     }
-    build_twiddles<I, LR>(tw, bpow, wf, bf, T);
+    build_twiddles<I, LR>(tw, wf, bf, T);
 
     const bool from_limbs = (a.flags & kFromLimbs) != 0;
     const bool square = (a.flags & kSquare) != 0;
@@ -448,7 +453,7 @@ __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, 
     }
     // the same LDS table now receives the inverse twiddles
     __syncthreads();
-    build_twiddles<I, LR>(tw, bpow, wi, bi_, T);
+    build_twiddles<I, LR>(tw, wi, bi_, T);
     __syncthreads();
     tile_transform<I, true, LR, 0>(xa, lds, tw);
     global_io<LR, 0, S::stages(G0), S::trlo(G0), true>(pa, kPlane, xa, tile_base, 0);
@@ -458,12 +463,11 @@ template <int LR>
 __global__ __launch_bounds__(1 << (LR - 4)) void k_mid_pass(PassArgs a) {
     __shared__ uint32_t lds[(1 << LR) + (1 << LR) / 16];
     __shared__ uint32_t tw[1 << LR];
-    __shared__ uint32_t bpow[16];
     const PassJob job = a.jobs ? load_desc(a.jobs + blockIdx.y) : a.job;
     switch (blockIdx.z) {
-        case 0: mid_body<0, LR>(a, job, lds, tw, bpow); break;
-        case 1: mid_body<1, LR>(a, job, lds, tw, bpow); break;
-        default: mid_body<2, LR>(a, job, lds, tw, bpow); break;
+        case 0: mid_body<0, LR>(a, job, lds, tw); break;
+        case 1: mid_body<1, LR>(a, job, lds, tw); break;
+        default: mid_body<2, LR>(a, job, lds, tw); break;
     }
 }
 

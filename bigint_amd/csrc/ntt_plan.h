@@ -30,6 +30,8 @@ struct PassJob {
     uint32_t a_cnt;       // ~0u: all of A
 };
 
+constexpr uint32_t kBetaPowStride = 16;  // words per block in the beta power tables
+
 struct PassArgs {
     PassJob job;            // used when jobs == nullptr
     const PassJob* jobs;    // device array, indexed by blockIdx.y
@@ -38,9 +40,11 @@ struct PassArgs {
     uint32_t log_s;         // row stride S
     uint32_t flags;
     const uint32_t* wtab;      // root table: [fwd/inv][prime][4096], Montgomery form
-    const uint32_t* beta;      // per-block roots for this pass, [prime][beta_stride], or null
+    // per-block roots of this pass: beta^(2^e) for e < log_r, [prime][beta_stride][kBetaPowStride],
+    // or null
+    const uint32_t* beta;
     const uint32_t* beta_inv;  // inverse roots (block pass only)
-    uint32_t beta_stride;
+    uint32_t beta_stride;      // blocks per prime
 };
 
 struct GarnerConsts {

@@ -52,23 +52,26 @@ def wtab(I, inv):
     return t
 
 
-def beta_table(I, log_n, log_s, nblk, inv):
+KBETA = 16  # words per block in the beta power tables (kBetaPowStride)
+
+
+def beta_table(I, log_n, log_s, nblk, inv, log_r):
+    """Host upload_beta: per block b, beta_b = w_{n/S}^{bitrev(b)} as beta_b^(2^e), e < log_r,
+    in Montgomery form, KBETA words per block."""
     if nblk <= 1:
         return None
+    assert log_r <= KBETA
     lb = nblk.bit_length() - 1
     w = root(I, log_n - log_s, inv)
-    return [to_mont(I, pow(w, bitrev(b, lb), P[I])) for b in range(nblk)]
+    return [[to_mont(I, pow(w, bitrev(b, lb) << e, P[I])) for e in range(log_r)] + [0] * (KBETA - log_r)
+            for b in range(nblk)]
 
 
-def build_tw(I, W, beta, log_r):
+def build_tw(I, W, bpow, log_r):
+    """Kernel build_twiddles: tw[k] = W[k] * beta^(2^(log_r - 1 - floor(log2 k)))."""
     R = 1 << log_r
-    if beta is None:
+    if bpow is None:
         return list(W[:R])
-    bpow = []
-    b = beta
-    for e in range(log_r):
-        bpow.append(b)
-        b = mont(I, b, b)
     tw = [0] * R
     for k in range(1, R):
         log_m = k.bit_length() - 1
@@ -282,7 +285,7 @@ def model_product(a, b, log_n):
     for I in range(3):
         Wf, Wi = wtab(I, False), wtab(I, True)
         for k, (log_r, log_c, log_s, nblk) in enumerate(outer):
-            bt = beta_table(I, log_n, log_s, nblk, False)
+            bt = beta_table(I, log_n, log_s, nblk, False, log_r)
             ntiles = 1 << (log_n - log_r - log_c)
             log_ncg = log_s - log_c
             gs = groups(log_r)
@@ -298,8 +301,8 @@ def model_product(a, b, log_n):
                     gio(res[op][I], n, X, 1 << gs[-1][0], log_c, 0, tile_base, log_s, 256, True)
         # middle pass
         nblk = n >> mid_log_r
-        bf = beta_table(I, log_n, 0, nblk, False)
-        bi = beta_table(I, log_n, 0, nblk, True)
+        bf = beta_table(I, log_n, 0, nblk, False, mid_log_r)
+        bi = beta_table(I, log_n, 0, nblk, True, mid_log_r)
         nthreads = (1 << mid_log_r) // KE
         gs = groups(mid_log_r)
         for blk in range(nblk):
@@ -319,7 +322,7 @@ def model_product(a, b, log_n):
             gio(res[0][I], n, XA, 1 << gs[0][0], 0, gs[0][1], tb, 0, nthreads, True)
         for k in range(len(outer) - 1, -1, -1):
             log_r, log_c, log_s, nblk = outer[k]
-            bt = beta_table(I, log_n, log_s, nblk, True)
+            bt = beta_table(I, log_n, log_s, nblk, True, log_r)
             ntiles = 1 << (log_n - log_r - log_c)
             log_ncg = log_s - log_c
             gs = groups(log_r)
