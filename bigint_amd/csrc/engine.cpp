@@ -113,6 +113,35 @@ struct ProfRec {
 
 // Host <-> device copies of caller (pageable) memory through two pinned chunks: the host copy of
 // one chunk overlaps the DMA of the other, and large host copies are split over a few threads.
+// Host side of a transfer: byte ranges in host memory, concatenated in order.
+struct HostSegs {
+    std::vector<char*> ptr;
+    std::vector<size_t> start;  // offset of each range in the concatenation, plus the total
+    HostSegs() : start{0} {}
+    void add(const void* p, size_t bytes) {
+        if (!bytes) return;
+        ptr.push_back((char*)p);
+        start.push_back(start.back() + bytes);
+    }
+    size_t total() const { return start.back(); }
+    // bytes [off, off + n) of the concatenation to (to_buf) or from buf
+    void copy(size_t off, size_t n, char* buf, bool to_buf) const {
+        size_t i = (size_t)(std::upper_bound(start.begin(), start.end(), off) - start.begin()) - 1;
+        while (n) {
+            const size_t in = off - start[i], len = std::min(n, start[i + 1] - off);
+            if (to_buf) std::memcpy(buf, ptr[i] + in, len);
+            else std::memcpy(ptr[i] + in, buf, len);
+            buf += len;
+            off += len;
+            n -= len;
+            ++i;
+        }
+    }
+};
+
+// Host <-> device copies through two pinned buffers: the host-side copy of one chunk (up to four
+// threads) overlaps the DMA of the other.  The host side may be a list of ranges, so batches
+// gather operands and scatter products without a staging copy.
 class PinnedPipe {
   public:
     static constexpr size_t kChunk = size_t(8) << 20;  // bytes per pinned buffer
@@ -120,15 +149,28 @@ class PinnedPipe {
     ~PinnedPipe() = default;  // pinned buffers live until the process ends
 
     hipError_t h2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
+        HostSegs h;
+        h.add(src, bytes);
+        return h2d(dst, h, st);
+    }
+    hipError_t d2h(void* dst, const void* src, size_t bytes, hipStream_t st) {
+        HostSegs h;
+        h.add(dst, bytes);
+        return d2h(h, src, st);
+    }
+
+    // Device bytes [0, src.total()) at dst receive the concatenation of src.
+    hipError_t h2d(void* dst, const HostSegs& src, hipStream_t st) {
         hipError_t e = ensure();
-        for (size_t off = 0, k = 0; e == hipSuccess && off < bytes; off += kChunk, ++k) {
+        const size_t bytes = src.total();
+        for (size_t off = 0; e == hipSuccess && off < bytes; off += kChunk) {
             const size_t n = std::min(kChunk, bytes - off), b = next_;
             next_ ^= 1;
             // the buffer may still feed a DMA of this call or of an earlier one
             if (pending_[b]) e = hipEventSynchronize(ev_[b]);
             if (e != hipSuccess) break;
             pending_[b] = false;
-            copy(buf_[b], (const char*)src + off, n);
+            copy(src, off, n, (char*)buf_[b], true);
             e = hipMemcpyAsync((char*)dst + off, buf_[b], n, hipMemcpyHostToDevice, st);
             if (e == hipSuccess) e = hipEventRecord(ev_[b], st);
             pending_[b] = e == hipSuccess;
@@ -136,9 +178,11 @@ class PinnedPipe {
         return e;
     }
 
-    // Waits for earlier work on st; returns when dst holds the data.
-    hipError_t d2h(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    // Device bytes [0, dst.total()) at src go to the ranges of dst.  Waits for earlier work on
+    // st; returns when dst holds the data.
+    hipError_t d2h(const HostSegs& dst, const void* src, hipStream_t st) {
         hipError_t e = ensure();
+        const size_t bytes = dst.total();
         const size_t nchunks = (bytes + kChunk - 1) / kChunk;
         auto issue = [&](size_t k) {
             const size_t off = k * kChunk, n = std::min(kChunk, bytes - off), b = k & 1;
@@ -152,7 +196,7 @@ class PinnedPipe {
             if (e == hipSuccess) e = hipEventSynchronize(ev_[k & 1]);
             if (e != hipSuccess) break;
             const size_t off = k * kChunk;
-            copy((char*)dst + off, buf_[k & 1], std::min(kChunk, bytes - off));
+            copy(dst, off, std::min(kChunk, bytes - off), (char*)buf_[k & 1], false);
         }
         pending_[0] = pending_[1] = false;  // every event recorded above has completed
         return e;
@@ -178,20 +222,21 @@ class PinnedPipe {
         return hipSuccess;
     }
 
-    static void copy(void* dst, const void* src, size_t n) {
+    // bytes [off, off + n) of h to or from buf, split over up to four threads
+    static void copy(const HostSegs& h, size_t off, size_t n, char* buf, bool to_buf) {
         constexpr size_t kSplit = size_t(2) << 20;
         const size_t nt = std::min<size_t>(4, std::max<size_t>(1, n / kSplit));
         if (nt == 1) {
-            std::memcpy(dst, src, n);
+            h.copy(off, n, buf, to_buf);
             return;
         }
         const size_t per = (n / nt + 63) & ~size_t(63);
         std::thread th[3];
         for (size_t t = 1; t < nt; ++t) {
             const size_t lo = std::min(n, t * per), hi = std::min(n, (t + 1) * per);
-            th[t - 1] = std::thread([=] { std::memcpy((char*)dst + lo, (const char*)src + lo, hi - lo); });
+            th[t - 1] = std::thread([&h, off, lo, hi, buf, to_buf] { h.copy(off + lo, hi - lo, buf + lo, to_buf); });
         }
-        std::memcpy(dst, src, std::min(n, per));
+        h.copy(off, std::min(n, per), buf, to_buf);
         for (size_t t = 1; t < nt; ++t) th[t - 1].join();
     }
 };
@@ -1227,20 +1272,22 @@ int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode) {
         e.stage_words = in_words + out_words;
     }
     uint32_t* dbuf = e.stage;
-    std::vector<uint32_t> hin(in_words);
+    // operands are gathered into the staging buffer and products scattered out of it through the
+    // pinned pipe, with no host-side staging copy
+    bi::HostSegs hin, hout;
     std::vector<bi_hip_dpair> dp(count);
     size_t io = 0, oo = in_words;
     for (size_t i = 0; i < count; ++i) {
         bi_hip_pair& p = pairs[i];
         const bool sq = p.a == p.b && p.la == p.lb;
-        std::memcpy(hin.data() + io, p.a, p.la * 4);
+        hin.add(p.a, p.la * 4);
         dp[i].a = dbuf + io;
         dp[i].la = p.la;
         io += p.la;
         if (sq) {
             dp[i].b = dp[i].a;
         } else {
-            std::memcpy(hin.data() + io, p.b, p.lb * 4);
+            hin.add(p.b, p.lb * 4);
             dp[i].b = dbuf + io;
             io += p.lb;
         }
@@ -1248,31 +1295,24 @@ int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode) {
         dp[i].out = dbuf + oo;
         dp[i].out_cap = p.la + p.lb;
         oo += p.la + p.lb;
+        hout.add(p.out, (p.la + p.lb) * 4);
     }
     int rc = BI_HIP_OK;
     {
         std::lock_guard<std::mutex> plk(e.pipe_mtx);
-        if (e.pipe.h2d(dbuf, hin.data(), in_words * 4, e.stream) != hipSuccess ||
-            hipStreamSynchronize(e.stream) != hipSuccess)
+        if (e.pipe.h2d(dbuf, hin, e.stream) != hipSuccess || hipStreamSynchronize(e.stream) != hipSuccess)
             rc = BI_HIP_EDEVICE;
     }
     if (!rc) rc = bi_hip_mul_batch_device(dp.data(), count, plan_mode, nullptr);
-    std::vector<uint32_t> hout;
     if (!rc) {
-        hout.resize(out_words);
         std::lock_guard<std::mutex> plk(e.pipe_mtx);
-        if (e.pipe.d2h(hout.data(), dbuf + in_words, out_words * 4, e.stream) != hipSuccess)
-            rc = BI_HIP_EDEVICE;
+        if (e.pipe.d2h(hout, dbuf + in_words, e.stream) != hipSuccess) rc = BI_HIP_EDEVICE;
     }
-    size_t o = 0;
     for (size_t i = 0; i < count; ++i) {
         bi_hip_pair& p = pairs[i];
         p.status = rc;
         if (rc) continue;
-        const size_t len = p.la + p.lb;
-        std::memcpy(p.out, hout.data() + o, len * 4);
-        o += len;
-        size_t n = len;
+        size_t n = p.la + p.lb;
         while (n > 1 && p.out[n - 1] == 0) --n;
         p.out_len = n;
     }

@@ -72,7 +72,11 @@ static std::vector<Term> leaves(size_type r) {
 // merge-addition milliseconds).
 static std::pair<BigInt, BigInt> split(size_type r, int plan_mode, double& gpu_levels_ms) {
     const bool verbose = std::getenv("PI_LEVELS") != nullptr;
+    const auto tl = clk::now();
     std::vector<Term> cur = leaves(r);
+    if (verbose)
+        std::cerr << "leaves: " << r << " terms, ms "
+                  << std::chrono::duration<double, std::milli>(clk::now() - tl).count() << "\n";
     int level = 0;
     while (cur.size() > 1) {
         const size_type m = cur.size() / 2;

@@ -38,20 +38,34 @@ auto words64(std::span<const uint32_t> x) -> size_type {
     return (bits + 63) / 64;
 }
 
-// Schoolbook on 28-bit limbs by columns: each column sums at most 586 partial products below
-// 2^56 (the host route only takes products of <= 16,384 bits), so a 128-bit accumulator holds it
-// with the carry from the column below.  No scratch array: the product is written as it goes.
-auto schoolbook(std::span<const uint32_t> a, std::span<const uint32_t> b) -> Digits {
-    const size_type na = a.size(), nb = b.size();
-    Digits r(na + nb, 0);
-    unsigned __int128 acc = 0;
+// Schoolbook on 28-bit limbs by columns.  Every partial product is below 2^56, so up to 256 of
+// them sum below 2^64: a column is summed in blocks of at most 256 products in 64-bit lanes (the
+// loop vectorizes), and each block sum is split at bit 28 into the column's own part and the
+// part carried into the next column.  The host route only takes products of <= 16,384 bits
+// (at most 586 terms per column), so the carried part stays below 2^40.  No scratch array: the
+// product is written as it goes.  Built for AVX2 with a generic fallback chosen at load time.
+__attribute__((target_clones("avx2", "default")))
+void schoolbook_cols(const uint32_t* a, size_type na, const uint32_t* b, size_type nb, uint32_t* r) {
+    uint64_t carry = 0;  // carry into column k
     for (size_type k = 0; k + 1 < na + nb; ++k) {
         const size_type lo = k + 1 > nb ? k + 1 - nb : 0, hi = std::min(k, na - 1);
-        for (size_type i = lo; i <= hi; ++i) acc += (uint64_t)a[i] * b[k - i];
-        r[k] = (uint32_t)acc & DIGIT_MASK;
-        acc >>= DIGIT_BITS;
+        uint64_t up = 0;
+        for (size_type i = lo; i <= hi;) {
+            const size_type e = std::min(hi + 1, i + 256);
+            uint64_t sum = 0;
+            for (; i < e; ++i) sum += (uint64_t)a[i] * b[k - i];
+            up += sum >> DIGIT_BITS;
+            carry += sum & DIGIT_MASK;
+        }
+        r[k] = (uint32_t)carry & DIGIT_MASK;
+        carry = (carry >> DIGIT_BITS) + up;
     }
-    r[na + nb - 1] = (uint32_t)acc;
+    r[na + nb - 1] = (uint32_t)carry;
+}
+
+auto schoolbook(std::span<const uint32_t> a, std::span<const uint32_t> b) -> Digits {
+    Digits r(a.size() + b.size());
+    schoolbook_cols(a.data(), a.size(), b.data(), b.size(), r.data());
     return r;
 }
 
@@ -181,12 +195,28 @@ auto mul_digits_batch(const std::vector<std::pair<const Digits*, const Digits*>>
     std::vector<Item> big;
     big.reserve(nbig);
     for (auto& part : big_parts) big.insert(big.end(), part.begin(), part.end());
+    // Sizing the result vectors touches (zero-fills) every page of the products: spread it over
+    // the pool's threads, products interleaved so that large ones land on different threads.
     std::vector<backend::Job> jobs(big.size());
-    for (size_t k = 0; k < big.size(); ++k) {
-        Item& g = big[k];
-        Digits& r = res[g.idx];
-        r.assign(g.za + g.zb + g.a.size() + g.b.size(), 0u);
-        jobs[k] = backend::Job{g.a, g.same ? g.a : g.b, g.same, r.data() + g.za + g.zb, 0};
+    auto size_range = [&](size_t t, size_t step) {
+        for (size_t k = t; k < big.size(); k += step) {
+            Item& g = big[k];
+            Digits& r = res[g.idx];
+            r.assign(g.za + g.zb + g.a.size() + g.b.size(), 0u);
+            jobs[k] = backend::Job{g.a, g.same ? g.a : g.b, g.same, r.data() + g.za + g.zb, 0};
+        }
+    };
+    size_t out_limbs = 0;
+    for (const Item& g : big) out_limbs += g.a.size() + g.b.size();
+    const size_t nsize = out_limbs < (size_t(1) << 20)
+                             ? 1
+                             : std::max<size_t>(1, std::min<size_t>(thread_pool_size(), big.size()));
+    if (nsize == 1) {
+        size_range(0, 1);
+    } else {
+        std::vector<std::thread> workers;
+        for (size_t t = 0; t < nsize; ++t) workers.emplace_back(size_range, t, nsize);
+        for (auto& w : workers) w.join();
     }
     timed_multiply(jobs, plan_mode);
     g_gpu_products.fetch_add(big.size(), std::memory_order_relaxed);

@@ -103,6 +103,27 @@ def test_integer_ops_small(calc):
     assert unhx(calc("inc", "-1")) == 0
 
 
+def test_host_schoolbook_extremes(calc):
+    """Products on the host route (<= 16,384 bits in total) with every limb all ones: the
+    column sums of the schoolbook are at their largest (up to 586 terms of (2^28 - 1)^2)."""
+    rng = random.Random(9)
+    def words64(bits):
+        return (bits + 63) // 64
+
+    cases = [(8192, 8192), (8191, 8192), (16000, 300), (16320, 1), (28 * 292, 8192), (1, 1)]
+    cases += [(rng.randrange(1, 8192), rng.randrange(1, 8192)) for _ in range(20)]
+    for a_bits, b_bits in cases:
+        assert (words64(a_bits) + words64(b_bits)) * 64 <= 16384  # the host route
+        a, b = (1 << a_bits) - 1, (1 << b_bits) - 1
+        assert unhx(calc("mul", hx(a), hx(b))) == a * b, (a_bits, b_bits)
+        if 2 * words64(a_bits) * 64 <= 16384:
+            assert unhx(calc("sq", hx(a))) == a * a, a_bits
+    for _ in range(20):
+        a = rng.getrandbits(8192) | (1 << 8191)
+        b = rng.getrandbits(8192) | (1 << 8191)
+        assert unhx(calc("mul", hx(a), hx(b))) == a * b
+
+
 def test_bitwise_small(calc):
     rng = random.Random(2)
     for _ in range(100):
