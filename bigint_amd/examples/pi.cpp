@@ -121,6 +121,11 @@ static std::pair<BigInt, BigInt> split(size_type r, int plan_mode, double& gpu_l
 
 int main(int argc, char** argv) {
     init_thread_pool(8);
+    // Decimal output splits x = q 10^k + r down to pieces below this many limbs, which are
+    // converted by repeated division (quadratic).  The split tree's products run as batches, so
+    // pieces smaller than the library default (2000 limbs) are cheaper overall; the digits are
+    // the same for any threshold.
+    BigInt::DEC_STRING_BRUTE_THRESHOLDS[1] = 256;
     size_type dec_digits = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 1000;
     const int plan_mode = argc > 2 ? std::atoi(argv[2]) : 1;
     const char* out_path = argc > 3 ? argv[3] : "pi.txt";

@@ -117,14 +117,25 @@ auto BigInt::convert_from_dec_string(const std::string& s, size_type start, size
 // ---- printing (src/bigint.cpp:126-265) ------------------------------------------------------------
 auto BigInt::to_dec_string_brute() const -> std::string {
     if (is_zero()) return "0";
-    constexpr size_type chunk_digits = 19;  // 10^19 < 2^64
-    constexpr uint64_t chunk = 10000000000000000000ull;
-    BigInt t = *this;
+    // Repeated division of the magnitude by 10^10, ten digits per step: every partial dividend
+    // rem * 2^28 + limb is below 10^10 * 2^28 < 2^62, so each limb costs one 64-bit division by a
+    // constant (a multiply-high), and each quotient limb stays below 2^28.
+    constexpr uint64_t chunk = 10000000000ull;
+    constexpr int chunk_digits = 10;
+    std::vector<uint32_t> t(data_.begin(), data_.end());
+    size_type n = t.size();
     std::string r;
-    r.reserve(chunk_digits + (size_type)std::ceil((double)data_.size() * DIGIT_BITS * std::log10(2.0)));
-    while (!t.is_zero()) {
-        uint64_t rem = t.unsigned_inplace_divmod(chunk);
-        for (size_type i = 0; i < chunk_digits && (rem != 0 || !t.is_zero()); ++i) {
+    r.reserve(chunk_digits + 1 + (size_type)std::ceil((double)data_.size() * DIGIT_BITS * std::log10(2.0)));
+    while (n > 0) {
+        uint64_t rem = 0;
+        for (size_type i = n; i-- > 0;) {
+            const uint64_t cur = (rem << DIGIT_BITS) | t[i];
+            const uint64_t q = cur / chunk;
+            t[i] = (uint32_t)q;
+            rem = cur - q * chunk;
+        }
+        while (n > 0 && t[n - 1] == 0) --n;
+        for (int i = 0; i < chunk_digits && (rem != 0 || n > 0); ++i) {
             r.push_back((char)('0' + rem % TEN));
             rem /= TEN;
         }
@@ -157,62 +168,102 @@ void BigInt::print(std::ostream& output, bool hex, bool direct) const {
 
 // Same digits as the reference's recursive split (src/bigint.cpp:126-265): x = q 10^k + r with
 // k half of x's decimal length, down to pieces below DEC_STRING_BRUTE_THRESHOLDS[1] limbs.
-// The decimal string of a number is unique, so three things here only change the speed:
-//   - 10^k and its reciprocal are computed once per distinct k (nodes of one depth share a few
-//     k values) instead of once per division, and the quotient is corrected until exact;
-//   - the brute-force conversions of the pieces run on the init_thread_pool threads;
+// The decimal string of a number is unique, so four things here only change the speed:
+//   - the split tree is walked one depth at a time: the quotient estimates q = floor(x / 10^k) of
+//     all nodes of a depth are one product batch (x times a reciprocal of 10^k), and so are the
+//     products q 10^k; the estimates are then corrected until exact;
+//   - 10^k and its reciprocal are computed once per distinct k (a depth has one or two);
+//   - subtractions, corrections and the brute-force conversions of the pieces run on the
+//     init_thread_pool threads;
 //   - digits are written in blocks.
 void BigInt::print_dec(std::ostream& output, size_type len) const {
+    if (is_neg_) {  // the sign, then the magnitude padded to len digits
+        output.put('-');
+        BigInt m(*this);
+        m.is_neg_ = false;
+        m.print_dec(output, len);
+        return;
+    }
     const size_type thr = DEC_STRING_BRUTE_THRESHOLDS[1];
     auto small = [&](const BigInt& x) { return x.data_.size() == 1 || x.data_.size() < thr; };
-    std::vector<std::pair<BigInt, size_type>> pieces;  // in output order
-    if (small(*this)) {
-        pieces.emplace_back(*this, len);
-    } else {
-        struct Div {
-            BigInt d;
-            BigFloat recip;
-            size_type prec;
+    auto parallel_for = [](size_t count, auto&& f) {
+        const size_t nt = std::min<size_t>(thread_pool_size(), count);
+        std::atomic<size_t> next{0};
+        auto work = [&] {
+            for (size_t i; (i = next.fetch_add(1)) < count;) f(i);
         };
-        std::map<size_type, Div> divs;
-        std::vector<std::pair<BigInt, size_type>> stack;
-        auto split = [&](const BigInt& x, size_type width) {
+        std::vector<std::thread> threads;
+        for (size_t t = 1; t < nt; ++t) threads.emplace_back(work);
+        work();
+        for (auto& t : threads) t.join();
+    };
+    struct Div {
+        BigInt d;      // 10^k
+        BigInt recip;  // digits of a reciprocal of 10^k: 1/10^k ~ recip * 2^(-28 shift)
+        int64_t shift;
+        size_type prec;
+    };
+    std::map<size_type, Div> divs;
+    std::vector<std::pair<BigInt, size_type>> pieces;  // in output order: (value, width)
+    pieces.emplace_back(*this, len);
+    for (;;) {
+        std::vector<size_t> at;  // pieces split at this depth
+        for (size_t i = 0; i < pieces.size(); ++i)
+            if (!small(pieces[i].first)) at.push_back(i);
+        if (at.empty()) break;
+        std::vector<size_type> ks(at.size());
+        std::map<size_type, size_type> need;  // k -> precision in limbs
+        for (size_t j = 0; j < at.size(); ++j) {
+            const BigInt& x = pieces[at[j]].first;
             const size_type est = (size_type)std::ceil((double)x.data_.size() * DIGIT_BITS * std::log10(2.0));
-            const size_type k = est / 2;
-            const size_type prec = x.data_.size() + 16;
+            ks[j] = est / 2;
+            size_type& p = need[ks[j]];
+            p = std::max(p, x.data_.size() + 16);
+        }
+        for (const auto& [k, prec] : need) {
             auto it = divs.find(k);
-            if (it == divs.end()) {
-                BigInt d = get_pow_of_ten((uint32_t)k);
-                BigFloat rc = BigFloat(d).reciprocal(prec + 2);
-                it = divs.emplace(k, Div{std::move(d), std::move(rc), prec + 2}).first;
-            } else if (it->second.prec < prec) {
-                it->second.recip = BigFloat(it->second.d).reciprocal(prec + 2);
-                it->second.prec = prec + 2;
-            }
-            const BigInt& d = it->second.d;
-            BigInt q(BigFloat::mul(BigFloat(x), it->second.recip, prec), RoundMode::Truncate);
-            BigInt r(x - q * d);
-            // exact truncated division of |x| by d: r in [0, d) with the sign of x
-            const bool neg = x.is_neg_;
-            while (!r.is_zero() && r.is_neg_ != neg) {
-                r += neg ? -d : d;
-                neg ? ++q : --q;
+            if (it != divs.end() && it->second.prec >= prec + 2) continue;
+            BigInt d = it == divs.end() ? get_pow_of_ten((uint32_t)k) : std::move(it->second.d);
+            const BigFloat rc = BigFloat(d).reciprocal(prec + 2);
+            BigInt r;
+            r.data_ = rc.get_data();
+            r.remove_leading_zero();
+            divs[k] = Div{std::move(d), std::move(r), rc.get_point_pos(), prec + 2};
+        }
+        // q = floor(x * recip * 2^(-28 shift)): within a few units of floor(x / 10^k)
+        std::vector<std::pair<const BigInt*, const BigInt*>> jobs(at.size());
+        for (size_t j = 0; j < at.size(); ++j) jobs[j] = {&pieces[at[j]].first, &divs.at(ks[j]).recip};
+        std::vector<BigInt> qs = mul_batch(jobs);
+        parallel_for(at.size(), [&](size_t j) { qs[j] >>= (int64_t)DIGIT_BITS * divs.at(ks[j]).shift; });
+        for (size_t j = 0; j < at.size(); ++j) jobs[j] = {&qs[j], &divs.at(ks[j]).d};
+        std::vector<BigInt> rs = mul_batch(jobs);
+        parallel_for(at.size(), [&](size_t j) {
+            const BigInt& d = divs.at(ks[j]).d;
+            BigInt& q = qs[j];
+            BigInt& r = rs[j];
+            r = pieces[at[j]].first - r;
+            while (r.is_neg_) {
+                r += d;
+                --q;
             }
             while (compare_abs(r, d) >= 0) {
-                r -= neg ? -d : d;
-                neg ? --q : ++q;
+                r -= d;
+                ++q;
             }
-            r.remove_sign();
-            stack.emplace_back(std::move(r), k);
-            stack.emplace_back(std::move(q), width > k ? width - k : 0);
-        };
-        split(*this, len);
-        while (!stack.empty()) {
-            auto [x, width] = std::move(stack.back());
-            stack.pop_back();
-            if (small(x)) pieces.emplace_back(std::move(x), width);
-            else split(x, width);
+        });
+        std::vector<std::pair<BigInt, size_type>> next;
+        next.reserve(pieces.size() + at.size());
+        for (size_t i = 0, j = 0; i < pieces.size(); ++i) {
+            if (j < at.size() && at[j] == i) {
+                const size_type width = pieces[i].second, k = ks[j];
+                next.emplace_back(std::move(qs[j]), width > k ? width - k : 0);
+                next.emplace_back(std::move(rs[j]), k);
+                ++j;
+            } else {
+                next.push_back(std::move(pieces[i]));
+            }
         }
+        pieces = std::move(next);
     }
     std::vector<std::string> strs(pieces.size());
     std::atomic<size_t> next{0};

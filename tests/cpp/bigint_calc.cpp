@@ -11,6 +11,7 @@
 //   frecip A pa prec | fprint A pa digits | ftodouble A pa | ffromdouble X (hex bits)
 //   fround A pa mode prec rel
 //   counts                   -> "<gpu products> <host products>"
+//   decthr N                 -> sets DEC_STRING_BRUTE_THRESHOLDS[1] (limbs) and prints "ok"
 //   batch N SEED MAXBITS     -> checksum of BigInt::mul_batch over N SplitMix64 pairs (see below)
 #include <bit>
 #include <cstdio>
@@ -198,6 +199,11 @@ int main() {
                     sum = (sum + r * (i + 1)) % M;
                 }
                 out << (uint64_t)sum;
+            } else if (op == "decthr") {
+                size_type t;
+                in >> t;
+                BigInt::DEC_STRING_BRUTE_THRESHOLDS[1] = t;
+                out << "ok";
             } else if (op == "counts") {
                 out << mul::gpu_product_count() << " " << mul::host_product_count();
             } else {

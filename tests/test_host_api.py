@@ -156,6 +156,24 @@ def test_decimal_round_trip_small(calc):
         assert unhx(calc("fromdec", str(a))) == a
 
 
+def test_decimal_split_tree_small(calc):
+    """Decimal output through the split tree (x = q 10^k + r, one depth at a time, quotients
+    corrected until exact) on the host route: the brute-force threshold is lowered to a few limbs
+    so that numbers of a few thousand bits split many levels deep, with zero runs to pad."""
+    rng = random.Random(14)
+    try:
+        for thr in [2, 3, 5, 40]:
+            assert calc("decthr", thr) == "ok"
+            for _ in range(25):
+                a = rnd(rng, 3000)
+                assert calc("todec", hx(a)) == str(a), (thr, a)
+            for z in [10 ** 600, 10 ** 600 - 1, 10 ** 500 + 7, 123 * 10 ** 300 + 10 ** 150 + 9,
+                      -(10 ** 400 + 1), (1 << 2800) - 1]:
+                assert calc("todec", hx(z)) == str(z), (thr, z)
+    finally:
+        assert calc("decthr", 2000) == "ok"
+
+
 def test_bigfloat_mul_add_small(calc):
     rng = random.Random(5)
     for _ in range(200):
