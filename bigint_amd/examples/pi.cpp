@@ -4,17 +4,22 @@
 // operation (BigFloat Newton iteration for 1/sqrt(10005), the reciprocal of 13591409*Q + P,
 // precision-truncated products, decimal printing), so the digits are the reference's.  The
 // splitting itself runs level by level: the reference merges terms with a streaming binary
-// counter (examples/pi.cpp:13-80), one `operator*` at a time; here every level's products go to
-// the engine together (BigInt::mul_batch), and the leaves and each level's additions are spread
-// over the init_thread_pool threads.  The merge (P, Q, R) o (P', Q', R') =
-// (P Q' + R P', Q Q', R R') is associative, so any bracketing gives the same P(1, r) and Q(1, r).
+// counter (examples/pi.cpp:13-80), one `operator*` at a time; here blocks of 64 consecutive terms
+// are merged first in fixed-size limb arrays (no BigInt per leaf or per merge), then every
+// level's products go to the engine together (BigInt::mul_batch), with the blocks and each
+// level's additions spread over the init_thread_pool threads.  The merge (P, Q, R) o (P', Q', R')
+// = (P Q' + R P', Q Q', R R') is associative, so any bracketing gives the same P(1, r) and
+// Q(1, r).
 //
 // usage: pi <decimal digits> [plan mode 0|1|2] [output file] [planner file]
+#include <bit>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
+#include <memory>
+#include <stdexcept>
 #include <thread>
 #include <vector>
 
@@ -28,10 +33,11 @@ struct Term {
 };
 
 // Runs make(lo, hi) on the init_thread_pool threads over [0, count) split into contiguous ranges
-// and concatenates the results in order.
+// (at least min_per_thread items each) and concatenates the results in order.
 template <class F>
-static std::vector<Term> parallel_terms(size_type count, F make) {
-    const size_type nt = std::max<size_type>(1, std::min<size_type>(thread_pool_size(), count / 4096));
+static std::vector<Term> parallel_terms(size_type count, F make, size_type min_per_thread = 4096) {
+    const size_type nt =
+        std::max<size_type>(1, std::min<size_type>(thread_pool_size(), count / min_per_thread));
     std::vector<std::vector<Term>> parts(nt);
     std::vector<std::thread> threads;
     const size_type per = (count + nt - 1) / nt;
@@ -45,26 +51,160 @@ static std::vector<Term> parallel_terms(size_type count, F make) {
     return out;
 }
 
+// ---- the lowest levels: blocks of kBlock terms merged in fixed-size limb arrays ----------------
+
+// Signed integer of at most kCap 28-bit limbs (n == 0 is zero).
+struct Small {
+    static constexpr uint32_t kCap = 384;
+    uint32_t n = 0;
+    bool neg = false;
+    uint32_t d[kCap];
+};
+
+static void small_set(Small& x, uint128_t v, bool neg) {
+    x.n = 0;
+    for (; v; v >>= DIGIT_BITS) x.d[x.n++] = (uint32_t)(v & DIGIT_MASK);
+    x.neg = neg && x.n;
+}
+
+static void small_mul_u64(Small& x, uint64_t v) {
+    uint128_t c = 0;
+    for (uint32_t i = 0; i < x.n; ++i) {
+        c += (uint128_t)x.d[i] * v;
+        x.d[i] = (uint32_t)(c & DIGIT_MASK);
+        c >>= DIGIT_BITS;
+    }
+    for (; c; c >>= DIGIT_BITS) x.d[x.n++] = (uint32_t)(c & DIGIT_MASK);
+    if (!v) x.n = 0;
+}
+
+// r = |a| |b| (r distinct from a and b; the caller sets the sign).  Column sums in 64 bits,
+// flushed every 256 products (each below 2^56).
+static void small_mul(Small& r, const Small& a, const Small& b) {
+    r.neg = false;
+    if (!a.n || !b.n) {
+        r.n = 0;
+        return;
+    }
+    const uint32_t na = a.n, nb = b.n;
+    if (na + nb > Small::kCap) throw std::length_error("pi: block term too large");
+    uint64_t carry = 0;
+    for (uint32_t k = 0; k + 1 < na + nb; ++k) {
+        const uint32_t lo = k + 1 > nb ? k + 1 - nb : 0, hi = std::min(k, na - 1);
+        uint64_t up = 0;
+        for (uint32_t i = lo; i <= hi;) {
+            const uint32_t e = std::min(hi + 1, i + 256);
+            uint64_t sum = 0;
+            for (; i < e; ++i) sum += (uint64_t)a.d[i] * b.d[k - i];
+            up += sum >> DIGIT_BITS;
+            carry += sum & DIGIT_MASK;
+        }
+        r.d[k] = (uint32_t)carry & DIGIT_MASK;
+        carry = (carry >> DIGIT_BITS) + up;
+    }
+    r.d[na + nb - 1] = (uint32_t)carry;
+    r.n = na + nb;
+    while (r.n && r.d[r.n - 1] == 0) --r.n;
+}
+
+static int small_cmp_abs(const Small& a, const Small& b) {
+    if (a.n != b.n) return a.n < b.n ? -1 : 1;
+    for (uint32_t i = a.n; i-- > 0;)
+        if (a.d[i] != b.d[i]) return a.d[i] < b.d[i] ? -1 : 1;
+    return 0;
+}
+
+// r = x + y (signed; r distinct from x and y)
+static void small_add(Small& r, const Small& x, const Small& y) {
+    if (x.neg == y.neg) {
+        const Small& a = x.n >= y.n ? x : y;
+        const Small& b = x.n >= y.n ? y : x;
+        if (a.n + 1 > Small::kCap) throw std::length_error("pi: block term too large");
+        uint32_t c = 0;
+        for (uint32_t i = 0; i < a.n; ++i) {
+            const uint32_t t = a.d[i] + (i < b.n ? b.d[i] : 0u) + c;
+            r.d[i] = t & DIGIT_MASK;
+            c = t >> DIGIT_BITS;
+        }
+        r.n = a.n;
+        if (c) r.d[r.n++] = c;
+        r.neg = x.neg && r.n;
+        return;
+    }
+    const bool x_big = small_cmp_abs(x, y) >= 0;
+    const Small& a = x_big ? x : y;
+    const Small& b = x_big ? y : x;
+    uint32_t borrow = 0;
+    for (uint32_t i = 0; i < a.n; ++i) {
+        const uint32_t s = (i < b.n ? b.d[i] : 0u) + borrow;
+        borrow = a.d[i] < s;
+        r.d[i] = (a.d[i] + (borrow << DIGIT_BITS) - s) & DIGIT_MASK;
+    }
+    r.n = a.n;
+    while (r.n && r.d[r.n - 1] == 0) --r.n;
+    r.neg = a.neg && r.n;
+}
+
+struct SmallTerm {
+    Small P, Q, R;
+};
+
 // Leaf i (examples/pi.cpp:26-44): Q = Q0 i^3, R = (2i-1)(6i-1)(6i-5), P = (-1)^i R (13591409 +
-// 545140134 i) with the same 128-bit intermediate arithmetic.
-static std::vector<Term> leaves(size_type r) {
+// 545140134 i), R and P with the same 128-bit intermediate arithmetic.
+static void leaf_term(uint64_t i, SmallTerm& t) {
     constexpr uint64_t Q0 = 10939058860032000ull;  // 640320^3 / 24
-    return parallel_terms(r, [](size_type lo, size_type hi) {
+    small_set(t.Q, Q0, false);
+    small_mul_u64(t.Q, i);
+    small_mul_u64(t.Q, i);
+    small_mul_u64(t.Q, i);
+    const auto rr = static_cast<uint128_t>(2 * i - 1) * (6 * i - 1) * (6 * i - 5);
+    const uint64_t p0 = 13591409 + 545140134 * i;
+    small_set(t.R, rr, false);
+    small_set(t.P, rr * p0, i % 2 != 0);
+}
+
+// (P, Q, R) of terms lo+1 .. hi: the merge (P, Q, R) o (P', Q', R') = (P Q' + R P', Q Q', R R').
+static void block_term(uint64_t lo, uint64_t hi, SmallTerm& out) {
+    if (hi - lo == 1) {
+        leaf_term(hi, out);
+        return;
+    }
+    const uint64_t mid = lo + (hi - lo) / 2;
+    auto parts = std::make_unique<SmallTerm[]>(2);
+    block_term(lo, mid, parts[0]);
+    block_term(mid, hi, parts[1]);
+    const SmallTerm& l = parts[0];
+    const SmallTerm& h = parts[1];
+    Small t1, t2;
+    small_mul(t1, l.P, h.Q);
+    t1.neg = l.P.neg && t1.n;
+    small_mul(t2, l.R, h.P);
+    t2.neg = h.P.neg && t2.n;
+    small_add(out.P, t1, t2);
+    small_mul(out.Q, l.Q, h.Q);
+    small_mul(out.R, l.R, h.R);
+}
+
+static BigInt to_bigint(const Small& x) {
+    return BigInt::from_digits(Digits(x.d, x.d + x.n), x.neg);
+}
+
+// Terms 1..r merged in blocks of kBlock consecutive terms (the last block may be shorter): the
+// first log2(kBlock) levels of the splitting without a BigInt per leaf or per merge.
+constexpr uint64_t kBlock = 64;
+
+static std::vector<Term> leaves(size_type r) {
+    const size_type nblocks = (r + kBlock - 1) / kBlock;
+    return parallel_terms(nblocks, [r](size_type lo, size_type hi) {
         std::vector<Term> t;
         t.reserve(hi - lo);
-        for (size_type i = lo + 1; i <= hi; ++i) {
-            BigInt q(Q0);
-            q *= i;
-            q *= i;
-            q *= i;
-            const auto rr = static_cast<uint128_t>(2 * i - 1) * (6 * i - 1) * (6 * i - 5);
-            const uint64_t p0 = 13591409 + 545140134 * (uint64_t)i;
-            BigInt p(rr * p0);
-            if (i % 2) p.flip_sign();
-            t.push_back(Term{std::move(p), std::move(q), BigInt(rr)});
+        SmallTerm st;
+        for (size_type b = lo; b < hi; ++b) {
+            block_term(b * kBlock, std::min<uint64_t>(r, (b + 1) * kBlock), st);
+            t.push_back(Term{to_bigint(st.P), to_bigint(st.Q), to_bigint(st.R)});
         }
         return t;
-    });
+    }, 16);
 }
 
 // Level-wise binary splitting; returns (P, Q) of terms 1..r.  With PI_LEVELS set in the
@@ -75,9 +215,9 @@ static std::pair<BigInt, BigInt> split(size_type r, int plan_mode, double& gpu_l
     const auto tl = clk::now();
     std::vector<Term> cur = leaves(r);
     if (verbose)
-        std::cerr << "leaves: " << r << " terms, ms "
+        std::cerr << "leaves: " << r << " terms in blocks of " << kBlock << ", ms "
                   << std::chrono::duration<double, std::milli>(clk::now() - tl).count() << "\n";
-    int level = 0;
+    int level = std::countr_zero(kBlock);  // the block merge did the levels below
     while (cur.size() > 1) {
         const size_type m = cur.size() / 2;
         const bool top = cur.size() <= 2;

@@ -542,6 +542,15 @@ auto BigInt::bitwise_not(size_type len) -> BigInt& {
     return *this;
 }
 
+auto BigInt::from_digits(Digits digits, bool negative) -> BigInt {
+    BigInt r;
+    r.data_ = std::move(digits);
+    if (r.data_.empty()) r.data_.push_back(0);
+    r.is_neg_ = negative;
+    r.remove_leading_zero();
+    return r;
+}
+
 auto BigInt::mul_batch(const std::vector<std::pair<const BigInt*, const BigInt*>>& pairs,
                        int plan_mode) -> std::vector<BigInt> {
     std::vector<std::pair<const Digits*, const Digits*>> dp;

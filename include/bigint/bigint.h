@@ -235,6 +235,11 @@ public:
 
     static auto get_pow_of_ten(uint32_t exponent) -> BigInt;
 
+    // The integer with magnitude sum digits[i] 2^(28 i) (every limb < 2^28) and the given sign,
+    // canonicalized (no leading zero limbs, no negative zero).  An extension of the reference
+    // API for callers that build numbers limb by limb.
+    static auto from_digits(Digits digits, bool negative = false) -> BigInt;
+
     // a_i * b_i for every pair, each equal to operator*; the big ones run as one grouped GPU
     // launch set (one product-tree level at a time).
     static auto mul_batch(const std::vector<std::pair<const BigInt*, const BigInt*>>& pairs,
