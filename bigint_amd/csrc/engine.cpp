@@ -1163,17 +1163,11 @@ int bi_hip_mul_device(const uint32_t* d_a, size_t la, const uint32_t* d_b, size_
                         d_out, st);
 }
 
-int bi_hip_mul(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, int is_square,
-               uint32_t* out, size_t out_cap, size_t* out_len) {
-    if (bi::check_operand(a, la) || (!is_square && bi::check_operand(b, lb)) || !out || !out_len)
-        return BI_HIP_EINVAL;
-    if (is_square) {
-        b = a;
-        lb = la;
-    }
-    if (a[la - 1] == 0 || b[lb - 1] == 0) return BI_HIP_EINVAL;
-    if (out_cap < la + lb) return BI_HIP_ECAP;
-    if (!bi::Engine::product_supported(la, lb)) return BI_HIP_ETOOBIG;
+namespace {
+
+// a * b for host limb arrays; limbs [la + lb - keep, la + lb) of the product go to out.
+int mul_host(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, int is_square,
+             size_t keep, uint32_t* out) {
     std::lock_guard<std::mutex> lk(engine().mtx);
     bi::Engine& e = engine();
     int rc = e.init(0);
@@ -1196,10 +1190,43 @@ int bi_hip_mul(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, int i
     if (!is_square && e.pipe.h2d(db, b, lb * 4, e.stream) != hipSuccess) return BI_HIP_EDEVICE;
     rc = e.run(da, (uint32_t)la, db, (uint32_t)lb, is_square != 0, dout, e.stream);
     if (rc) return rc;
-    if (e.pipe.d2h(out, dout, (la + lb) * 4, e.stream) != hipSuccess) return BI_HIP_EDEVICE;
+    if (e.pipe.d2h(out, dout + (la + lb - keep), keep * 4, e.stream) != hipSuccess) return BI_HIP_EDEVICE;
+    return BI_HIP_OK;
+}
+
+int check_host_product(const uint32_t* a, size_t la, const uint32_t*& b, size_t& lb, int is_square) {
+    if (bi::check_operand(a, la) || (!is_square && bi::check_operand(b, lb))) return BI_HIP_EINVAL;
+    if (is_square) {
+        b = a;
+        lb = la;
+    }
+    if (a[la - 1] == 0 || b[lb - 1] == 0) return BI_HIP_EINVAL;
+    if (!bi::Engine::product_supported(la, lb)) return BI_HIP_ETOOBIG;
+    return BI_HIP_OK;
+}
+
+}  // namespace
+
+int bi_hip_mul(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, int is_square,
+               uint32_t* out, size_t out_cap, size_t* out_len) {
+    if (!out || !out_len) return BI_HIP_EINVAL;
+    if (int rc = check_host_product(a, la, b, lb, is_square)) return rc;
+    if (out_cap < la + lb) return BI_HIP_ECAP;
+    if (int rc = mul_host(a, la, b, lb, is_square, la + lb, out)) return rc;
     size_t n = la + lb;
     while (n > 1 && out[n - 1] == 0) --n;
     *out_len = n;
+    return BI_HIP_OK;
+}
+
+int bi_hip_mul_top(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, int is_square,
+                   size_t keep, uint32_t* out, size_t* out_len) {
+    if (!out || !out_len || keep == 0) return BI_HIP_EINVAL;
+    if (int rc = check_host_product(a, la, b, lb, is_square)) return rc;
+    const size_t k = std::min(keep, la + lb);
+    if (int rc = mul_host(a, la, b, lb, is_square, k, out)) return rc;
+    // operands with nonzero top limbs: the product has la + lb or la + lb - 1 limbs
+    *out_len = la + lb - (out[k - 1] == 0 && la + lb > 1 ? 1 : 0);
     return BI_HIP_OK;
 }
 

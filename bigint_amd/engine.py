@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libbigint_hip.so")
 OK, EINVAL, ETOOBIG, ENOMEM, EDEVICE, ENODEV, ECAP = range(7)
 PLAN_FIXED, PLAN_TABLE, PLAN_LEARNED = 0, 1, 2
 
-EXPORTS = ["bi_hip_init", "bi_hip_mul", "bi_hip_mul_device", "bi_hip_mul_batch",
+EXPORTS = ["bi_hip_init", "bi_hip_mul", "bi_hip_mul_top", "bi_hip_mul_device", "bi_hip_mul_batch",
            "bi_hip_synchronize", "bi_hip_transform_log2", "bi_hip_strerror", "bi_hip_profile",
            "bi_hip_profile_read", "bi_hip_mul_batch_device", "bi_hip_plan_count",
            "bi_hip_plan_allowed", "bi_hip_plan_cost", "bi_hip_mul_batch_device_planned", "bi_hip_choose_plans",
@@ -62,6 +62,8 @@ def lib() -> ctypes.CDLL:
         L.bi_hip_init.argtypes = [ctypes.c_int]
         L.bi_hip_mul.argtypes = [u32p, ctypes.c_size_t, u32p, ctypes.c_size_t, ctypes.c_int, u32p,
                                  ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+        L.bi_hip_mul_top.argtypes = [u32p, ctypes.c_size_t, u32p, ctypes.c_size_t, ctypes.c_int,
+                                     ctypes.c_size_t, u32p, ctypes.POINTER(ctypes.c_size_t)]
         L.bi_hip_mul_device.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
                                         ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p,
                                         ctypes.c_size_t, ctypes.c_void_p]
@@ -113,6 +115,20 @@ def mul(a: np.ndarray, b: np.ndarray | None = None) -> np.ndarray:
     if rc:
         raise EngineError(rc, "bi_hip_mul")
     return out[: n.value]
+
+
+def mul_top(a: np.ndarray, b: np.ndarray | None, keep: int) -> tuple[np.ndarray, int]:
+    """The top min(keep, |a| + |b|) limbs of a * b (b=None squares a) and the length of the whole
+    product without leading zeros."""
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    sq = b is None
+    bb = a if sq else np.ascontiguousarray(b, dtype=np.uint32)
+    out = np.zeros(max(1, min(keep, a.size + bb.size)), dtype=np.uint32)
+    n = ctypes.c_size_t()
+    rc = lib().bi_hip_mul_top(_p(a), a.size, _p(bb), bb.size, int(sq), keep, _p(out), ctypes.byref(n))
+    if rc:
+        raise EngineError(rc, "bi_hip_mul_top")
+    return out, n.value
 
 
 def mul_device(d_a: int, la: int, d_b: int, lb: int, square: bool, d_out: int, out_cap: int,

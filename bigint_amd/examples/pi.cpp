@@ -28,6 +28,10 @@
 using namespace bigint;
 using clk = std::chrono::steady_clock;
 
+#ifndef PI_DEC_PIECE_LIMBS
+#define PI_DEC_PIECE_LIMBS 256
+#endif
+
 struct Term {
     BigInt P, Q, R;
 };
@@ -262,10 +266,11 @@ static std::pair<BigInt, BigInt> split(size_type r, int plan_mode, double& gpu_l
 int main(int argc, char** argv) {
     init_thread_pool(8);
     // Decimal output splits x = q 10^k + r down to pieces below this many limbs, which are
-    // converted by repeated division (quadratic).  The split tree's products run as batches, so
-    // pieces smaller than the library default (2000 limbs) are cheaper overall; the digits are
-    // the same for any threshold.
-    BigInt::DEC_STRING_BRUTE_THRESHOLDS[1] = 256;
+    // converted by repeated division (quadratic).  On the engine the split tree's products run
+    // as batches, so pieces smaller than the library default (2000 limbs) are cheaper overall; a
+    // build on a product backend without batching (the CPU baseline) keeps the default through
+    // PI_DEC_PIECE_LIMBS.  The digits are the same for any threshold.
+    BigInt::DEC_STRING_BRUTE_THRESHOLDS[1] = PI_DEC_PIECE_LIMBS;
     size_type dec_digits = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 1000;
     const int plan_mode = argc > 2 ? std::atoi(argv[2]) : 1;
     const char* out_path = argc > 3 ? argv[3] : "pi.txt";
@@ -282,6 +287,13 @@ int main(int argc, char** argv) {
     const auto t_split = clk::now();
     const double eng_split = mul::engine_time_ms();
 
+    const bool verbose = std::getenv("PI_LEVELS") != nullptr;
+    auto phase = [&, t = clk::now()](const char* name) mutable {
+        if (!verbose) return;
+        const auto now = clk::now();
+        std::cerr << name << " ms " << std::chrono::duration<double, std::milli>(now - t).count() << "\n";
+        t = now;
+    };
     BigFloat pi(Q);
     pi *= 4270934400;
     BigFloat s(1.0 / std::sqrt(10005));
@@ -296,8 +308,15 @@ int main(int argc, char** argv) {
             s = BigFloat::mul(s, a - ss, cur);
         }
     }
+    phase("1/sqrt(10005)");
     pi = BigFloat::mul(pi, s, inner);
-    pi = BigFloat::mul(pi, BigFloat(13591409 * Q + P).reciprocal(inner), inner);
+    phase("Q * 4270934400 / sqrt(10005)");
+    BigFloat den(13591409 * Q + P);
+    phase("13591409 Q + P");
+    BigFloat rden = den.reciprocal(inner);
+    phase("reciprocal");
+    pi = BigFloat::mul(pi, rden, inner);
+    phase("final product");
     const auto t2 = clk::now();
     const double eng_compute = mul::engine_time_ms();
 

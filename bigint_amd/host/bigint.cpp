@@ -749,35 +749,46 @@ auto BigFloat::add_or_sub(const BigFloat& a, const BigFloat& b, bool is_sub) -> 
     const int64_t pp = std::max(a.point_pos_ - tza, b.point_pos_ - tzb);
     const int64_t lr = std::max(la - a.point_pos_, lb - b.point_pos_) + pp;
     const int64_t oa = pp - a.point_pos_, ob = pp - b.point_pos_;
-    // signed limb accumulation, then normalisation
-    std::vector<int64_t> acc((size_t)lr, 0);
-    for (int64_t i = tza; i < la; ++i) acc[(size_t)(i + oa)] = a.data_[(size_t)i];
+    // limb i of the result frame holds a.data_[i - oa] for i in [a_lo, a_hi), likewise for b
+    const int64_t a_lo = tza + oa, a_hi = la + oa, b_lo = tzb + ob, b_hi = lb + ob;
+    const uint32_t* ad = a.data_.data() - oa;
+    const uint32_t* bd = b.data_.data() - ob;
+    auto A = [&](int64_t i) -> uint32_t { return i >= a_lo && i < a_hi ? ad[i] : 0u; };
+    auto B = [&](int64_t i) -> uint32_t { return i >= b_lo && i < b_hi ? bd[i] : 0u; };
     const bool subtract = a.is_neg_ != (b.is_neg_ != is_sub);
-    for (int64_t i = tzb; i < lb; ++i) acc[(size_t)(i + ob)] += subtract ? -(int64_t)b.data_[(size_t)i] : (int64_t)b.data_[(size_t)i];
     BigFloat res;
     res.point_pos_ = pp;
-    res.is_neg_ = a.is_neg_;
-    // sign of the signed-limb value = sign of its top nonzero limb
-    int64_t top = lr - 1;
-    while (top >= 0 && acc[(size_t)top] == 0) --top;
-    if (top < 0) {
-        res.data_.assign(1, 0u);
-        res.point_pos_ = 0;
-        res.is_neg_ = false;
-        return res;
-    }
-    if (acc[(size_t)top] < 0) {
-        res.is_neg_ = !res.is_neg_;
-        for (auto& x : acc) x = -x;
-    }
     res.data_.resize((size_t)lr + 1);
-    int64_t carry = 0;
-    for (int64_t i = 0; i < lr; ++i) {
-        int64_t v = acc[(size_t)i] + carry;
-        carry = v >> DIGIT_BITS;  // arithmetic shift: floor division
-        res.data_[(size_t)i] = (uint32_t)(v & DIGIT_MASK);
+    uint32_t* r = res.data_.data();
+    if (!subtract) {  // |a| + |b| with the sign of a
+        res.is_neg_ = a.is_neg_;
+        uint32_t c = 0;
+        for (int64_t i = 0; i < lr; ++i) {
+            const uint32_t t = A(i) + B(i) + c;
+            r[i] = t & DIGIT_MASK;
+            c = t >> DIGIT_BITS;
+        }
+        r[lr] = c;
+    } else {  // the larger magnitude minus the smaller, with the larger one's sign
+        int64_t top = lr - 1;
+        while (top >= 0 && A(top) == B(top)) --top;
+        if (top < 0) {
+            res.data_.assign(1, 0u);
+            res.point_pos_ = 0;
+            res.is_neg_ = false;
+            return res;
+        }
+        const bool a_big = A(top) > B(top);
+        res.is_neg_ = a_big ? a.is_neg_ : !a.is_neg_;
+        uint32_t borrow = 0;
+        for (int64_t i = 0; i < lr; ++i) {
+            const uint32_t x = a_big ? A(i) : B(i);
+            const uint32_t y = (a_big ? B(i) : A(i)) + borrow;
+            borrow = x < y;
+            r[i] = (x + (borrow << DIGIT_BITS) - y) & DIGIT_MASK;
+        }
+        r[lr] = 0;
     }
-    res.data_[(size_t)lr] = (uint32_t)carry;
     res.remove_leading_zero();
     res.remove_tail_zero();
     return res;

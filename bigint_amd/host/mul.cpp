@@ -114,6 +114,35 @@ auto mul_digits(const Digits& a, const Digits& b, size_type output_precision,
 
     const std::span<const uint32_t> sa(a.data() + off_a, a.size() - off_a);
     const std::span<const uint32_t> sb(b.data() + off_b, b.size() - off_b);
+    if (output_precision) {
+        // Only the top output_precision limbs survive (below): on the engine route, fetch just
+        // the top output_precision + 1 limbs of the product (its length is la + lb or
+        // la + lb - 1) instead of the whole product.
+        size_type la = sa.size(), lb = sb.size();
+        while (la > 1 && sa[la - 1] == 0) --la;
+        while (lb > 1 && sb[lb - 1] == 0) --lb;
+        const size_type keep = output_precision + 1;
+        if ((words64(sa) + words64(sb)) * 64 > HOST_MAX_TOTAL_BITS && la + lb > keep) {
+            Digits top(keep);
+            const auto t0 = std::chrono::steady_clock::now();
+            const size_type n = backend::multiply_top(sa.first(la), sb.first(lb), same, keep, top.data());
+            g_backend_ns.fetch_add((uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                                       std::chrono::steady_clock::now() - t0).count(),
+                                   std::memory_order_relaxed);
+            g_gpu_products.fetch_add(1, std::memory_order_relaxed);
+            // top[i] is product limb (la + lb - keep + i); keep limbs [n - precision, n), then
+            // drop their trailing zero limbs
+            size_type off_r = n - output_precision;
+            size_type i = off_r - (la + lb - keep);
+            while (top[i] == 0) {
+                ++i;
+                ++off_r;
+            }
+            Digits res(top.begin() + (std::ptrdiff_t)i, top.begin() + (std::ptrdiff_t)(i + (n - off_r)));
+            if (p_result_point_pos) *p_result_point_pos -= (int64_t)(off_a + off_b + off_r);
+            return res;
+        }
+    }
     Digits res = mul_limbs(sa, sb, same);
     while (res.size() > 1 && res.back() == 0) res.pop_back();
 

@@ -20,6 +20,11 @@ struct Job {
 // Each job's product into its out buffer; throws on failure.
 void multiply(std::vector<Job>& jobs, int plan_mode);
 
+// The top min(keep, |a| + |b|) limbs of a * b into out (keep >= 1); returns the length of the
+// whole product without leading zeros.  Throws on failure.
+auto multiply_top(std::span<const uint32_t> a, std::span<const uint32_t> b, bool square, size_t keep,
+                  uint32_t* out) -> size_t;
+
 // Install a plan table / cost model file; false if unsupported or unusable.
 bool load_planner(const char* path);
 

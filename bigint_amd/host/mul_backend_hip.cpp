@@ -30,6 +30,16 @@ void multiply(std::vector<Job>& jobs, int plan_mode) {
     for (size_t k = 0; k < jobs.size(); ++k) jobs[k].out_len = hp[k].out_len;
 }
 
+auto multiply_top(std::span<const uint32_t> a, std::span<const uint32_t> b, bool square, size_t keep,
+                  uint32_t* out) -> size_t {
+    size_t len = 0;
+    const int rc = bi_hip_mul_top(a.data(), a.size(), square ? a.data() : b.data(),
+                                  square ? a.size() : b.size(), square ? 1 : 0, keep, out, &len);
+    if (rc == BI_HIP_ENOMEM) throw std::bad_alloc();
+    if (rc != BI_HIP_OK) throw engine_error(rc, bi_hip_strerror(rc));
+    return len;
+}
+
 bool load_planner(const char* path) { return bi_hip_load_planner(path) == BI_HIP_OK; }
 
 }  // namespace bigint::mul::backend

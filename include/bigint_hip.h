@@ -73,6 +73,14 @@ int bi_hip_init(int device);
 int bi_hip_mul(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, int is_square,
                uint32_t* out, size_t out_cap, size_t* out_len);
 
+/* The top limbs of a * b for host limb arrays: limbs [la + lb - k, la + lb) of the product go to
+ * out[0 .. k) with k = min(keep, la + lb), keep >= 1, and *out_len receives the length of the
+ * whole product without leading zero limbs.  Serves precision-truncated products (the
+ * reference's mul_digits with output_precision, include/bigint/mul.h:227-243, 302-312), whose
+ * low limbs are dropped: only the kept limbs cross to the host. */
+int bi_hip_mul_top(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, int is_square,
+                   size_t keep, uint32_t* out, size_t* out_len);
+
 /* Device-resident variant: a, b, out are device pointers; exactly la + lb limbs are written
  * (the top one may be zero).  Asynchronous on `stream` (a hipStream_t; NULL = the engine's
  * stream), except that a chunked plan waits for earlier work on the stream before it reuses the

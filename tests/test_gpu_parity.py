@@ -134,6 +134,30 @@ def test_error_paths():
     assert e.value.code == engine.ETOOBIG
 
 
+def test_mul_top_matches_full_product():
+    """bi_hip_mul_top (the precision-truncated route of mul_digits) returns exactly the top limbs
+    of the oracle's product and the whole product's length, including products whose top limb
+    is zero and squares."""
+    rng = random.Random(21)
+    cases = [(L.random_operand(56000, 1), L.random_operand(84000, 2)),
+             (L.random_operand(200000, 3), None),
+             (np.array([1] + [0] * 2999 + [1], np.uint32), np.array([5] * 1999 + [1], np.uint32)),
+             (np.full(3000, (1 << 28) - 1, np.uint32), np.full(2500, (1 << 28) - 1, np.uint32))]
+    for a, b in cases:
+        full = oracle.mul_limbs(a, b)
+        m = a.size + (a.size if b is None else b.size)
+        padded = np.zeros(m, np.uint32)
+        padded[: full.size] = full
+        for keep in [1, 2, 7, rng.randrange(1, m), m - 1, m, m + 5]:
+            top, n = engine.mul_top(a, b, keep)
+            k = min(keep, m)
+            assert n == full.size, keep
+            assert np.array_equal(top[:k], padded[m - k:]), keep
+    with pytest.raises(engine.EngineError) as e:
+        engine.mul_top(cases[0][0], cases[0][1], 0)
+    assert e.value.code == engine.EINVAL
+
+
 def test_batch_matches_single():
     rng = random.Random(4)
     pairs = []
