@@ -19,8 +19,8 @@ ARCH = "gfx950"
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++20", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + os.path.join(ROOT, "include"), "-Wall"]
-SOURCES = ["ntt_kernels.hip", "planner_kernels.hip", "engine.cpp"]
-HOST_SOURCES = ["host/mul.cpp", "host/mul_backend_hip.cpp", "host/bigint.cpp"]
+SOURCES = ["ntt_kernels.hip", "add_kernels.hip", "planner_kernels.hip", "engine.cpp"]
+HOST_SOURCES = ["host/mul.cpp", "host/mul_backend_hip.cpp", "host/bigint.cpp", "host/device_int.cpp"]
 HOST_LIB = os.path.join(HERE, "libbigint.so")
 CXX = os.environ.get("BIGINT_CXX", "/opt/rocm/llvm/bin/clang++")
 HOST_FLAGS = ["-O2", "-std=gnu++20", "-fPIC", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"),

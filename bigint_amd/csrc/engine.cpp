@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/bigint_hip.h"
+#include "add.h"
 #include "ntt_plan.h"
 #include "ntt_primes.h"
 #include "planner.h"
@@ -597,6 +598,10 @@ public:
     std::mutex stage_mtx;
     uint32_t* stage = nullptr;
     size_t stage_words = 0;
+    // scratch of bi_hip_add_batch_device (descriptors, tile maps, carries, flags), grow-only
+    std::mutex add_mtx;
+    uint8_t* add_scratch = nullptr;
+    size_t add_bytes = 0;
 
     // Plan of one product: transform length, optional chunking of the longer operand, and the
     // size of the contiguous block pass (which fixes the radix split of the outer stages).
@@ -1531,6 +1536,116 @@ int bi_hip_profile_read(bi_hip_kstat* stats, size_t cap) {
         stats[r.kind].launches += 1;
     }
     return BI_HIP_OK;
+}
+
+int bi_hip_add_batch_device(bi_hip_dadd* jobs, size_t count, void* stream) {
+    if (!jobs && count) return BI_HIP_EINVAL;
+    if (count == 0) return BI_HIP_OK;
+    if (count > 0xFFFFu) return BI_HIP_EINVAL;  // one grid row per job
+    std::vector<bi::AddJob> hj(count);
+    size_t tiles = 0;
+    uint32_t max_tiles = 0;
+    for (size_t i = 0; i < count; ++i) {
+        const bi_hip_dadd& d = jobs[i];
+        if (!d.x || !d.y || !d.out || d.lx == 0 || d.ly == 0) return BI_HIP_EINVAL;
+        const size_t width = std::max(d.lx, d.ly) + (d.subtract ? 0 : 1);
+        if (d.out_cap < width) return BI_HIP_ECAP;
+        const size_t nt = (width + bi::kAddTileLimbs - 1) / bi::kAddTileLimbs;
+        if (nt > 0xFFFFFFFFull) return BI_HIP_ETOOBIG;
+        hj[i] = bi::AddJob{d.x, d.lx, d.y, d.ly, d.out, width, nullptr, nullptr, nullptr,
+                           d.subtract ? 1u : 0u, (uint32_t)nt};
+        tiles += nt;
+        max_tiles = std::max(max_tiles, (uint32_t)nt);
+    }
+    if (max_tiles > 0x7FFFFFFFu) return BI_HIP_ETOOBIG;
+    bi::Engine& e = engine();
+    {
+        std::lock_guard<std::mutex> lk(e.mtx);
+        if (int rc = e.init(0)) return rc;
+    }
+    std::lock_guard<std::mutex> lk(e.add_mtx);
+    hipStream_t st = stream ? (hipStream_t)stream : e.stream;
+    // scratch: descriptors, then maps and carries (one word per tile each), then 2 flags per job
+    const size_t desc_bytes = (count * sizeof(bi::AddJob) + 255) & ~size_t(255);
+    const size_t need = desc_bytes + (2 * tiles + 2 * count) * 4;
+    if (need > e.add_bytes) {
+        if (e.add_scratch && hipStreamSynchronize(st) != hipSuccess) return BI_HIP_EDEVICE;
+        if (e.add_scratch) (void)hipFree(e.add_scratch);
+        e.add_scratch = nullptr;
+        e.add_bytes = 0;
+        if (hipMalloc((void**)&e.add_scratch, need) != hipSuccess) return BI_HIP_ENOMEM;
+        e.add_bytes = need;
+    }
+    uint32_t* words = (uint32_t*)(e.add_scratch + desc_bytes);
+    uint32_t* flags = words + 2 * tiles;
+    size_t off = 0;
+    for (size_t i = 0; i < count; ++i) {
+        hj[i].maps = words + off;
+        hj[i].carry_in = words + tiles + off;
+        hj[i].flags = flags + 2 * i;
+        off += hj[i].ntiles;
+    }
+    auto* d_jobs = (bi::AddJob*)e.add_scratch;
+    if (hipMemcpyAsync(d_jobs, hj.data(), count * sizeof(bi::AddJob), hipMemcpyHostToDevice, st) != hipSuccess)
+        return BI_HIP_EDEVICE;
+    if (bi::launch_add_batch(d_jobs, (uint32_t)count, max_tiles, st) != hipSuccess) return BI_HIP_EDEVICE;
+    std::vector<uint32_t> hf(2 * count);
+    if (hipMemcpyAsync(hf.data(), flags, hf.size() * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return BI_HIP_EDEVICE;
+    for (size_t i = 0; i < count; ++i) {
+        jobs[i].swapped = (int)hf[2 * i];
+        jobs[i].out_len = hf[2 * i + 1];
+    }
+    return BI_HIP_OK;
+}
+
+int bi_hip_device_alloc(size_t bytes, void** ptr) {
+    if (!ptr) return BI_HIP_EINVAL;
+    *ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(engine().mtx);
+        if (int rc = engine().init(0)) return rc;
+    }
+    if (bytes == 0) return BI_HIP_OK;
+    return hipMalloc(ptr, bytes) == hipSuccess ? BI_HIP_OK : BI_HIP_ENOMEM;
+}
+
+int bi_hip_device_free(void* ptr) {
+    if (!ptr) return BI_HIP_OK;
+    return hipFree(ptr) == hipSuccess ? BI_HIP_OK : BI_HIP_EDEVICE;
+}
+
+int bi_hip_copy_to_device(void* d_dst, const void* const* h_src, const size_t* bytes, size_t count) {
+    if ((!h_src || !bytes) && count) return BI_HIP_EINVAL;
+    bi::HostSegs h;
+    for (size_t i = 0; i < count; ++i) h.add(h_src[i], bytes[i]);
+    if (h.total() == 0) return BI_HIP_OK;
+    if (!d_dst) return BI_HIP_EINVAL;
+    bi::Engine& e = engine();
+    {
+        std::lock_guard<std::mutex> lk(e.mtx);
+        if (int rc = e.init(0)) return rc;
+    }
+    std::lock_guard<std::mutex> plk(e.pipe_mtx);
+    if (e.pipe.h2d(d_dst, h, e.stream) != hipSuccess || hipStreamSynchronize(e.stream) != hipSuccess)
+        return BI_HIP_EDEVICE;
+    return BI_HIP_OK;
+}
+
+int bi_hip_copy_to_host(void* const* h_dst, const size_t* bytes, size_t count, const void* d_src) {
+    if ((!h_dst || !bytes) && count) return BI_HIP_EINVAL;
+    bi::HostSegs h;
+    for (size_t i = 0; i < count; ++i) h.add(h_dst[i], bytes[i]);
+    if (h.total() == 0) return BI_HIP_OK;
+    if (!d_src) return BI_HIP_EINVAL;
+    bi::Engine& e = engine();
+    {
+        std::lock_guard<std::mutex> lk(e.mtx);
+        if (int rc = e.init(0)) return rc;
+    }
+    std::lock_guard<std::mutex> plk(e.pipe_mtx);
+    return e.pipe.d2h(h, d_src, e.stream) == hipSuccess ? BI_HIP_OK : BI_HIP_EDEVICE;
 }
 
 int bi_hip_synchronize(void) {

@@ -21,10 +21,18 @@ EXPORTS = ["bi_hip_init", "bi_hip_mul", "bi_hip_mul_top", "bi_hip_mul_device", "
            "bi_hip_profile_read", "bi_hip_mul_batch_device", "bi_hip_plan_count",
            "bi_hip_plan_allowed", "bi_hip_plan_cost", "bi_hip_mul_batch_device_planned", "bi_hip_choose_plans",
            "bi_hip_set_planner", "bi_hip_load_planner", "bi_hip_mlp_param_count",
-           "bi_hip_mlp_forward", "bi_hip_mlp_grad", "bi_hip_mlp_adam", "bi_hip_set_graphs"]
+           "bi_hip_mlp_forward", "bi_hip_mlp_grad", "bi_hip_mlp_adam", "bi_hip_set_graphs",
+           "bi_hip_add_batch_device", "bi_hip_device_alloc", "bi_hip_device_free",
+           "bi_hip_copy_to_device", "bi_hip_copy_to_host"]
 KERNEL_NAMES = ["fill", "outer_fwd", "mid", "outer_inv", "garner_carry", "outer_first"]
 
 u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+class DAdd(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_void_p), ("lx", ctypes.c_size_t), ("y", ctypes.c_void_p),
+                ("ly", ctypes.c_size_t), ("subtract", ctypes.c_int), ("out", ctypes.c_void_p),
+                ("out_cap", ctypes.c_size_t), ("swapped", ctypes.c_int), ("out_len", ctypes.c_size_t)]
 
 
 class Pair(ctypes.Structure):
@@ -80,6 +88,13 @@ def lib() -> ctypes.CDLL:
         L.bi_hip_mlp_grad.argtypes = [vp, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp, vp]
         L.bi_hip_mlp_adam.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                       ctypes.c_float, ctypes.c_float, ctypes.c_int, vp]
+        L.bi_hip_add_batch_device.argtypes = [ctypes.POINTER(DAdd), ctypes.c_size_t, ctypes.c_void_p]
+        L.bi_hip_device_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]
+        L.bi_hip_device_free.argtypes = [ctypes.c_void_p]
+        L.bi_hip_copy_to_device.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                            ctypes.POINTER(ctypes.c_size_t), ctypes.c_size_t]
+        L.bi_hip_copy_to_host.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t),
+                                          ctypes.c_size_t, ctypes.c_void_p]
         L.bi_hip_strerror.restype = ctypes.c_char_p
         L.bi_hip_strerror.argtypes = [ctypes.c_int]
         _lib = L
@@ -129,6 +144,18 @@ def mul_top(a: np.ndarray, b: np.ndarray | None, keep: int) -> tuple[np.ndarray,
     if rc:
         raise EngineError(rc, "bi_hip_mul_top")
     return out, n.value
+
+
+def add_batch_device(jobs: list[tuple[int, int, int, int, bool, int, int]]) -> list[tuple[bool, int]]:
+    """Device-resident signed-magnitude additions: jobs of (d_x, lx, d_y, ly, subtract, d_out,
+    out_cap) with raw device addresses; returns (swapped, out_len) per job."""
+    arr = (DAdd * len(jobs))()
+    for i, (x, lx, y, ly, sub, out, cap) in enumerate(jobs):
+        arr[i] = DAdd(x, lx, y, ly, int(sub), out, cap, 0, 0)
+    rc = lib().bi_hip_add_batch_device(arr, len(jobs), None)
+    if rc:
+        raise EngineError(rc, "bi_hip_add_batch_device")
+    return [(bool(a.swapped), int(a.out_len)) for a in arr]
 
 
 def mul_device(d_a: int, la: int, d_b: int, lb: int, square: bool, d_out: int, out_cap: int,

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "bigint/bigint.h"
+#include "bigint/device.h"
 
 using namespace bigint;
 using clk = std::chrono::steady_clock;
@@ -214,6 +215,10 @@ static std::vector<Term> leaves(size_type r) {
 // Level-wise binary splitting; returns (P, Q) of terms 1..r.  With PI_LEVELS set in the
 // environment, one line per level goes to stderr (merges, largest Q in limbs, product and
 // merge-addition milliseconds).
+//
+// Host levels run until every product of the next level is above the host schoolbook's range;
+// from there the tree stays in the product backend's memory (DeviceInt): each level is one
+// product batch and one batch of signed additions, and only P(1, r) and Q(1, r) come back.
 static std::pair<BigInt, BigInt> split(size_type r, int plan_mode, double& gpu_levels_ms) {
     const bool verbose = std::getenv("PI_LEVELS") != nullptr;
     const auto tl = clk::now();
@@ -222,7 +227,13 @@ static std::pair<BigInt, BigInt> split(size_type r, int plan_mode, double& gpu_l
         std::cerr << "leaves: " << r << " terms in blocks of " << kBlock << ", ms "
                   << std::chrono::duration<double, std::milli>(clk::now() - tl).count() << "\n";
     int level = std::countr_zero(kBlock);  // the block merge did the levels below
-    while (cur.size() > 1) {
+    // R R' is a level's smallest product; at >= 320 limbs per factor it is above 16,384 bits
+    auto all_big = [&] {
+        for (const Term& t : cur)
+            if (t.R.get_data().size() < 320) return false;
+        return true;
+    };
+    while (cur.size() > 1 && !all_big()) {
         const size_type m = cur.size() / 2;
         const bool top = cur.size() <= 2;
         std::vector<std::pair<const BigInt*, const BigInt*>> jobs;
@@ -260,7 +271,66 @@ static std::pair<BigInt, BigInt> split(size_type r, int plan_mode, double& gpu_l
         ++level;
         cur = std::move(next);
     }
-    return {std::move(cur[0].P), std::move(cur[0].Q)};
+    if (cur.size() == 1) return {std::move(cur[0].P), std::move(cur[0].Q)};
+
+    struct DTerm {
+        DeviceInt P, Q, R;
+    };
+    const auto tu = clk::now();
+    std::vector<DTerm> dc(cur.size());
+    {
+        std::vector<const BigInt*> xs;
+        xs.reserve(3 * cur.size());
+        for (const Term& t : cur) {
+            xs.push_back(&t.P);
+            xs.push_back(&t.Q);
+            xs.push_back(&t.R);
+        }
+        std::vector<DeviceInt> d = DeviceInt::upload(xs);
+        for (size_t i = 0; i < cur.size(); ++i) dc[i] = DTerm{std::move(d[3 * i]), std::move(d[3 * i + 1]), std::move(d[3 * i + 2])};
+        cur.clear();
+    }
+    if (verbose)
+        std::cerr << "upload of " << dc.size() << " terms, ms "
+                  << std::chrono::duration<double, std::milli>(clk::now() - tu).count() << "\n";
+    while (dc.size() > 1) {
+        const size_type m = dc.size() / 2;
+        const bool top = dc.size() <= 2;
+        const size_type per = top ? 3 : 4;
+        std::vector<std::pair<const DeviceInt*, const DeviceInt*>> jobs;
+        jobs.reserve(per * m);
+        for (size_type k = 0; k < m; ++k) {
+            const DTerm& l = dc[2 * k];
+            const DTerm& h = dc[2 * k + 1];
+            jobs.emplace_back(&l.P, &h.Q);
+            jobs.emplace_back(&l.R, &h.P);
+            jobs.emplace_back(&l.Q, &h.Q);
+            if (!top) jobs.emplace_back(&l.R, &h.R);
+        }
+        const auto t0 = clk::now();
+        std::vector<DeviceInt> prod = DeviceInt::mul_batch(jobs, plan_mode);
+        std::vector<std::pair<const DeviceInt*, const DeviceInt*>> adds(m);
+        for (size_type k = 0; k < m; ++k) adds[k] = {&prod[per * k], &prod[per * k + 1]};
+        std::vector<DeviceInt> sums = DeviceInt::add_batch(adds);
+        const double pms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+        gpu_levels_ms += pms;
+        std::vector<DTerm> next(m);
+        for (size_type k = 0; k < m; ++k)
+            next[k] = DTerm{std::move(sums[k]), std::move(prod[per * k + 2]),
+                            top ? DeviceInt() : std::move(prod[per * k + 3])};
+        if (dc.size() % 2) next.push_back(std::move(dc.back()));
+        if (verbose)
+            std::cerr << "level " << level << " (device): merges " << m << ", largest Q limbs "
+                      << next.front().Q.len() << ", products + additions ms " << pms << "\n";
+        ++level;
+        dc = std::move(next);
+    }
+    const auto td = clk::now();
+    std::vector<BigInt> pq = DeviceInt::download({&dc[0].P, &dc[0].Q});
+    if (verbose)
+        std::cerr << "download of P, Q, ms "
+                  << std::chrono::duration<double, std::milli>(clk::now() - td).count() << "\n";
+    return {std::move(pq[0]), std::move(pq[1])};
 }
 
 int main(int argc, char** argv) {

@@ -100,6 +100,39 @@ typedef struct bi_hip_dpair {
     size_t out_cap;
 } bi_hip_dpair;
 
+/* One signed-magnitude addition of a device-resident batch: x, y, out are device pointers to
+ * 28-bit limbs (top limbs may be zero).  subtract == 0: out = |x| + |y| (out_cap >= max(lx, ly)
+ * + 1); else out = the larger magnitude minus the smaller (out_cap >= max(lx, ly)), with
+ * swapped = 1 when |y| > |x|.  out_len receives the result length without leading zero limbs (0
+ * for zero). */
+typedef struct bi_hip_dadd {
+    const uint32_t* x;
+    size_t lx;
+    const uint32_t* y;
+    size_t ly;
+    int subtract;
+    uint32_t* out;
+    size_t out_cap;
+    int swapped;
+    size_t out_len;
+} bi_hip_dadd;
+
+/* Batched additions on `stream` (NULL = the engine's stream); returns once swapped and out_len
+ * are known.  For product-tree levels kept in HBM, whose merge P = P_l Q_r + R_l P_r adds two
+ * products (examples/pi.cpp:45-58 does it with BigInt operator+, include/bigint/bigint.h, and
+ * src/bigint.cpp:267-548 on the host). */
+int bi_hip_add_batch_device(bi_hip_dadd* jobs, size_t count, void* stream);
+
+/* Device memory for callers that keep numbers in HBM between batches (a product tree):
+ * hipMalloc / hipFree on the engine's device, and copies through the engine's pinned buffers
+ * that gather host ranges into (or scatter device bytes out to) one contiguous device range.
+ * Both copies are synchronous.  No counterpart in the reference (its numbers live in host
+ * memory). */
+int bi_hip_device_alloc(size_t bytes, void** ptr);
+int bi_hip_device_free(void* ptr);
+int bi_hip_copy_to_device(void* d_dst, const void* const* h_src, const size_t* bytes, size_t count);
+int bi_hip_copy_to_host(void* const* h_dst, const size_t* bytes, size_t count, const void* d_src);
+
 /* Device-resident batch, asynchronous on `stream` (NULL = the engine's stream).
  * BI_HIP_PLAN_FIXED runs one launch sequence per product; the other modes group products of
  * equal transform length into one launch sequence per group.  The call waits for earlier work

@@ -12,6 +12,7 @@
 //   fround A pa mode prec rel
 //   counts                   -> "<gpu products> <host products>"
 //   decthr N                 -> sets DEC_STRING_BRUTE_THRESHOLDS[1] (limbs) and prints "ok"
+//   devmuladd A B C D        -> A*B + C*D and A*A through DeviceInt (backend memory): "S SQ"
 //   batch N SEED MAXBITS     -> checksum of BigInt::mul_batch over N SplitMix64 pairs (see below)
 #include <bit>
 #include <cstdio>
@@ -22,6 +23,7 @@
 #include <vector>
 
 #include "bigint/bigint.h"
+#include "bigint/device.h"
 
 using namespace bigint;
 
@@ -199,6 +201,15 @@ int main() {
                     sum = (sum + r * (i + 1)) % M;
                 }
                 out << (uint64_t)sum;
+            } else if (op == "devmuladd") {
+                std::string c, d;
+                in >> a >> b >> c >> d;
+                const BigInt A = H(a), B = H(b), C = H(c), D = H(d);
+                auto dv = DeviceInt::upload({&A, &B, &C, &D});
+                auto pr = DeviceInt::mul_batch({{&dv[0], &dv[1]}, {&dv[2], &dv[3]}, {&dv[0], &dv[0]}});
+                auto sm = DeviceInt::add_batch({{&pr[0], &pr[1]}});
+                auto back = DeviceInt::download({&sm[0], &pr[2]});
+                out << X(back[0]) << " " << X(back[1]);
             } else if (op == "decthr") {
                 size_type t;
                 in >> t;

@@ -158,6 +158,50 @@ def test_mul_top_matches_full_product():
     assert e.value.code == engine.EINVAL
 
 
+def test_device_add_batch_against_python_ints():
+    """bi_hip_add_batch_device: sums and differences of device limb arrays, bit-exact against
+    Python ints, with carries and borrows running through whole 4096-limb tiles, equal
+    magnitudes (zero), both subtraction directions, unequal lengths and zero top limbs."""
+    from bigint_amd import hip
+    rng = random.Random(33)
+    M = (1 << 28) - 1
+
+    def limbs_of(v, n):
+        return np.array([(v >> (28 * i)) & M for i in range(n)], np.uint32)
+
+    def val(a):
+        return sum(int(x) << (28 * i) for i, x in enumerate(a))
+
+    cases = []
+    for lx, ly in [(1, 1), (5, 3), (4096, 4096), (4097, 1), (1, 9000), (12289, 12288), (30000, 29999)]:
+        cases.append((L.random_operand(28 * lx, rng.getrandbits(63))[:lx],
+                      L.random_operand(28 * ly, rng.getrandbits(63))[:ly]))
+    ones = np.full(9000, M, np.uint32)
+    cases.append((ones, np.array([1], np.uint32)))            # carry through three tiles
+    big = np.zeros(9000, np.uint32)
+    big[-1] = 1
+    cases.append((big, np.array([1], np.uint32)))             # borrow through three tiles
+    same = L.random_operand(28 * 5000, 7)
+    cases.append((same, same.copy()))                         # equal magnitudes
+    cases.append((np.append(same, np.uint32(0)), same.copy()))  # zero top limb
+    bufs, jobs, expect = [], [], []
+    for x, y in cases:
+        for sub in (False, True):
+            dx, dy = hip.Buffer.from_array(x), hip.Buffer.from_array(y)
+            cap = max(x.size, y.size) + 1
+            do = hip.Buffer(4 * cap)
+            bufs += [dx, dy, do]
+            jobs.append((dx.ptr, x.size, dy.ptr, y.size, sub, do.ptr, cap))
+            vx, vy = val(x), val(y)
+            expect.append((do, vx + vy if not sub else abs(vx - vy), sub and vy > vx))
+    res = engine.add_batch_device(jobs)
+    for (do, want, swapped), (sw, n) in zip(expect, res):
+        got = do.download(np.uint32, max(n, 1))
+        assert val(got[:n]) == want
+        assert n == (want.bit_length() + 27) // 28  # 0 for zero
+        assert sw == swapped
+
+
 def test_batch_matches_single():
     rng = random.Random(4)
     pairs = []

@@ -271,6 +271,30 @@ def test_large_ops_through_engine(calc):
 
 
 @pytest.mark.gpu
+def test_device_int_mul_add(calc):
+    """DeviceInt (numbers kept in HBM between batches): A*B + C*D and A*A through the engine's
+    device product batch and device additions, against Python ints; signs, carries into a new
+    limb, exact cancellation to zero, and operands big enough for chunked transforms."""
+    rng = random.Random(17)
+    cases = []
+    for bits in [30000, 200000, 3000000]:
+        for _ in range(3):
+            a, b = rnd(rng, bits) or 1, rnd(rng, bits) or 1
+            c, d = rnd(rng, bits) or 1, rnd(rng, bits) or 1
+            cases.append((a, b, c, d))
+    x = rng.getrandbits(100000) | 1
+    y = rng.getrandbits(90000) | 1
+    cases.append((x, y, -x, y))                   # cancels to zero
+    cases.append((x, y, x - 1, -y))               # y
+    ones = (1 << 60000) - 1
+    cases.append((ones, ones, ones, 2 * ones + 1))  # carries into a new limb
+    for a, b, c, d in cases:
+        s, sq = calc("devmuladd", hx(a), hx(b), hx(c), hx(d)).split()
+        assert unhx(s) == a * b + c * d
+        assert unhx(sq) == a * a
+
+
+@pytest.mark.gpu
 def test_large_reciprocal_matches_model(calc):
     rng = random.Random(10)
     for prec in [600, 3000]:

@@ -658,3 +658,104 @@ def test_crt_recovers_every_coefficient():
             assert kernel_crt(*r, n) == x
             if r[0] < P[0]:
                 assert kernel_crt(r[0] + P[0], r[1], r[2], n) == x
+
+
+# ---- batched device additions (bigint_amd/csrc/add_kernels.hip) -------------------------------
+
+ATILE, ATHREADS, ALIMBS = 4096, 256, 16
+AIDENT = 0b10
+
+
+def amap_apply(m, c):
+    return (m >> c) & 1
+
+
+def amap_compose(second, first):
+    return amap_apply(second, amap_apply(first, 0)) | (amap_apply(second, amap_apply(first, 1)) << 1)
+
+
+def astep(a, b, c, sub):
+    if not sub:
+        s = a + b + c
+        return s >> 28, s & MASK
+    t = b + c
+    return (1 if a < t else 0), (a - t) & MASK
+
+
+def athread_map(a, b, sub):
+    c0, c1 = 0, 1
+    for x, y in zip(a, b):
+        c0, _ = astep(x, y, c0, sub)
+        c1, _ = astep(x, y, c1, sub)
+    return c0 | (c1 << 1)
+
+
+def model_add(x, y, sub):
+    """The three kernels of one job: tile maps, per-job scan, apply (with the thread-level
+    exclusive scan inside a tile).  Returns (limbs, swapped, length)."""
+    width = max(len(x), len(y)) + (0 if sub else 1)
+    ntiles = -(-width // ATILE)
+    lim = lambda v, i: v[i] if i < len(v) else 0
+
+    def tile_limbs(t, first, second):
+        base = t * ATILE
+        return ([lim(first, base + i) for i in range(ATILE)], [lim(second, base + i) for i in range(ATILE)])
+
+    def tile_total(a, b, s):
+        tot = AIDENT
+        for th in range(ATHREADS):
+            tot = amap_compose(athread_map(a[th * ALIMBS:(th + 1) * ALIMBS], b[th * ALIMBS:(th + 1) * ALIMBS], s), tot)
+        return tot
+
+    maps = []
+    for t in range(ntiles):
+        a, b = tile_limbs(t, x, y)
+        if not sub:
+            maps.append(tile_total(a, b, False))
+        else:
+            maps.append(tile_total(a, b, True) | (tile_total(b, a, True) << 2))
+    run0 = run1 = AIDENT
+    carry_in = []
+    for t in range(ntiles):
+        carry_in.append(amap_apply(run0, 0) | (amap_apply(run1, 0) << 1))
+        run0 = amap_compose(maps[t] & 3, run0)
+        if sub:
+            run1 = amap_compose((maps[t] >> 2) & 3, run1)
+    swap = sub and amap_apply(run0, 0) == 1
+    out = [None] * width
+    top = 0
+    for t in range(ntiles):
+        a, b = tile_limbs(t, y, x) if swap else tile_limbs(t, x, y)
+        c = (carry_in[t] >> (1 if swap else 0)) & 1
+        excl = AIDENT
+        for th in range(ATHREADS):
+            ta, tb = a[th * ALIMBS:(th + 1) * ALIMBS], b[th * ALIMBS:(th + 1) * ALIMBS]
+            cc = amap_apply(excl, c)
+            for q in range(ALIMBS):
+                cc, o = astep(ta[q], tb[q], cc, sub)
+                i = t * ATILE + th * ALIMBS + q
+                if i < width:
+                    out[i] = o
+                    if o:
+                        top = max(top, i + 1)
+            excl = amap_compose(athread_map(ta, tb, sub), excl)
+    assert None not in out
+    return out, swap, top
+
+
+@pytest.mark.parametrize("lx,ly", [(1, 1), (3, 7), (4096, 4096), (4097, 5), (2, 8200)])
+def test_device_add_model(lx, ly):
+    rng = random.Random(lx * 10007 + ly)
+    val = lambda v: sum(d << (28 * i) for i, d in enumerate(v))
+    cases = [([rng.randrange(1 << 28) for _ in range(lx)], [rng.randrange(1 << 28) for _ in range(ly)])]
+    cases.append(([MASK] * lx, [1] + [0] * (ly - 1)))              # carries through every limb
+    cases.append(([0] * (lx - 1) + [1], [1] * min(ly, lx)))       # borrows through every limb
+    same = [rng.randrange(1 << 28) for _ in range(lx)]
+    cases.append((same, list(same)))
+    for x, y in cases:
+        for sub in (False, True):
+            out, swap, n = model_add(x, y, sub)
+            want = val(x) + val(y) if not sub else abs(val(x) - val(y))
+            assert val(out) == want
+            assert swap == (sub and val(y) > val(x))
+            assert n == (want.bit_length() + 27) // 28  # 0 for zero
