@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <array>
 #include <utility>
 
@@ -889,6 +890,9 @@ constexpr auto mid_table(std::integer_sequence<int, LRs...>) {
     return std::array<KernelFn, sizeof...(LRs)>{&k_mid_pass<LRs + 4>...};
 }
 
+// Batched launches put one job per grid row; groups beyond the row limit run as several launches.
+constexpr uint32_t kMaxGridY = 65535;
+
 hipError_t launch_outer_pass(const PassArgs& a, bool inv, uint32_t njobs, uint32_t nops,
                              hipStream_t st) {
     static constexpr auto fwd = outer_table<false>(std::make_integer_sequence<int, 8>{});
@@ -898,9 +902,13 @@ hipError_t launch_outer_pass(const PassArgs& a, bool inv, uint32_t njobs, uint32
     const uint32_t ntiles = 1u << (a.log_n - a.log_r - a.log_c);
     // a first forward pass (from limbs) handles the three primes inside each block
     const bool fused = !inv && (a.flags & kFromLimbs);
-    dim3 grid(ntiles, njobs, fused ? nops : kNumPrimes * nops);
     (void)hipGetLastError();  // report this launch's error, not an earlier ignored one
-    hipLaunchKernelGGL((fused ? first : inv ? bwd : fwd)[a.log_r - 1], grid, dim3(kThreads), 0, st, a);
+    for (uint32_t j0 = 0; j0 < njobs; j0 += kMaxGridY) {  // one grid row per job
+        PassArgs b = a;
+        if (a.jobs) b.jobs = a.jobs + j0;
+        dim3 grid(ntiles, std::min(kMaxGridY, njobs - j0), fused ? nops : kNumPrimes * nops);
+        hipLaunchKernelGGL((fused ? first : inv ? bwd : fwd)[a.log_r - 1], grid, dim3(kThreads), 0, st, b);
+    }
     return hipGetLastError();
 }
 
@@ -909,9 +917,13 @@ hipError_t launch_mid_pass(const PassArgs& a, uint32_t njobs, hipStream_t st) {
     if (a.log_r < 4 || a.log_r > 12) return hipErrorInvalidValue;
     const uint32_t nblk = 1u << (a.log_n - a.log_r);
     const uint32_t threads = (1u << a.log_r) / kE;
-    dim3 grid(nblk, njobs, kNumPrimes);
     (void)hipGetLastError();  // report this launch's error, not an earlier ignored one
-    hipLaunchKernelGGL(tab[a.log_r - 4], grid, dim3(threads), 0, st, a);
+    for (uint32_t j0 = 0; j0 < njobs; j0 += kMaxGridY) {
+        PassArgs b = a;
+        if (a.jobs) b.jobs = a.jobs + j0;
+        dim3 grid(nblk, std::min(kMaxGridY, njobs - j0), kNumPrimes);
+        hipLaunchKernelGGL(tab[a.log_r - 4], grid, dim3(threads), 0, st, b);
+    }
     return hipGetLastError();
 }
 
@@ -927,10 +939,14 @@ hipError_t launch_diag_accumulate(const uint32_t* spec_a, const uint32_t* spec_b
 hipError_t launch_garner_carry(const GarnerArgs& a, uint32_t max_out_len, uint32_t njobs,
                                bool chunked, hipStream_t st) {
     const uint32_t ntiles = (max_out_len + kCarryTile - 1u) / kCarryTile;
-    dim3 grid(ntiles, njobs, 1);
     (void)hipGetLastError();  // report this launch's error, not an earlier ignored one
-    hipLaunchKernelGGL(chunked ? k_garner_carry<true> : k_garner_carry<false>, grid, dim3(kThreads), 0,
-                       st, a);
+    for (uint32_t j0 = 0; j0 < njobs; j0 += kMaxGridY) {
+        GarnerArgs b = a;
+        if (a.jobs) b.jobs = a.jobs + j0;
+        dim3 grid(ntiles, std::min(kMaxGridY, njobs - j0), 1);
+        hipLaunchKernelGGL(chunked ? k_garner_carry<true> : k_garner_carry<false>, grid, dim3(kThreads),
+                           0, st, b);
+    }
     return hipGetLastError();
 }
 

@@ -32,6 +32,9 @@ using clk = std::chrono::steady_clock;
 #ifndef PI_DEC_PIECE_LIMBS
 #define PI_DEC_PIECE_LIMBS 256
 #endif
+#ifndef PI_DEVICE_MIN_R_LIMBS
+#define PI_DEVICE_MIN_R_LIMBS 0
+#endif
 
 struct Term {
     BigInt P, Q, R;
@@ -216,9 +219,9 @@ static std::vector<Term> leaves(size_type r) {
 // environment, one line per level goes to stderr (merges, largest Q in limbs, product and
 // merge-addition milliseconds).
 //
-// Host levels run until every product of the next level is above the host schoolbook's range;
-// from there the tree stays in the product backend's memory (DeviceInt): each level is one
-// product batch and one batch of signed additions, and only P(1, r) and Q(1, r) come back.
+// After the block merge the tree stays in the product backend's memory (DeviceInt): each level
+// is one product batch and one batch of signed additions, and only P(1, r) and Q(1, r) come
+// back.  Host levels run before that only while R is below PI_DEVICE_MIN_R_LIMBS.
 static std::pair<BigInt, BigInt> split(size_type r, int plan_mode, double& gpu_levels_ms) {
     const bool verbose = std::getenv("PI_LEVELS") != nullptr;
     const auto tl = clk::now();
@@ -227,10 +230,13 @@ static std::pair<BigInt, BigInt> split(size_type r, int plan_mode, double& gpu_l
         std::cerr << "leaves: " << r << " terms in blocks of " << kBlock << ", ms "
                   << std::chrono::duration<double, std::milli>(clk::now() - tl).count() << "\n";
     int level = std::countr_zero(kBlock);  // the block merge did the levels below
-    // R R' is a level's smallest product; at >= 320 limbs per factor it is above 16,384 bits
+    // The tree moves to the backend once every R has PI_DEVICE_MIN_R_LIMBS limbs: on the engine
+    // at once (its batches beat host schoolbook on these levels); a backend without batching
+    // (the CPU baseline build) waits until R R', a level's smallest product, is above the host
+    // schoolbook's 16,384 bits (320 limbs per factor).
     auto all_big = [&] {
         for (const Term& t : cur)
-            if (t.R.get_data().size() < 320) return false;
+            if (t.R.get_data().size() < PI_DEVICE_MIN_R_LIMBS) return false;
         return true;
     };
     while (cur.size() > 1 && !all_big()) {

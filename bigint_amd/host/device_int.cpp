@@ -2,11 +2,26 @@
 #include "bigint/device.h"
 
 #include <algorithm>
+#include <chrono>
 #include <span>
+
+#include "bigint/mul.h"
 
 #include "mul_backend.h"
 
 namespace bigint {
+namespace {
+
+// Runs f and adds its wall time to the engine time (bigint::mul::engine_time_ms).
+template <class F>
+void timed(F&& f) {
+    const auto t0 = std::chrono::steady_clock::now();
+    f();
+    mul::add_engine_time_ns((uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                                std::chrono::steady_clock::now() - t0).count());
+}
+
+}  // namespace
 
 auto DeviceInt::upload(const std::vector<const BigInt*>& xs) -> std::vector<DeviceInt> {
     std::vector<DeviceInt> res(xs.size());
@@ -28,7 +43,7 @@ auto DeviceInt::upload(const std::vector<const BigInt*>& xs) -> std::vector<Devi
         res[i].neg_ = xs[i]->sign() < 0;
         off += d.size();
     }
-    mul::backend::dev_upload(base, src);
+    timed([&] { mul::backend::dev_upload(base, src); });
     return res;
 }
 
@@ -38,7 +53,7 @@ auto DeviceInt::download(const std::vector<const DeviceInt*>& xs) -> std::vector
     for (size_t i = 0; i < xs.size(); ++i) digits[i].resize(xs[i]->len_);
     for (size_t i = 0; i < xs.size(); ++i) {
         if (!xs[i]->len_) continue;
-        mul::backend::dev_download({std::span<uint32_t>(digits[i].data(), digits[i].size())}, xs[i]->limbs_);
+        timed([&] { mul::backend::dev_download({std::span<uint32_t>(digits[i].data(), digits[i].size())}, xs[i]->limbs_); });
     }
     std::vector<BigInt> res;
     res.reserve(xs.size());
@@ -70,7 +85,7 @@ auto DeviceInt::mul_batch(const std::vector<std::pair<const DeviceInt*, const De
         res[i].neg_ = a.neg_ != b.neg_;
         off += a.len_ + b.len_;
     }
-    mul::backend::dev_mul_batch(jobs, plan_mode);
+    timed([&] { mul::backend::dev_mul_batch(jobs, plan_mode); });
     return res;
 }
 
@@ -98,7 +113,7 @@ auto DeviceInt::add_batch(const std::vector<std::pair<const DeviceInt*, const De
         res[i].limbs_ = base + off;
         off += std::max(a.len_, b.len_) + 1;
     }
-    mul::backend::dev_add_batch(jobs);
+    timed([&] { mul::backend::dev_add_batch(jobs); });
     for (size_t k = 0; k < jobs.size(); ++k) {
         DeviceInt& r = res[at[k]];
         const DeviceInt& a = *pairs[at[k]].first;

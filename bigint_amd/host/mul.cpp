@@ -74,6 +74,7 @@ auto schoolbook(std::span<const uint32_t> a, std::span<const uint32_t> b) -> Dig
 auto gpu_product_count() noexcept -> uint64_t { return g_gpu_products.load(); }
 auto host_product_count() noexcept -> uint64_t { return g_host_products.load(); }
 auto engine_time_ms() noexcept -> double { return (double)g_backend_ns.load() * 1e-6; }
+void add_engine_time_ns(uint64_t ns) noexcept { g_backend_ns.fetch_add(ns, std::memory_order_relaxed); }
 
 auto load_planner(const char* path) -> bool { return backend::load_planner(path); }
 

@@ -42,6 +42,8 @@ auto gpu_product_count() noexcept -> uint64_t;
 auto host_product_count() noexcept -> uint64_t;
 // Wall time spent inside the product backend (engine calls, including staging copies).
 auto engine_time_ms() noexcept -> double;
+// adds to engine_time_ms (the DeviceInt batches report their backend time here)
+void add_engine_time_ns(uint64_t ns) noexcept;
 
 // Install a tuned plan table and cost model (the binary file bigint_amd/planner.py writes) for
 // the engine's plan modes 1 (table) and 2 (learned); false if the backend has no planner or the
