@@ -460,6 +460,68 @@ __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, 
     global_io<LR, 0, S::stages(G0), S::trlo(G0), true>(pa, kPlane, xa, tile_base, 0);
 }
 
+// Block pass of a plain product or square (flags 0 or kSquare) over two adjacent tiles per
+// block: the second tile's A residues are loaded before the first tile is computed, so their
+// latency is hidden behind the first tile's work (its B residues, needed later, load then).
+template <int I, int LR>
+__device__ __forceinline__ void mid_pair_body(const PassArgs& a, const PassJob& job, uint32_t* lds,
+                                              uint32_t* tw) {
+    using S = Sched<LR>;
+    constexpr uint32_t T = 1u << (LR - 4);
+    constexpr int G0 = 0;
+    constexpr LimbSrc kPlane{nullptr, 0u, 0u, 0u};
+    const uint32_t n = 1u << a.log_n;
+    const bool square = (a.flags & kSquare) != 0;
+    uint32_t* pa = job.res_a + (size_t)I * n;
+    uint32_t* pb = job.res_b + (size_t)I * n;
+    const uint32_t* wf = a.wtab + I * kTile;
+    const uint32_t* wi = a.wtab + (kNumPrimes + I) * kTile;
+    const uint32_t blk0 = 2u * blockIdx.x;
+    uint32_t xa[kE], xb[kE], na[kE];
+    global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pa, kPlane, xa, blk0 << LR, 0);
+    if (!square) global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pb, kPlane, xb, blk0 << LR, 0);
+    global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pa, kPlane, na, (blk0 + 1u) << LR, 0);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const uint32_t blk = blk0 + (uint32_t)t;
+        if (t == 1) {
+#pragma unroll
+            for (int k = 0; k < kE; ++k) xa[k] = na[k];
+            if (!square) global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pb, kPlane, xb, blk << LR, 0);
+            __syncthreads();  // the first tile is done with tw and lds
+        }
+        build_twiddles<I, LR>(tw, wf, block_pows<I>(a.beta, a.beta_stride, blk), T);
+        __syncthreads();
+        tile_transform<I, false, LR, 0>(xa, lds, tw);
+        if (!square) {
+            tile_transform<I, false, LR, 0>(xb, lds, tw);
+#pragma unroll
+            for (int k = 0; k < kE; ++k) xa[k] = pointwise<I>(xa[k], xb[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kE; ++k) xa[k] = pointwise<I>(xa[k], xa[k]);
+        }
+        __syncthreads();
+        build_twiddles<I, LR>(tw, wi, block_pows<I>(a.beta_inv, a.beta_stride, blk), T);
+        __syncthreads();
+        tile_transform<I, true, LR, 0>(xa, lds, tw);
+        global_io<LR, 0, S::stages(G0), S::trlo(G0), true>(pa, kPlane, xa, blk << LR, 0);
+    }
+}
+
+// grid.x = n >> (LR + 1) tile pairs, grid.y = job, grid.z = prime.
+template <int LR>
+__global__ __launch_bounds__(1 << (LR - 4)) void k_mid_pair(PassArgs a) {
+    __shared__ uint32_t lds[(1 << LR) + (1 << LR) / 16];
+    __shared__ uint32_t tw[1 << LR];
+    const PassJob job = a.jobs ? load_desc(a.jobs + blockIdx.y) : a.job;
+    switch (blockIdx.z) {
+        case 0: mid_pair_body<0, LR>(a, job, lds, tw); break;
+        case 1: mid_pair_body<1, LR>(a, job, lds, tw); break;
+        default: mid_pair_body<2, LR>(a, job, lds, tw); break;
+    }
+}
+
 template <int LR>
 __global__ __launch_bounds__(1 << (LR - 4)) void k_mid_pass(PassArgs a) {
     __shared__ uint32_t lds[(1 << LR) + (1 << LR) / 16];
@@ -889,6 +951,10 @@ template <int... LRs>
 constexpr auto mid_table(std::integer_sequence<int, LRs...>) {
     return std::array<KernelFn, sizeof...(LRs)>{&k_mid_pass<LRs + 4>...};
 }
+template <int... LRs>
+constexpr auto pair_table(std::integer_sequence<int, LRs...>) {
+    return std::array<KernelFn, sizeof...(LRs)>{&k_mid_pair<LRs + 4>...};
+}
 
 // Batched launches put one job per grid row; groups beyond the row limit run as several launches.
 constexpr uint32_t kMaxGridY = 65535;
@@ -914,6 +980,7 @@ hipError_t launch_outer_pass(const PassArgs& a, bool inv, uint32_t njobs, uint32
 
 hipError_t launch_mid_pass(const PassArgs& a, uint32_t njobs, hipStream_t st) {
     static constexpr auto tab = mid_table(std::make_integer_sequence<int, 9>{});
+    static constexpr auto ptab = pair_table(std::make_integer_sequence<int, 9>{});
     if (a.log_r < 4 || a.log_r > 12) return hipErrorInvalidValue;
     const uint32_t nblk = 1u << (a.log_n - a.log_r);
     const uint32_t threads = (1u << a.log_r) / kE;
@@ -921,8 +988,12 @@ hipError_t launch_mid_pass(const PassArgs& a, uint32_t njobs, hipStream_t st) {
     for (uint32_t j0 = 0; j0 < njobs; j0 += kMaxGridY) {
         PassArgs b = a;
         if (a.jobs) b.jobs = a.jobs + j0;
-        dim3 grid(nblk, std::min(kMaxGridY, njobs - j0), kNumPrimes);
-        hipLaunchKernelGGL(tab[a.log_r - 4], grid, dim3(threads), 0, st, b);
+        // plain products and squares: two tiles per block when the launch still has at least
+        // 256 tile pairs per prime (small launches keep one tile per block for parallelism)
+        const uint32_t nj = std::min(kMaxGridY, njobs - j0);
+        const bool pair = (a.flags & ~kSquare) == 0 && nblk > 1 && (nblk / 2u) * nj >= 256u;
+        dim3 grid(pair ? nblk / 2 : nblk, nj, kNumPrimes);
+        hipLaunchKernelGGL((pair ? ptab : tab)[a.log_r - 4], grid, dim3(threads), 0, st, b);
     }
     return hipGetLastError();
 }
