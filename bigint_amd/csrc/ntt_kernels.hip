@@ -461,8 +461,9 @@ __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, 
 }
 
 // Block pass of a plain product or square (flags 0 or kSquare) over two adjacent tiles per
-// block: the second tile's A residues are loaded before the first tile is computed, so their
-// latency is hidden behind the first tile's work (its B residues, needed later, load then).
+// block: the second tile's A residues are loaded before the first tile is computed and its B
+// residues into the first tile's B registers once the pointwise product has consumed them, so
+// both loads are in flight while the first tile is transformed.
 template <int I, int LR>
 __device__ __forceinline__ void mid_pair_body(const PassArgs& a, const PassJob& job, uint32_t* lds,
                                               uint32_t* tw) {
@@ -487,7 +488,6 @@ __device__ __forceinline__ void mid_pair_body(const PassArgs& a, const PassJob& 
         if (t == 1) {
 #pragma unroll
             for (int k = 0; k < kE; ++k) xa[k] = na[k];
-            if (!square) global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pb, kPlane, xb, blk << LR, 0);
             __syncthreads();  // the first tile is done with tw and lds
         }
         build_twiddles<I, LR>(tw, wf, block_pows<I>(a.beta, a.beta_stride, blk), T);
@@ -497,6 +497,8 @@ __device__ __forceinline__ void mid_pair_body(const PassArgs& a, const PassJob& 
             tile_transform<I, false, LR, 0>(xb, lds, tw);
 #pragma unroll
             for (int k = 0; k < kE; ++k) xa[k] = pointwise<I>(xa[k], xb[k]);
+            if (t == 0)  // the second tile's B, into the registers just freed
+                global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pb, kPlane, xb, (blk0 + 1u) << LR, 0);
         } else {
 #pragma unroll
             for (int k = 0; k < kE; ++k) xa[k] = pointwise<I>(xa[k], xa[k]);
