@@ -53,7 +53,12 @@ void check(int rc) {
 auto dev_alloc(size_t words) -> std::shared_ptr<void> {
     void* p = nullptr;
     check(bi_hip_device_alloc(words * 4, &p));
-    return std::shared_ptr<void>(p, [](void* q) { (void)bi_hip_device_free(q); });
+    // product batches run asynchronously on the engine's stream: wait for them before a buffer
+    // they may still read or write goes back
+    return std::shared_ptr<void>(p, [](void* q) {
+        (void)bi_hip_synchronize();
+        (void)bi_hip_device_free(q);
+    });
 }
 
 void dev_upload(uint32_t* dst, const std::vector<std::span<const uint32_t>>& src) {
