@@ -202,6 +202,23 @@ def test_device_add_batch_against_python_ints():
         assert sw == swapped
 
 
+def test_batch_beyond_one_launch_row_limit():
+    """A grouped batch of 70,000 products (one transform length, so one group): its launches put
+    one job per grid row and split past 65,535 rows.  Every product is checked modulo two primes,
+    a sample limb for limb against the oracle."""
+    rng = np.random.default_rng(41)
+    count = 70000
+    a = rng.integers(1, 1 << 28, size=(count, 60), dtype=np.uint32)
+    b = rng.integers(1, 1 << 28, size=(count, 50), dtype=np.uint32)
+    pairs = [(a[i], b[i]) for i in range(count)]
+    outs = engine.mul_batch(pairs, engine.PLAN_TABLE)
+    for q in L.RESIDUE_PRIMES[:2]:
+        for i in range(count):
+            assert L.residue_mod(outs[i], q) == L.residue_mod(a[i], q) * L.residue_mod(b[i], q) % q, i
+    for i in [0, 1, 65534, 65535, 65536, count - 1]:
+        assert np.array_equal(outs[i], oracle.mul_limbs(a[i], b[i])), i
+
+
 def test_batch_matches_single():
     rng = random.Random(4)
     pairs = []
