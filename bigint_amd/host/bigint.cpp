@@ -168,11 +168,12 @@ void BigInt::print(std::ostream& output, bool hex, bool direct) const {
 
 // Same digits as the reference's recursive split (src/bigint.cpp:126-265): x = q 10^k + r with
 // k half of x's decimal length, down to pieces below DEC_STRING_BRUTE_THRESHOLDS[1] limbs.
-// The decimal string of a number is unique, so four things here only change the speed:
+// The decimal string of a number is unique, so five things here only change the speed:
 //   - the split tree is walked one depth at a time: the quotient estimates q = floor(x / 10^k) of
 //     all nodes of a depth are one product batch (x times a reciprocal of 10^k), and so are the
 //     products q 10^k; the estimates are then corrected until exact;
-//   - 10^k and its reciprocal are computed once per distinct k (a depth has one or two);
+//   - 10^k and its reciprocal are computed once per distinct k (a depth has one or two), the
+//     reciprocal only as precise as the quotient needs (about half of x's limbs);
 //   - subtractions, corrections and the brute-force conversions of the pieces run on the
 //     init_thread_pool threads;
 //   - digits are written in blocks.
@@ -217,8 +218,13 @@ void BigInt::print_dec(std::ostream& output, size_type len) const {
             const BigInt& x = pieces[at[j]].first;
             const size_type est = (size_type)std::ceil((double)x.data_.size() * DIGIT_BITS * std::log10(2.0));
             ks[j] = est / 2;
+            // q = floor(x / 10^k) has about |x| - |10^k| limbs; a reciprocal with relative error
+            // below 2^(-28 p) puts the estimate x * recip within 1 of x / 10^k once p exceeds
+            // |q| + 1 limbs, so p = |x| - (limbs of 10^k, rounded down) + 8 leaves a margin
+            const size_type dlimbs = (size_type)((double)ks[j] * std::log2(10.0) / DIGIT_BITS);
+            const size_type prec = x.data_.size() + 8 - std::min(dlimbs, x.data_.size());
             size_type& p = need[ks[j]];
-            p = std::max(p, x.data_.size() + 16);
+            p = std::max(p, prec);
         }
         for (const auto& [k, prec] : need) {
             auto it = divs.find(k);
