@@ -502,10 +502,13 @@ public:
             const int rc = choose_plans(&p, 1, BI_HIP_PLAN_TABLE, nullptr, chosen, st);
             if (rc) return rc;
         }
-        if (planned && (chosen[0].chunk_len || chosen[0].mid_lr < std::min<uint32_t>(12, (uint32_t)lg0)))
+        // chunked and re-split plans need the batch path's shared buffers; an unchunked plan with
+        // any block-pass size launches directly (kernel arguments by value: no descriptor upload,
+        // no wait for earlier work on the stream)
+        if (planned && (chosen[0].chunk_len || chosen[0].split_both || chosen[0].lg != (uint32_t)lg0))
             return run_batch(&p, 1, BI_HIP_PLAN_TABLE, nullptr, st);
         const uint32_t lg = (uint32_t)lg0;
-        Plan* pl = plan(lg);
+        Plan* pl = planned ? plan(lg, chosen[0].mid_lr) : plan(lg);
         if (!pl) return BI_HIP_ENOMEM;
         const uint32_t n = 1u << lg;
         const uint32_t out_len = la + lb;
