@@ -203,6 +203,7 @@ void BigInt::print_dec(std::ostream& output, size_type len) const {
         BigInt recip;  // digits of a reciprocal of 10^k: 1/10^k ~ recip * 2^(-28 shift)
         int64_t shift;
         size_type prec;
+        BigFloat rc;   // the reciprocal itself
     };
     std::map<size_type, Div> divs;
     std::vector<std::pair<BigInt, size_type>> pieces;  // in output order: (value, width)
@@ -230,11 +231,36 @@ void BigInt::print_dec(std::ostream& output, size_type len) const {
             auto it = divs.find(k);
             if (it != divs.end() && it->second.prec >= prec + 2) continue;
             BigInt d = it == divs.end() ? get_pow_of_ten((uint32_t)k) : std::move(it->second.d);
-            const BigFloat rc = BigFloat(d).reciprocal(prec + 2);
+            // a reciprocal of a larger power at this precision or better, from a shallower depth,
+            // gives this one by one product: 1/10^k = 1/10^k2 * 10^(k2 - k) (relative error of the
+            // parent's, plus the truncation to prec + 2 limbs); else Newton's iteration
+            const Div* parent = nullptr;
+            size_type k2 = 0;
+            for (auto up = divs.upper_bound(k); up != divs.end(); ++up)
+                if (up->second.prec >= prec + 2) {
+                    parent = &up->second;
+                    k2 = up->first;
+                    break;
+                }
+            BigFloat rc;
+            if (parent) {
+                // 10^(k2 - k) from 10^k: k2 - k is within a few units of k (a child holds about
+                // half of its parent's digits)
+                const int64_t dm = (int64_t)(k2 - k) - (int64_t)k;
+                uint64_t p10 = 1;
+                for (int64_t i = 0; i < (dm < 0 ? -dm : dm) && i < 19; ++i) p10 *= 10u;
+                BigInt pw = dm >= 0 && dm <= 18 ? d * p10
+                            : dm < 0 && dm >= -18 ? d / p10
+                                                  : get_pow_of_ten((uint32_t)(k2 - k));
+                rc = BigFloat::mul(parent->rc, BigFloat(std::move(pw)), prec + 2);
+            } else {
+                rc = BigFloat(d).reciprocal(prec + 2);
+            }
             BigInt r;
             r.data_ = rc.get_data();
             r.remove_leading_zero();
-            divs[k] = Div{std::move(d), std::move(r), rc.get_point_pos(), prec + 2};
+            const int64_t shift = rc.get_point_pos();
+            divs[k] = Div{std::move(d), std::move(r), shift, prec + 2, std::move(rc)};
         }
         // q = floor(x * recip * 2^(-28 shift)): within a few units of floor(x / 10^k)
         std::vector<std::pair<const BigInt*, const BigInt*>> jobs(at.size());
