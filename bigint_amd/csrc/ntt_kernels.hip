@@ -635,6 +635,11 @@ __device__ __forceinline__ Pieces garner(const GarnerJob& jb, const GarnerConsts
     return Pieces{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)(x >> 64)};
 }
 
+// acc + x mod P_p for a residue x of prime p's inverse output (P0 outputs lie in [0, 2 P0)).
+__device__ __forceinline__ uint32_t add_residue(uint32_t p, uint32_t acc, uint32_t x) {
+    return p == 0u ? add_mod<0>(acc, fold1p(x)) : (p == 1u ? add_mod<1>(acc, x) : add_mod<2>(acc, x));
+}
+
 constexpr uint32_t kCarryLimbsPerThread = 16;
 constexpr uint32_t kCarryWordsPerThread = 14;  // 14 * 32 = 16 * 28 bits
 constexpr uint32_t kCarryTile = kThreads * kCarryLimbsPerThread;  // 4096 limbs per tile
@@ -680,77 +685,135 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     // 1. pieces of coefficients j0-2 .. j0+13: the two before the thread's words feed its
     //    first two sums (recomputed here rather than exchanged).
     constexpr uint32_t NQ = kCarryWordsPerThread + 2;
-    uint32_t w0[NQ], w1[NQ], w2[NQ];
+    // word sums s_q = w0_{q+2} + w1_{q+1} + w2_q < 3 * 2^32 of the thread's 14 words (pieces
+    // indexed from word j0 - 2), and w0_1, w1_0 for the tile's local carry
+    uint32_t s_lo[kCarryWordsPerThread], s_hi[kCarryWordsPerThread];
+    uint32_t w0_1 = 0u, w1_0 = 0u;
     const uint32_t n = jb.n;
     const bool wrap_here = jb.wrap_a && j0 - 2u <= n && n < j0 + kCarryWordsPerThread;
-    if (j0 >= 2u && !wrap_here) {
-        // all residues first (16 per plane, as 8 aligned pairs), then the CRT: many loads in
-        // flight per thread.  j0 - 2, n and chunk lengths are even, so pairs never straddle a
-        // transform's end.
-        uint32_t r0[NQ], r1[NQ], r2[NQ];
-        const uint32_t jl = j0 - 2u;
-        bool any;  // some coefficient of the run lies inside a transform
+    const bool fast = j0 >= 2u && !wrap_here;
+    // residues of the thread's run, words j0 - 2 .. j0 + 13 of each plane.  j0 - 2, n and chunk
+    // lengths are even, so aligned pairs never straddle a transform's end.
+    uint32_t r0[NQ], r1[NQ], r2[NQ];
+    // residues of word base - 3, for the tile's local carry below
+    uint32_t h3[3] = {0u, 0u, 0u};
+    // The block's words [base - 4, base + kWordTile) of each plane are read coalesced (lane k of
+    // a wave reads pair k) and staged in LDS one plane at a time, where each thread picks up its
+    // run of 16; read straight from memory, the runs 56 bytes apart would spread every wave's
+    // load over 28 cache lines (CRT+carry at 2^24 points: 112 -> 97 us).  Words outside every
+    // transform read as zero; a chunked product's chunk residues are summed on the way.  The
+    // output limbs go through the same LDS array.
+    constexpr uint32_t kStage = kWordTile + 4u;  // 4 halo words + tile
+    constexpr uint32_t kResWords = kStage > kCarryTile ? kStage : kCarryTile;
+    __shared__ __align__(16) uint32_t s_res[kResWords];
+    {
+        constexpr uint32_t kPairs = kWordTile / (2u * kThreads);
+        static_assert(kPairs * 2u * kThreads == kWordTile, "pairs per thread");
+        u32x2 v[3][kPairs], h[2] = {{0u, 0u}, {0u, 0u}};
         if constexpr (!CHUNKED) {
-            any = jl < n;
 #pragma unroll
-            for (uint32_t i = 0; i < NQ; i += 2) {
-                if (jl + i < n) {
-                    const u32x2 v0 = *gmem(reinterpret_cast<const u32x2*>(jb.y + jl + i));
-                    const u32x2 v1 = *gmem(reinterpret_cast<const u32x2*>(jb.y + n + jl + i));
-                    const u32x2 v2 = *gmem(reinterpret_cast<const u32x2*>(jb.y + 2u * n + jl + i));
-                    r0[i] = v0.x; r0[i + 1] = v0.y;
-                    r1[i] = v1.x; r1[i + 1] = v1.y;
-                    r2[i] = v2.x; r2[i + 1] = v2.y;
-                } else {
-                    r0[i] = r0[i + 1] = r1[i] = r1[i + 1] = r2[i] = r2[i + 1] = 0u;
+            for (uint32_t p = 0; p < 3u; ++p) {
+                const uint32_t* yp = jb.y + (size_t)p * n;
+#pragma unroll
+                for (uint32_t k = 0; k < kPairs; ++k) {
+                    const uint32_t w = base + 2u * (k * kThreads + tid);
+                    v[p][k] = w < n ? *gmem(reinterpret_cast<const u32x2*>(yp + w)) : u32x2{0u, 0u};
                 }
             }
+            if (tid < 3u && base >= 4u) {  // thread p loads plane p's halo
+                const uint32_t* yp = jb.y + (size_t)tid * n;
+                if (base - 4u < n) h[0] = *gmem(reinterpret_cast<const u32x2*>(yp + base - 4u));
+                if (base - 2u < n) h[1] = *gmem(reinterpret_cast<const u32x2*>(yp + base - 2u));
+            }
         } else {
-            // chunk cc covers words [cc L, cc L + n); the chunks reaching the run are
-            // (jl - n) / L < cc <= (jl + NQ - 1) / L.  Their residues add mod each prime.
+            // chunk cc covers words [cc L, cc L + n); those reaching [lo, hi] are
+            // (lo - n) / L < cc <= hi / L
+#pragma unroll
+            for (uint32_t p = 0; p < 3u; ++p)
+#pragma unroll
+                for (uint32_t k = 0; k < kPairs; ++k) v[p][k] = u32x2{0u, 0u};
             const uint32_t L = jb.chunk_len;
             const bool p2 = jb.log_chunk < 32u;
-            const uint32_t top = jl + NQ - 1u;
-            uint32_t c_hi = p2 ? (top >> jb.log_chunk) : top / L;
+            const uint32_t lo = base >= 4u ? base - 4u : 0u, hi = base + kWordTile - 1u;
+            uint32_t c_hi = p2 ? (hi >> jb.log_chunk) : hi / L;
             c_hi = c_hi < jb.nchunks - 1u ? c_hi : jb.nchunks - 1u;
-            const uint32_t c_lo = jl >= n ? (p2 ? ((jl - n) >> jb.log_chunk) : (jl - n) / L) + 1u : 0u;
-            any = c_lo <= c_hi;
-#pragma unroll
-            for (uint32_t i = 0; i < NQ; ++i) r0[i] = r1[i] = r2[i] = 0u;
+            const uint32_t c_lo = lo >= n ? (p2 ? ((lo - n) >> jb.log_chunk) : (lo - n) / L) + 1u : 0u;
             for (uint32_t cc = c_lo; cc <= c_hi; ++cc) {
-                const uint32_t* y = jb.y + (size_t)cc * jb.chunk_stride;
-                const int32_t off0 = (int32_t)jl - (int32_t)(cc * L);
+                const uint32_t* yc = jb.y + (size_t)cc * jb.chunk_stride;
+                const uint32_t c0 = cc * L;  // first word of chunk cc
 #pragma unroll
-                for (uint32_t i = 0; i < NQ; i += 2) {
-                    const int32_t off = off0 + (int32_t)i;
-                    if (off >= 0 && off < (int32_t)n) {
-                        const u32x2 v0 = *gmem(reinterpret_cast<const u32x2*>(y + off));
-                        const u32x2 v1 = *gmem(reinterpret_cast<const u32x2*>(y + n + off));
-                        const u32x2 v2 = *gmem(reinterpret_cast<const u32x2*>(y + 2u * n + off));
-                        r0[i] = add_mod<0>(r0[i], fold1p(v0.x));  // P0 outputs lie in [0, 2P)
-                        r0[i + 1] = add_mod<0>(r0[i + 1], fold1p(v0.y));
-                        r1[i] = add_mod<1>(r1[i], v1.x);
-                        r1[i + 1] = add_mod<1>(r1[i + 1], v1.y);
-                        r2[i] = add_mod<2>(r2[i], v2.x);
-                        r2[i + 1] = add_mod<2>(r2[i + 1], v2.y);
+                for (uint32_t p = 0; p < 3u; ++p) {
+#pragma unroll
+                    for (uint32_t k = 0; k < kPairs; ++k) {
+                        const uint32_t w = base + 2u * (k * kThreads + tid);
+                        if (w >= c0 && w - c0 < n) {
+                            const u32x2 x = *gmem(reinterpret_cast<const u32x2*>(yc + (size_t)p * n + (w - c0)));
+                            v[p][k] = u32x2{add_residue(p, v[p][k].x, x.x), add_residue(p, v[p][k].y, x.y)};
+                        }
+                    }
+                }
+                if (tid < 3u && base >= 4u) {
+#pragma unroll
+                    for (uint32_t e = 0; e < 2u; ++e) {
+                        const uint32_t w = base - 4u + 2u * e;
+                        if (w >= c0 && w - c0 < n) {
+                            const u32x2 x = *gmem(reinterpret_cast<const u32x2*>(yc + (size_t)tid * n + (w - c0)));
+                            h[e] = u32x2{add_residue(tid, h[e].x, x.x), add_residue(tid, h[e].y, x.y)};
+                        }
                     }
                 }
             }
         }
+        const uint32_t o = tid * kCarryWordsPerThread + 2u;  // staged index of word j0 - 2
+#pragma unroll
+        for (uint32_t p = 0; p < 3u; ++p) {
+            if (p > 0u) __syncthreads();  // the previous plane has been read
+#pragma unroll
+            for (uint32_t k = 0; k < kPairs; ++k)
+                *reinterpret_cast<u32x2*>(s_res + 4u + 2u * (k * kThreads + tid)) = v[p][k];
+            if (tid == p) {
+                *reinterpret_cast<u32x2*>(s_res) = h[0];
+                *reinterpret_cast<u32x2*>(s_res + 2u) = h[1];
+            }
+            __syncthreads();
+            if (tid == 0u) h3[p] = s_res[1];
+            if (fast) {
+                uint32_t* r = p == 0u ? r0 : (p == 1u ? r1 : r2);
+#pragma unroll
+                for (uint32_t i = 0; i < NQ; i += 2) {
+                    const u32x2 x = *reinterpret_cast<const u32x2*>(s_res + o + i);
+                    r[i] = x.x;
+                    r[i + 1] = x.y;
+                }
+            }
+        }
+    }
+    if (fast) {
+        const uint32_t jl = j0 - 2u;
+        uint64_t acc2 = 0, acc1 = 0;
 #pragma unroll
         for (uint32_t i = 0; i < NQ; ++i) {
             // coefficients outside every transform are zero: no CRT (whole threads skip it in
             // the tiles of short products)
-            const bool in = CHUNKED ? any : jl + i < n;
+            const bool in = CHUNKED || jl + i < n;
             const Pieces p = in ? split_coeff(r0[i], r1[i], r2[i], jb, a.k) : Pieces{0u, 0u, 0u};
-            w0[i] = p.w0;
-            w1[i] = p.w1;
-            w2[i] = p.w2;
+            // summed as they come (acc2, acc1: partial sums of words i - 2 and i - 1), so the
+            // pieces of the run are never all live at once
+            if (i == 0) w1_0 = p.w1;
+            if (i == 1) w0_1 = p.w0;
+            if (i >= 2) {
+                const uint64_t sum = acc2 + p.w0;
+                s_lo[i - 2] = (uint32_t)sum;
+                s_hi[i - 2] = (uint32_t)(sum >> 32);
+            }
+            acc2 = acc1 + p.w1;
+            acc1 = p.w2;
         }
     } else {
         // the wrapped coefficient and the product's first two words
         const uint32_t tile_chunk =
             (jb.nchunks > 1 && jb.log_chunk == 0xFFu) ? base / jb.chunk_len : 0u;
+        uint32_t w0[NQ], w1[NQ], w2[NQ];
 #pragma unroll 2
         for (uint32_t i = 0; i < NQ; ++i) {
             Pieces p{0u, 0u, 0u};
@@ -766,6 +829,14 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
             w1[i] = p.w1;
             w2[i] = p.w2;
         }
+#pragma unroll
+        for (uint32_t q = 0; q < kCarryWordsPerThread; ++q) {
+            const uint64_t sum = (uint64_t)w0[q + 2] + w1[q + 1] + w2[q];
+            s_lo[q] = (uint32_t)sum;
+            s_hi[q] = (uint32_t)(sum >> 32);
+        }
+        w0_1 = w0[1];
+        w1_0 = w1[0];
     }
 
     // The carry into the tile, resolved locally when possible: the previous tile's last word has
@@ -775,21 +846,18 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     uint32_t local = 0xFFu;
     if (tid == 0 && tile != 0) {
         const uint32_t j = base - 3u;
-        const uint32_t c = (jb.nchunks > 1 && jb.log_chunk == 0xFFu) ? j / jb.chunk_len : 0u;
-        const Pieces p3 = garner(jb, a.k, j, c);
-        const uint64_t sl = (uint64_t)w0[1] + w1[0] + p3.w2;
+        Pieces p3;
+        if (!(jb.wrap_a && j == n)) {
+            p3 = split_coeff(h3[0], h3[1], h3[2], jb, a.k);  // staged (zero past the transform)
+        } else {
+            const uint32_t c = (jb.nchunks > 1 && jb.log_chunk == 0xFFu) ? j / jb.chunk_len : 0u;
+            p3 = garner(jb, a.k, j, c);
+        }
+        const uint64_t sl = (uint64_t)w0_1 + w1_0 + p3.w2;
         if ((uint32_t)sl <= 0xFFFFFFFDu) local = (uint32_t)(sl >> 32);
     }
 
-    // 2. word sums s_i = w0_i + w1_{i-1} + w2_{i-2} < 3 * 2^32, 14 consecutive words per thread
-    uint32_t s_lo[kCarryWordsPerThread], s_hi[kCarryWordsPerThread];
-#pragma unroll
-    for (uint32_t q = 0; q < kCarryWordsPerThread; ++q) {
-        const uint64_t sum = (uint64_t)w0[q + 2] + w1[q + 1] + w2[q];
-        s_lo[q] = (uint32_t)sum;
-        s_hi[q] = (uint32_t)(sum >> 32);
-    }
-    // 3. this thread's carry map (carry-out for each carry-in 0..3, four chains run side by
+    // 2. this thread's carry map (carry-out for each carry-in 0..3, four chains run side by
     //    side), then an inclusive scan of maps across the block
     uint32_t m;
     {
@@ -819,7 +887,7 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     if (lane == 0) excl = kMapIdentity;
     excl = map_compose(excl, wave_prefix);
 
-    // 4. carry into the tile: resolved locally, or by a decoupled look-back of the first wave
+    // 3. carry into the tile: resolved locally, or by a decoupled look-back of the first wave
     //    (64 predecessor tiles read at once).  Every tile publishes its status either way.
     if (wave == 0) {
         uint32_t agg = kMapIdentity;
@@ -869,7 +937,7 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     }
     __syncthreads();
 
-    // 5. apply the carries, cut the 14 words into 16 limbs and write them
+    // 4. apply the carries, cut the 14 words into 16 limbs and write them
     uint32_t carry = map_apply(excl, s_carry_in);
     uint32_t wv[kCarryWordsPerThread];
 #pragma unroll
@@ -886,16 +954,34 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
         if (bo > 4u) v |= wv[wi + 1] << (32u - bo);
         outv[q] = v & kDigitMask;
     }
-    const uint32_t i0 = tile * kCarryTile + tid * kCarryLimbsPerThread;
-    if (i0 + kCarryLimbsPerThread <= jb.out_len && ((reinterpret_cast<uintptr_t>(jb.out + i0) & 15u) == 0)) {
-        auto dst = gmem(reinterpret_cast<u32x4*>(jb.out + i0));
+    // through LDS (every thread has read its residues: barriers passed since), then written
+    // lane-contiguous
 #pragma unroll
-        for (uint32_t q = 0; q < kCarryLimbsPerThread / 4; ++q)
-            dst[q] = u32x4{outv[4 * q], outv[4 * q + 1], outv[4 * q + 2], outv[4 * q + 3]};
+    for (uint32_t q = 0; q < kCarryLimbsPerThread / 4; ++q)
+        *reinterpret_cast<u32x4*>(s_res + tid * kCarryLimbsPerThread + 4u * q) =
+            u32x4{outv[4 * q], outv[4 * q + 1], outv[4 * q + 2], outv[4 * q + 3]};
+    __syncthreads();
+    uint32_t* out = jb.out + (size_t)tile * kCarryTile;
+    const uint32_t lim = jb.out_len - tile * kCarryTile;  // limbs of this tile to write
+    if ((reinterpret_cast<uintptr_t>(out) & 15u) == 0) {
+#pragma unroll
+        for (uint32_t k = 0; k < kCarryLimbsPerThread / 4; ++k) {
+            const uint32_t idx = 4u * (k * kThreads + tid);
+            const u32x4 x = *reinterpret_cast<const u32x4*>(s_res + idx);
+            if (idx + 4u <= lim) {
+                *gmem(reinterpret_cast<u32x4*>(out + idx)) = x;
+            } else {
+                if (idx < lim) gmem(out)[idx] = x.x;
+                if (idx + 1u < lim) gmem(out)[idx + 1u] = x.y;
+                if (idx + 2u < lim) gmem(out)[idx + 2u] = x.z;
+            }
+        }
     } else {
 #pragma unroll
-        for (uint32_t q = 0; q < kCarryLimbsPerThread; ++q)
-            if (i0 + q < jb.out_len) gmem(jb.out)[i0 + q] = outv[q];
+        for (uint32_t m = 0; m < kCarryLimbsPerThread; ++m) {
+            const uint32_t idx = m * kThreads + tid;
+            if (idx < lim) gmem(out)[idx] = s_res[idx];
+        }
     }
 }
 
