@@ -236,6 +236,38 @@ template <int LR, int LC, int ST, int TL, bool STORE>
 __device__ __forceinline__ void global_io(uint32_t* __restrict__ plane, const LimbSrc& src,
                                           uint32_t (&x)[kE], uint32_t tile_base, uint32_t log_s) {
     using Ly = Layout<LR, LC, ST, TL>;
+    if (!STORE && src.limbs) {
+        // Every limb load is issued before any word is cut, so the thread waits for memory once
+        // (with the branch inside the element loop each word waited for its own two loads).
+        // Byte offsets stay 32-bit: a limb string is far below 4 GiB.
+        uint32_t lo[kE], hi[kE], g[kE];
+        const uint32_t last = src.nlimbs - 1u;
+        const char* base = reinterpret_cast<const char*>(src.limbs);
+#pragma unroll
+        for (int sg = 0; sg < Ly::NSG; ++sg) {
+            const uint32_t c = Ly::col(sg);
+#pragma unroll
+            for (int q = 0; q < Ly::EG; ++q) {
+                const int e = sg * Ly::EG + q;
+                const uint32_t idx = tile_base + c + (Ly::row(sg, q) << log_s);
+                g[e] = idx < src.cnt ? src.off + idx : ~0u;  // ~0u: past the source, word 0
+                const uint32_t k = ((g[e] == ~0u ? 0u : g[e]) * 8u) / 7u;
+                const uint32_t k0 = k < last ? k : last, k1 = k + 1u < last ? k + 1u : last;
+                lo[e] = *gmem(reinterpret_cast<const uint32_t*>(base + (k0 << 2)));
+                hi[e] = *gmem(reinterpret_cast<const uint32_t*>(base + (k1 << 2)));
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < kE; ++e) {
+            // bits [32g, 32g + 32) lie in limbs k = floor(32g / 28) and k + 1 from bit
+            // o = 32g - 28k of limb k; limbs at and past nlimbs read as zero
+            const uint32_t gg = g[e] == ~0u ? 0u : g[e];
+            const uint32_t k = (gg * 8u) / 7u, o = gg * 32u - k * 28u;
+            const uint32_t l = k < src.nlimbs ? lo[e] : 0u, h = k + 1u < src.nlimbs ? hi[e] : 0u;
+            x[e] = g[e] == ~0u ? 0u : ((l >> o) | (h << (28u - o)));
+        }
+        return;
+    }
 #pragma unroll
     for (int sg = 0; sg < Ly::NSG; ++sg) {
         const uint32_t c = Ly::col(sg);
@@ -243,11 +275,6 @@ __device__ __forceinline__ void global_io(uint32_t* __restrict__ plane, const Li
         for (int q = 0; q < Ly::EG; ++q) {
             const uint32_t idx = tile_base + c + (Ly::row(sg, q) << log_s);
             if (STORE) gmem(plane)[idx] = x[sg * Ly::EG + q];
-            else if (src.limbs) {
-                const bool in = idx < src.cnt;
-                const uint32_t w = limb_word(src.limbs, src.nlimbs, src.off + (in ? idx : 0u));
-                x[sg * Ly::EG + q] = in ? w : 0u;
-            }
             else x[sg * Ly::EG + q] = gmem(plane)[idx];
         }
     }
