@@ -245,7 +245,7 @@ class PinnedPipe {
 class Engine {
 public:
     std::mutex mtx;
-    bool profiling = false;
+    uint32_t prof_mask = 0;  // bit k: bracket launches of kernel class k with events
     std::vector<ProfRec> prof;
     std::vector<hipEvent_t> event_pool;
 
@@ -259,10 +259,10 @@ public:
         if (hipEventCreate(&e) != hipSuccess) return nullptr;
         return e;
     }
-    // Bracket one launch with events when profiling.
+    // Bracket one launch with events when its kernel class is being profiled.
     template <class F>
     hipError_t timed(int kind, double bytes, hipStream_t st, F&& launch) {
-        if (!profiling) return launch();
+        if (!((prof_mask >> kind) & 1u)) return launch();
         ProfRec r{kind, get_event(), get_event(), bytes};
         if (!r.e0 || !r.e1) return hipErrorOutOfMemory;
         hipError_t e = hipEventRecord(r.e0, st);
@@ -1038,7 +1038,7 @@ public:
         // A level's launch sets go to the GPU as one captured graph (kernels only: the status
         // words are cleared by a fill kernel, not a memset node).  A graph of the same shape as
         // the previous one is updated in place instead of instantiated again.
-        const bool use_graph = graphs && !profiling && launches.size() > 1;
+        const bool use_graph = graphs && prof_mask == 0 && launches.size() > 1;
         const int rc = use_graph ? launch_as_graph(launch_all, st) : launch_all();
         if (rc) return rc;
         for (uint32_t i : split) {
@@ -1514,16 +1514,19 @@ int bi_hip_mlp_adam(float* d_params, const float* d_grad, float* d_m, float* d_v
                         (hipStream_t)stream) == hipSuccess ? BI_HIP_OK : BI_HIP_EDEVICE;
 }
 
-int bi_hip_profile(int enable) {
+int bi_hip_profile_kinds(uint32_t mask) {
+    mask &= (1u << BI_HIP_K_COUNT) - 1u;
     std::lock_guard<std::mutex> lk(engine().mtx);
     bi::Engine& e = engine();
-    if (enable) {
+    if (mask) {
         if (e.ready && hipStreamSynchronize(e.stream) != hipSuccess) return BI_HIP_EDEVICE;
         e.prof_clear();
     }
-    e.profiling = enable != 0;
+    e.prof_mask = mask;
     return BI_HIP_OK;
 }
+
+int bi_hip_profile(int enable) { return bi_hip_profile_kinds(enable ? ~0u : 0u); }
 
 int bi_hip_profile_read(bi_hip_kstat* stats, size_t cap) {
     if (!stats || cap < BI_HIP_K_COUNT) return BI_HIP_EINVAL;

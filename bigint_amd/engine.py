@@ -18,6 +18,7 @@ PLAN_FIXED, PLAN_TABLE, PLAN_LEARNED = 0, 1, 2
 
 EXPORTS = ["bi_hip_init", "bi_hip_mul", "bi_hip_mul_top", "bi_hip_mul_device", "bi_hip_mul_batch",
            "bi_hip_synchronize", "bi_hip_transform_log2", "bi_hip_strerror", "bi_hip_profile",
+           "bi_hip_profile_kinds",
            "bi_hip_profile_read", "bi_hip_mul_batch_device", "bi_hip_plan_count",
            "bi_hip_plan_allowed", "bi_hip_plan_cost", "bi_hip_mul_batch_device_planned", "bi_hip_choose_plans",
            "bi_hip_set_planner", "bi_hip_load_planner", "bi_hip_mlp_param_count",
@@ -80,6 +81,7 @@ def lib() -> ctypes.CDLL:
                                               ctypes.c_void_p]
         L.bi_hip_transform_log2.argtypes = [ctypes.c_size_t, ctypes.c_size_t]
         L.bi_hip_profile.argtypes = [ctypes.c_int]
+        L.bi_hip_profile_kinds.argtypes = [ctypes.c_uint32]
         L.bi_hip_profile_read.argtypes = [ctypes.POINTER(KStat), ctypes.c_size_t]
         vp = ctypes.c_void_p
         L.bi_hip_plan_allowed.argtypes = [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]
@@ -200,6 +202,16 @@ def synchronize() -> None:
 
 def profile(enable: bool) -> None:
     lib().bi_hip_profile(int(enable))
+
+
+def profile_kinds(names) -> None:
+    """Record events only around launches of the named kernel classes (an empty list stops)."""
+    mask = 0
+    for name in names:
+        mask |= 1 << KERNEL_NAMES.index(name)
+    rc = lib().bi_hip_profile_kinds(mask)
+    if rc:
+        raise EngineError(rc, "bi_hip_profile_kinds")
 
 
 def profile_read() -> dict:

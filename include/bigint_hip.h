@@ -179,6 +179,9 @@ typedef struct bi_hip_kstat {
 
 /* enable != 0 clears the counters and starts recording; 0 stops. */
 int bi_hip_profile(int enable);
+/* As bi_hip_profile, but only launches of the kernel classes whose bit (1 << BI_HIP_K_*) is set
+ * in mask are bracketed with events, so the other launches run without them; 0 stops. */
+int bi_hip_profile_kinds(uint32_t mask);
 /* Waits for the recorded launches and copies BI_HIP_K_COUNT entries into stats. */
 int bi_hip_profile_read(bi_hip_kstat* stats, size_t cap);
 
