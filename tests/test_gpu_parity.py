@@ -386,3 +386,27 @@ def test_batch_graph_replays_match_eager():
         assert all(np.array_equal(x, y) for x, y in zip(other, eager[::-1]))
     for (a, b), e in zip(pairs[:10], eager[:10]):
         assert np.array_equal(e, oracle.mul_limbs(a, b))
+
+
+def test_profile_kinds_brackets_only_chosen_classes():
+    """bi_hip_profile_kinds records events only around launches of the chosen kernel classes;
+    the product is the same with every launch bracketed, with one class, and unprofiled."""
+    a = L.random_operand(1 << 21, L.SEED)
+    b = L.random_operand(1 << 21, L.SEED + 1)
+    ref = engine.mul(a, b)
+    engine.profile(True)
+    p_all = engine.mul(a, b)
+    every = engine.profile_read()
+    engine.profile(False)
+    used = [k for k, v in every.items() if v["launches"]]
+    assert len(used) >= 2, every
+    dom = max(used, key=lambda k: every[k]["ms"])
+    engine.profile_kinds([dom])
+    p_one = engine.mul(a, b)
+    one = engine.profile_read()
+    engine.profile_kinds([])
+    assert one[dom]["launches"] == every[dom]["launches"]
+    assert one[dom]["bytes"] == every[dom]["bytes"]
+    assert all(v["launches"] == 0 for k, v in one.items() if k != dom), one
+    assert np.array_equal(p_all, ref) and np.array_equal(p_one, ref)
+    check_residues(a, b, ref)
