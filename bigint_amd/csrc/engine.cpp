@@ -384,6 +384,35 @@ public:
         return raw;
     }
 
+    // The workspace, its status words and the descriptor buffer are shared by every call, whatever
+    // stream it runs on.  shared_done is recorded after the last queued use (on shared_stream), so
+    // a call on another stream first waits for it: work queued on two streams without a host
+    // synchronisation between them still uses the buffers one call after the other.
+    hipEvent_t shared_done = nullptr;
+    hipStream_t shared_stream = nullptr;
+
+    // Queue, on st, a wait for the last use of the shared buffers on another stream.
+    int order_shared(hipStream_t st) {
+        if (!shared_stream || shared_stream == st) return BI_HIP_OK;
+        return hipStreamWaitEvent(st, shared_done, 0) == hipSuccess ? BI_HIP_OK : BI_HIP_EDEVICE;
+    }
+    // Block the host until every queued use of the shared buffers is done (before it rewrites the
+    // descriptor buffer with a synchronous copy).
+    int wait_shared() {
+        if (!shared_stream) return BI_HIP_OK;
+        return hipEventSynchronize(shared_done) == hipSuccess ? BI_HIP_OK : BI_HIP_EDEVICE;
+    }
+    // Record that work using the shared buffers has been queued on st.
+    int used_shared(hipStream_t st) {
+        if (!shared_done && hipEventCreateWithFlags(&shared_done, hipEventDisableTiming) != hipSuccess) {
+            shared_done = nullptr;
+            return BI_HIP_EDEVICE;
+        }
+        if (hipEventRecord(shared_done, st) != hipSuccess) return BI_HIP_EDEVICE;
+        shared_stream = st;
+        return BI_HIP_OK;
+    }
+
     // device workspace of at least `words` u32
     uint32_t* workspace(size_t words) {
         if (words > ws_words) {
@@ -534,9 +563,12 @@ public:
             gj.wrap_la = la;
             gj.wrap_lb = lb;
         }
-        return launch_products(pl, square ? kSquareKind : kPlain, 1, job, nullptr, 1, gj, nullptr, out_len,
-                               (double)la + (square ? 0.0 : (double)lb), (double)out_len, status,
-                               ntiles + 1, st);
+        if (int rc = order_shared(st)) return rc;
+        const int rc = launch_products(pl, square ? kSquareKind : kPlain, 1, job, nullptr, 1, gj, nullptr,
+                                       out_len, (double)la + (square ? 0.0 : (double)lb), (double)out_len,
+                                       status, ntiles + 1, st);
+        if (rc) return rc;
+        return used_shared(st);
     }
 
     // Batches as captured graphs: on by default (bi_hip_set_graphs); the last executable graph is
@@ -623,9 +655,12 @@ public:
     // Coefficients stay below min(wa, wb) 2^64 <= 2^90 < P0 P1 P2 while min(wa, wb) <= 2^26.
     static constexpr uint32_t kSplitLog = 25;
     static constexpr uint32_t kSplitChunk = 1u << kSplitLog;
+    // The product's limb count must stay below 2^32 - 2^13, so tile counts such as
+    // (la + lb + 4095) / 4096 and word offsets stay exact in the kernels' 32-bit arithmetic.
+    static constexpr uint64_t kMaxProductLimbs = 0xFFFFFFFFull - 8192u;
     static bool product_supported(size_t la, size_t lb) {
-        if (la == 0 || lb == 0 || la > 0xFFFFFFFFull || lb > 0xFFFFFFFFull) return false;
-        if (la + lb > 0xFFFFFFFFull) return false;
+        if (la == 0 || lb == 0 || la > kMaxProductLimbs || lb > kMaxProductLimbs) return false;
+        if (la + lb > kMaxProductLimbs) return false;
         if (bi_hip_transform_log2(la, lb) >= 0) return true;
         return std::min(words_of((uint32_t)la), words_of((uint32_t)lb)) <= (1u << 26);
     }
@@ -878,8 +913,7 @@ public:
             const bi_hip_dpair& p = pairs[i];
             const bool sq = p.b == nullptr || (p.b == p.a && p.lb == p.la);
             const size_t lb = sq ? p.la : p.lb;
-            if (!p.a || p.la == 0 || lb == 0 || !p.out || p.la > 0xFFFFFFFFull || lb > 0xFFFFFFFFull)
-                return BI_HIP_EINVAL;
+            if (!p.a || p.la == 0 || lb == 0 || !p.out) return BI_HIP_EINVAL;
             if (p.out_cap < p.la + lb) return BI_HIP_ECAP;
             if (!product_supported(p.la, lb)) return BI_HIP_ETOOBIG;
             Item it{(uint32_t)i, (uint32_t)p.la, (uint32_t)lb, sq, false, chosen[i], 1, 0};
@@ -1018,7 +1052,8 @@ public:
             L.status = status;
             launches.push_back(L);
         }
-        if (hipStreamSynchronize(st) != hipSuccess) return BI_HIP_EDEVICE;
+        // the descriptors of earlier calls may still be read, on this stream or another
+        if (wait_shared() || hipStreamSynchronize(st) != hipSuccess) return BI_HIP_EDEVICE;
         if (hipMemcpy(desc, hdesc.data(), off, hipMemcpyHostToDevice) != hipSuccess)
             return BI_HIP_EDEVICE;
         auto launch_all = [&]() -> int {
@@ -1041,6 +1076,7 @@ public:
         const bool use_graph = graphs && prof_mask == 0 && launches.size() > 1;
         const int rc = use_graph ? launch_as_graph(launch_all, st) : launch_all();
         if (rc) return rc;
+        if (int r2 = used_shared(st)) return r2;
         for (uint32_t i : split) {
             const bi_hip_dpair& p = pairs[i];
             const bool sq = p.b == nullptr || (p.b == p.a && p.lb == p.la);
@@ -1091,7 +1127,7 @@ public:
             if (hipMalloc((void**)&desc, need) != hipSuccess) return BI_HIP_ENOMEM;
             desc_bytes = need;
         }
-        if (hipStreamSynchronize(st) != hipSuccess ||
+        if (wait_shared() || hipStreamSynchronize(st) != hipSuccess ||
             hipMemcpy(desc, fwd.data(), fwd_bytes, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(desc + inv_off, inv.data(), inv.size() * sizeof(PassJob), hipMemcpyHostToDevice) != hipSuccess)
             return BI_HIP_EDEVICE;
@@ -1116,20 +1152,16 @@ public:
         g.chunk_len = L;
         g.log_chunk = kSplitLog;
         g.chunk_stride = 3 * n;
-        return launch_products(pl, kInverse, nd, inv[0], nd == 1 ? nullptr : d_inv, 1, g, nullptr, out_len,
-                               0.0, (double)out_len, status, nt + 1, st);
+        rc = launch_products(pl, kInverse, nd, inv[0], nd == 1 ? nullptr : d_inv, 1, g, nullptr, out_len,
+                             0.0, (double)out_len, status, nt + 1, st);
+        if (rc) return rc;
+        return used_shared(st);
     }
 };
 
 Engine& engine() {
     static Engine e;
     return e;
-}
-
-int check_operand(const uint32_t* a, size_t la) {
-    if (!a || la == 0) return BI_HIP_EINVAL;
-    if (la > 0xFFFFFFFFull) return BI_HIP_ETOOBIG;
-    return BI_HIP_OK;
 }
 
 }  // namespace
@@ -1159,10 +1191,11 @@ int bi_hip_transform_log2(size_t la, size_t lb) {
 
 int bi_hip_mul_device(const uint32_t* d_a, size_t la, const uint32_t* d_b, size_t lb,
                       int is_square, uint32_t* d_out, size_t out_cap, void* stream) {
-    if (bi::check_operand(d_a, la) || (!is_square && bi::check_operand(d_b, lb)) || !d_out)
-        return BI_HIP_EINVAL;
+    if (!d_a || la == 0 || (!is_square && (!d_b || lb == 0)) || !d_out) return BI_HIP_EINVAL;
     if (is_square) lb = la;
     if (out_cap < la + lb) return BI_HIP_ECAP;
+    // lengths are checked as size_t before run() takes them as 32-bit counts
+    if (!bi::Engine::product_supported(la, lb)) return BI_HIP_ETOOBIG;
     std::lock_guard<std::mutex> lk(engine().mtx);
     int rc = engine().init(0);
     if (rc) return rc;
@@ -1203,7 +1236,7 @@ int mul_host(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, int is_
 }
 
 int check_host_product(const uint32_t* a, size_t la, const uint32_t*& b, size_t& lb, int is_square) {
-    if (bi::check_operand(a, la) || (!is_square && bi::check_operand(b, lb))) return BI_HIP_EINVAL;
+    if (!a || la == 0 || (!is_square && (!b || lb == 0))) return BI_HIP_EINVAL;
     if (is_square) {
         b = a;
         lb = la;
@@ -1254,6 +1287,8 @@ int bi_hip_mul_batch_device(const bi_hip_dpair* pairs, size_t count, int plan_mo
             const size_t lb = sq ? p.la : p.lb;
             if (!p.a || !p.out || p.la == 0 || lb == 0) return BI_HIP_EINVAL;
             if (p.out_cap < p.la + lb) return BI_HIP_ECAP;
+            // lengths are checked as size_t before run() takes them as 32-bit counts
+            if (!bi::Engine::product_supported(p.la, lb)) return BI_HIP_ETOOBIG;
             rc = e.run(p.a, (uint32_t)p.la, sq ? p.a : p.b, (uint32_t)lb, sq, p.out, st, false);
             if (rc) return rc;
         }
@@ -1270,7 +1305,7 @@ int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode) {
     for (size_t i = 0; i < count; ++i) {
         bi_hip_pair& p = pairs[i];
         const bool sq = p.a == p.b && p.la == p.lb;
-        if (bi::check_operand(p.a, p.la) || (!sq && bi::check_operand(p.b, p.lb)) || !p.out) {
+        if (!p.a || p.la == 0 || (!sq && (!p.b || p.lb == 0)) || !p.out) {
             p.status = BI_HIP_EINVAL;
             return BI_HIP_EINVAL;
         }
@@ -1657,6 +1692,7 @@ int bi_hip_copy_to_host(void* const* h_dst, const size_t* bytes, size_t count, c
 int bi_hip_synchronize(void) {
     std::lock_guard<std::mutex> lk(engine().mtx);
     if (!engine().ready) return BI_HIP_OK;
+    if (engine().wait_shared()) return BI_HIP_EDEVICE;  // products queued on callers' streams
     return hipStreamSynchronize(engine().stream) == hipSuccess ? BI_HIP_OK : BI_HIP_EDEVICE;
 }
 

@@ -181,13 +181,22 @@ __device__ __forceinline__ void lds_io(uint32_t* lds, uint32_t (&x)[kE]) {
     }
 }
 
+// Limb index k = floor(32g / 28) = g + floor(g / 7) of transform word g, in 32-bit arithmetic
+// without overflow: an operand has fewer than 2^32 limbs, so g < 7 * 2^29 and k < 2^32 (8g itself
+// would overflow from g = 2^29 on).
+__device__ __forceinline__ uint32_t word_limb(uint32_t g) { return g + g / 7u; }
+
+// Bit offset o = 32g - 28k of word g inside limb k, in {0, 4, ..., 24}: the exact value is small,
+// so the wrapping 32-bit difference is exact.
+__device__ __forceinline__ uint32_t word_shift(uint32_t g, uint32_t k) { return g * 32u - k * 28u; }
+
 // Transform input word g of an operand: bits [32g, 32g + 32) of its limb string.  The bits lie
 // in limbs k = floor(32g / 28) and k + 1, from bit o = 32g - 28k (o in {0, 4, ..., 24}) of limb k;
 // limbs at and past nlimbs read as zero.
 __device__ __forceinline__ uint32_t limb_word(const uint32_t* __restrict__ limbs, uint32_t nlimbs,
                                               uint32_t g) {
-    const uint32_t k = (g * 8u) / 7u;
-    const uint32_t o = g * 32u - k * 28u;
+    const uint32_t k = word_limb(g);
+    const uint32_t o = word_shift(g, k);
     // loads at clamped (always valid) indices, then selects: no branch around the loads
     const uint32_t last = nlimbs - 1u;
     uint32_t lo = gmem(limbs)[k < last ? k : last];
@@ -239,10 +248,9 @@ __device__ __forceinline__ void global_io(uint32_t* __restrict__ plane, const Li
     if (!STORE && src.limbs) {
         // Every limb load is issued before any word is cut, so the thread waits for memory once
         // (with the branch inside the element loop each word waited for its own two loads).
-        // Byte offsets stay 32-bit: a limb string is far below 4 GiB.
+        // Limb indices are below 2^32; the loads index with them as 64-bit offsets.
         uint32_t lo[kE], hi[kE], g[kE];
         const uint32_t last = src.nlimbs - 1u;
-        const char* base = reinterpret_cast<const char*>(src.limbs);
 #pragma unroll
         for (int sg = 0; sg < Ly::NSG; ++sg) {
             const uint32_t c = Ly::col(sg);
@@ -251,10 +259,10 @@ __device__ __forceinline__ void global_io(uint32_t* __restrict__ plane, const Li
                 const int e = sg * Ly::EG + q;
                 const uint32_t idx = tile_base + c + (Ly::row(sg, q) << log_s);
                 g[e] = idx < src.cnt ? src.off + idx : ~0u;  // ~0u: past the source, word 0
-                const uint32_t k = ((g[e] == ~0u ? 0u : g[e]) * 8u) / 7u;
+                const uint32_t k = word_limb(g[e] == ~0u ? 0u : g[e]);
                 const uint32_t k0 = k < last ? k : last, k1 = k + 1u < last ? k + 1u : last;
-                lo[e] = *gmem(reinterpret_cast<const uint32_t*>(base + (k0 << 2)));
-                hi[e] = *gmem(reinterpret_cast<const uint32_t*>(base + (k1 << 2)));
+                lo[e] = gmem(src.limbs)[(size_t)k0];
+                hi[e] = gmem(src.limbs)[(size_t)k1];
             }
         }
 #pragma unroll
@@ -262,7 +270,7 @@ __device__ __forceinline__ void global_io(uint32_t* __restrict__ plane, const Li
             // bits [32g, 32g + 32) lie in limbs k = floor(32g / 28) and k + 1 from bit
             // o = 32g - 28k of limb k; limbs at and past nlimbs read as zero
             const uint32_t gg = g[e] == ~0u ? 0u : g[e];
-            const uint32_t k = (gg * 8u) / 7u, o = gg * 32u - k * 28u;
+            const uint32_t k = word_limb(gg), o = word_shift(gg, k);
             const uint32_t l = k < src.nlimbs ? lo[e] : 0u, h = k + 1u < src.nlimbs ? hi[e] : 0u;
             x[e] = g[e] == ~0u ? 0u : ((l >> o) | (h << (28u - o)));
         }

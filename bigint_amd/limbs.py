@@ -5,8 +5,9 @@ below 2^28, value = sum(d[i] * 2^(28*i)) (include/bigint/bigint_base.h:21-26).
 
 Operands for the benchmark workloads come from SplitMix64: operand A uses seed
 0x9e3779b97f4a7c15 (the seed of the reference's regression, tests/measure_digit_bits.cpp:140),
-B uses seed+1 and batch pair i uses seed+2+i.  An operand of b bits takes ceil(b/64) words of the
-stream, is masked to b bits and gets bit b-1 set, then is repacked into 28-bit limbs.
+B uses seed+1 and batch pair i uses seeds seed+2+2i and seed+3+2i (batch_seeds).  An operand of
+b bits takes ceil(b/64) words of the stream, is masked to b bits and gets bit b-1 set, then is
+repacked into 28-bit limbs.
 """
 from __future__ import annotations
 
@@ -122,6 +123,12 @@ def batch_bits(count: int, seed: int = SEED, lo_exp: int = 12, span: int = 12) -
     u = (xs >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
     bits = np.floor(np.exp2(lo_exp + span * u)).astype(np.int64)
     return [(int(bits[2 * i]), int(bits[2 * i + 1])) for i in range(count)]
+
+
+def batch_seeds(count: int, seed: int = SEED) -> list[tuple[int, int]]:
+    """SplitMix64 seeds of the batched workload's operands: pair i takes seed + 2 + 2i for its
+    first operand and seed + 3 + 2i for its second (seed and seed + 1 are the single product's)."""
+    return [((seed + 2 + 2 * i) & _M64, (seed + 3 + 2 * i) & _M64) for i in range(count)]
 
 
 # Primes below 2^31 for size-independent product checks: (a mod q)(b mod q) = (a*b) mod q.

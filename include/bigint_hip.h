@@ -22,7 +22,13 @@
  *
  * Thread safety: every entry point may be called from any thread; calls are serialised by an
  * internal mutex (the reference serialises pool phases the same way,
- * src/mul/ntt_multithread.cpp:259-260).
+ * src/mul/ntt_multithread.cpp:259-260).  The asynchronous entry points share one device
+ * workspace; a call queued on another stream than the previous call waits on the device for the
+ * previous call's work (hipStreamWaitEvent), so products queued on several streams without a
+ * host synchronisation in between are computed one after the other and stay exact.
+ *
+ * Limbs must be below 2^28 (the Digits invariant).  This is not checked: a larger limb gives a
+ * wrong product but never an access outside the operands.
  */
 #ifndef BIGINT_HIP_H
 #define BIGINT_HIP_H
@@ -36,7 +42,7 @@ extern "C" {
 
 enum {
     BI_HIP_OK = 0,
-    BI_HIP_EINVAL = 1,   /* bad argument: null pointer, zero length, top limb zero, limb >= 2^28 */
+    BI_HIP_EINVAL = 1,   /* bad argument: null pointer, zero length, top limb zero (host paths) */
     BI_HIP_ETOOBIG = 2,  /* product beyond a 2^26-point transform (see bi_hip_transform_log2) */
     BI_HIP_ENOMEM = 3,   /* device or host allocation failed */
     BI_HIP_EDEVICE = 4,  /* HIP runtime error */
@@ -149,7 +155,7 @@ int bi_hip_set_graphs(int enable);
  * returns the first nonzero status or BI_HIP_OK. */
 int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode);
 
-/* Wait for all work queued by the engine. */
+/* Wait for all work queued by the engine, on its own stream and on callers' streams. */
 int bi_hip_synchronize(void);
 
 /* Transform length (log2) of an unchunked la x lb product, or -1 if too large.  The transforms

@@ -14,6 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 _MT = None
+_CHK = None
 
 u32p = ctypes.POINTER(ctypes.c_uint32)
 u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -22,7 +23,7 @@ szp = ctypes.POINTER(ctypes.c_size_t)
 
 def build(force: bool = False) -> None:
     """Compile liboracle.so and libcpu_mt.so with the recipe in oracle/Makefile."""
-    targets = [os.path.join(HERE, "liboracle.so"), os.path.join(HERE, "libcpu_mt.so")]
+    targets = [os.path.join(HERE, n) for n in ("liboracle.so", "libcpu_mt.so", "libbatch_check.so")]
     if force or not all(os.path.exists(t) for t in targets):
         subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"] + ([] if not force else ["all"]),
                        check=True)
@@ -138,3 +139,46 @@ def mt_lib() -> ctypes.CDLL:
         L.cpu_pool_threads.restype = ctypes.c_uint32
         _MT = L
     return _MT
+
+
+# ---- large-batch checker (batch_check.c) ------------------------------------------------------
+
+def chk_lib() -> ctypes.CDLL:
+    global _CHK
+    if _CHK is None:
+        path = os.path.join(HERE, "libbatch_check.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.chk_random_operand.restype = ctypes.c_size_t
+        L.chk_random_operand.argtypes = [ctypes.c_uint64, ctypes.c_uint64, u32p]
+        L.chk_residues.restype = ctypes.c_int
+        L.chk_residues.argtypes = [u32p, u64p, u64p, ctypes.c_size_t, ctypes.c_uint32, u32p, ctypes.c_int]
+        _CHK = L
+    return _CHK
+
+
+def random_operands(bits: list[int], seeds: list[int]) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """limbs.random_operand(bits[i], seeds[i]) for every i, concatenated into one pool; returns
+    (pool, offsets, lengths) in limbs."""
+    lens = np.array([-(-b // 28) for b in bits], np.uint64)
+    offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.uint64)
+    pool = np.zeros(int(lens.sum()), np.uint32)
+    L = chk_lib()
+    base = pool.ctypes.data
+    for b, s, o in zip(bits, seeds, offs):
+        L.chk_random_operand(int(b), int(s) & ((1 << 64) - 1),
+                             ctypes.cast(base + 4 * int(o), u32p))
+    return pool, offs, lens
+
+
+def residues(pool: np.ndarray, offs: np.ndarray, lens: np.ndarray, q: int, threads: int = 16) -> np.ndarray:
+    """Value of each segment pool[offs[i] : offs[i] + lens[i]] (28-bit limbs) mod q."""
+    pool = np.ascontiguousarray(pool, dtype=np.uint32)
+    offs = np.ascontiguousarray(offs, dtype=np.uint64)
+    lens = np.ascontiguousarray(lens, dtype=np.uint64)
+    assert offs.size == lens.size and (lens.size == 0 or int((offs + lens).max()) <= pool.size)
+    out = np.zeros(offs.size, np.uint32)
+    if chk_lib().chk_residues(_p32(pool), _p64(offs), _p64(lens), offs.size, q, _p32(out), threads):
+        raise ValueError("modulus out of range")
+    return out.astype(np.int64)

@@ -126,6 +126,12 @@ int main() {
                 if (op == "fmul") out << XF(BigFloat::mul(fa, fb, prec));
                 else if (op == "fadd") out << XF(fa + fb);
                 else out << XF(fa - fb);
+            } else if (op == "fsqr") {  // one object as both operands: the squaring route
+                long long pa;
+                size_t prec;
+                in >> a >> pa >> prec;
+                BigFloat fa = F(a, pa);
+                out << XF(BigFloat::mul(fa, fa, prec));
             } else if (op == "frecip") {
                 long long pa;
                 size_t prec;

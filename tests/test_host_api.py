@@ -303,6 +303,95 @@ def test_large_reciprocal_matches_model(calc):
         assert calc("frecip", hx(a), 650, prec) == xf(BF.reciprocal(f, prec))
 
 
+def _newton_bound_ok(a, pa, res, prec):
+    """|x * b - 1| < 2^(-28 (prec - 2)) for x = D 2^(-28 pp) and b = a 2^(-28 pa): the error bound
+    of the doubling Newton iteration (src/bigint.cpp:1044-1071) at its final precision."""
+    neg, D, pp = res
+    e = 28 * (pp + pa)
+    err = abs(D * a - (1 << e)) if e >= 0 else abs(D * a * (1 << -e) - 1)
+    return err * (1 << (28 * max(1, prec - 2))) < (1 << max(e, 0))
+
+
+def _parse_f(s):
+    sign, d, pp = s.split()
+    return (sign == "-", int(d, 16), int(pp))
+
+
+@pytest.mark.gpu
+def test_reciprocal_at_engine_sizes(calc):
+    """BigFloat::reciprocal at precisions of 2^12 limbs and more, where the Newton products run on
+    the engine.  Pinned by what the mathematics fixes: the Newton error bound for random
+    divisors; 1/(B-1) = sum B^-k (every limb 1) and 1/(B+1) = sum (B-1) B^-2k (limbs alternating
+    B-1, 0), B = 2^28, up to the last few limbs; and 2^(28k-j) for b = 2^(j-28k), one nonzero
+    limb.  Random divisors are also compared limb for limb with the Python model."""
+    B = 1 << 28
+    rng = random.Random(1044)
+    g0, _ = map(int, calc("counts").split())
+    for prec in [4096, 20000, 60000]:
+        for a, pa in [(B - 1, 0), (B + 1, 0), (B - 1, 7), (B + 1, -3)]:
+            res = _parse_f(calc("frecip", hx(a), pa, prec))
+            assert _newton_bound_ok(a, pa, res, prec), (a, pa, prec)
+            neg, D, pp = res
+            n = (D.bit_length() + 27) // 28
+            assert n >= prec - 2
+            limbs = [(D >> (28 * i)) & (B - 1) for i in range(n)]
+            top = limbs[4:]  # all but the lowest four limbs
+            if a == B - 1:
+                assert all(x == 1 for x in top), (pa, prec)
+            else:
+                pattern = [B - 1 if (n - 1 - i) % 2 == 0 else 0 for i in range(4, n)]
+                assert top == pattern, (pa, prec)
+        for k, j in [(0, 0), (1, 5), (300, 27), (5000, 13)]:
+            res = _parse_f(calc("frecip", hx(1 << j), k, prec))  # b = 2^j * 2^(-28k)
+            neg, D, pp = res
+            # one nonzero limb (the final shift may leave a zero limb below it) and the value
+            # exactly 1/b = 2^(28k - j)
+            assert not neg and D & (D - 1) == 0 and D < (B << 28), (k, j, prec)
+            assert D.bit_length() - 1 - 28 * pp == 28 * k - j, (k, j, prec)
+    for prec, bits in [(4096, 28 * 5000), (20000, 28 * 12000), (30000, 28 * 30000)]:
+        a = rng.getrandbits(bits) | (1 << (bits - 1)) | 1
+        pa = rng.randrange(-50, 50)
+        s = calc("frecip", hx(a), pa, prec)
+        assert _newton_bound_ok(a, pa, _parse_f(s), prec), prec
+        assert s == xf(BF.reciprocal(BF.from_int(a, -28 * pa), prec)), prec
+    g1, _ = map(int, calc("counts").split())
+    assert g1 > g0  # the Newton products went through the engine
+
+
+@pytest.mark.gpu
+def test_precision_products_at_engine_sizes(calc):
+    """BigFloat::mul with 0 < precision < la + lb on operands of 20k-120k limbs, the engine route
+    that fetches only the kept top limbs (include/bigint/mul.h:227-243, 302-312).  Cases: the cut
+    inside a run of zero limbs of the product (stripped as trailing zeros), operands truncated to
+    precision limbs whose new low limbs are zero, squares, and cuts at la + lb - 1.  Expected: the
+    Python model of mul_digits (the exact product of the truncated operands, truncated again)."""
+    B = 1 << 28
+    rng = random.Random(302)
+    g0, _ = map(int, calc("counts").split())
+    X = rng.getrandbits(28 * 20000) | (1 << (28 * 20000 - 1))
+    Y = rng.getrandbits(28 * 2000) | (1 << (28 * 2000 - 1)) | 1
+    m = 3000
+    a = X * B ** m + 1          # limbs 1 .. m-1 are zero
+    la, lb = 20000 + m, 2000
+    cases = [(a, Y, la), (a, Y, la + 1), (a, Y, la + lb - 1), (a, Y, la - 10), (a, Y, 20000 + 5)]
+    P = rng.getrandbits(28 * 60000) | (1 << (28 * 60000 - 1))
+    Q = rng.getrandbits(28 * 50000) | (1 << (28 * 50000 - 1))
+    cases += [(P, Q, 30000), (P, Q, 80000), (P, Q, 109999), (P, -Q, 55000)]
+    R = rng.getrandbits(28 * 120000) | (1 << (28 * 120000 - 1))
+    S = rng.getrandbits(28 * 120000) | (1 << (28 * 120000 - 1))
+    cases += [(R, S, 100000), (R, S, 239999)]
+    for x, y, prec in cases:
+        px, py = rng.randrange(-40, 40), rng.randrange(-40, 40)
+        fx, fy = BF.from_int(x, -28 * px), BF.from_int(y, -28 * py)
+        assert calc("fmul", hx(x), px, hx(y), py, prec) == xf(BF.mul(fx, fy, prec)), prec
+    for x, prec in [(a, la), (a, la + 7), (P, 45000), (R, 200000)]:
+        px = rng.randrange(-40, 40)
+        fx = BF.from_int(x, -28 * px)
+        assert calc("fsqr", hx(x), px, prec) == xf(BF.mul(fx, fx, prec, same=True)), prec
+    g1, _ = map(int, calc("counts").split())
+    assert g1 > g0
+
+
 def _splitmix(seed):
     st = seed
     M = (1 << 64) - 1

@@ -59,3 +59,19 @@ def test_batch_bits_range():
     # log-uniform: median near 2^18
     med = sorted(flat)[len(flat) // 2]
     assert (1 << 17) < med < (1 << 19)
+
+
+def test_batch_checker_matches_python():
+    """oracle/batch_check.c (seeded operands and residues for gigabit batches) against
+    limbs.random_operand and limbs.residue_mod."""
+    from oracle import oracle
+    bits = [1, 27, 28, 29, 63, 64, 65, 1000, 123457, 1 << 20]
+    seeds = [L.SEED + i for i in range(len(bits))]
+    pool, offs, lens = oracle.random_operands(bits, seeds)
+    for b, s, o, n in zip(bits, seeds, offs, lens):
+        assert np.array_equal(pool[int(o):int(o + n)], L.random_operand(b, s)), b
+    for q in L.RESIDUE_PRIMES:
+        r = oracle.residues(pool, offs, lens, q, threads=3)
+        for i, (o, n) in enumerate(zip(offs, lens)):
+            assert r[i] == L.residue_mod(pool[int(o):int(o + n)], q)
+    assert L.batch_seeds(2) == [(L.SEED + 2, L.SEED + 3), (L.SEED + 4, L.SEED + 5)]

@@ -188,10 +188,26 @@ def gio(plane, n, X, EG, log_c, log_trlo, tile_base, log_s, nthreads, store, lim
                 X[tid][slot] = plane[idx]
 
 
+M32 = 0xFFFFFFFF
+
+
+def word_limb(g):
+    """Kernel word_limb in 32-bit arithmetic: k = g + g / 7 (= floor(32 g / 28))."""
+    assert 0 <= g <= M32
+    k = g + g // 7
+    assert k <= M32, "limb index overflows 32 bits"
+    return k
+
+
+def word_shift(g, k):
+    """Kernel word_shift: 32 g - 28 k computed modulo 2^32."""
+    return (g * 32 - k * 28) & M32
+
+
 def limb_word(limbs, g):
     """Kernel limb_word: bits [32g, 32g+32) of the limb string, from limbs k and k+1."""
-    k = (g * 8) // 7
-    o = g * 32 - k * 28
+    k = word_limb(g)
+    o = word_shift(g, k)
     assert o in (0, 4, 8, 12, 16, 20, 24)
     lo = limbs[k] if k < len(limbs) else 0
     hi = limbs[k + 1] if k + 1 < len(limbs) else 0
@@ -414,6 +430,35 @@ def test_limb_words_reassemble_the_integer(nlimbs):
     assert w[-1] != 0 and all(0 <= x < 1 << 32 for x in w)
     # one word past the end reads zero
     assert limb_word(limbs, len(w)) == 0
+
+
+@pytest.mark.parametrize("nlimbs", [700_000_000, 613_566_757, 0xFFFFFFFF - 8192])
+def test_word_index_math_at_long_operands(nlimbs):
+    """The first pass's word -> limb index math in the kernels' 32-bit arithmetic, replayed for
+    operands far beyond 2^29 words (where 8 g overflows 32 bits): for words sampled over the whole
+    operand and every word of its last 7-word groups, k = g + g / 7 and o = 32 g - 28 k (wrapping)
+    equal the exact floor(32 g / 28) and its remainder, and k, k + 1 stay in 32 bits."""
+    import numpy as np
+    nw = words_of(nlimbs)
+    rng = np.random.default_rng(nlimbs)
+    g = np.concatenate([rng.integers(0, nw, 200_000, dtype=np.int64),
+                        np.arange(max(0, nw - 4096), nw, dtype=np.int64),
+                        np.arange(0, 4096, dtype=np.int64), [(1 << 29) - 1, 1 << 29]]).astype(np.int64)
+    g = g[g < nw]
+    gu = g.astype(np.uint32)
+    k = gu + gu // np.uint32(7)                                   # word_limb, uint32
+    o = gu * np.uint32(32) - k * np.uint32(28)                    # word_shift, wrapping uint32
+    exact_k = (32 * g) // 28
+    assert np.array_equal(k.astype(np.int64), exact_k)
+    assert np.array_equal(o.astype(np.int64), 32 * g - 28 * exact_k)
+    assert set(np.unique(o).tolist()) <= {0, 4, 8, 12, 16, 20, 24}
+    assert int(k.max()) + 1 <= M32 and int(k.max()) < nlimbs
+    # the formula it replaces, (8 g) / 7 in 32 bits, is wrong from g = 2^29 on
+    old = (gu * np.uint32(8)) // np.uint32(7)
+    assert (nw <= 1 << 29) or not np.array_equal(old.astype(np.int64), exact_k)
+    # tile counts of the product stay exact in 32 bits at the largest supported product
+    out_len = min(2 * nlimbs, 0xFFFFFFFF - 8192)
+    assert out_len + 4095 <= M32 and (out_len + 4095) // 4096 * 3584 <= M32
 
 
 @pytest.mark.parametrize("la,lb", [(1, 1), (8, 9), (4096, 3), (5000, 5000), (9000, 4)])
