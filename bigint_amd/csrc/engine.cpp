@@ -21,6 +21,14 @@
 #include "ntt_primes.h"
 #include "planner.h"
 
+// Scratch of bi_hip_mul_device_noalloc: residues of both operands (three planes of n words each)
+// and the carry-scan status words of one product on a transform of up to 2^kNoAllocMaxLog
+// points.  Each device holds its own copy (24 MiB) from the time the module is loaded until the
+// process ends.  Host code uses the device address from hipGetSymbolAddress, never the name.
+constexpr uint32_t kNoAllocMaxLog = 20;
+constexpr size_t kNoAllocWords = ((size_t)6 << kNoAllocMaxLog) + 1024;
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
 namespace bi {
 namespace {
 
@@ -260,9 +268,10 @@ public:
         return e;
     }
     // Bracket one launch with events when its kernel class is being profiled.
+    bool suppress_prof = false;  // set while the allocation-free entry point launches
     template <class F>
     hipError_t timed(int kind, double bytes, hipStream_t st, F&& launch) {
-        if (!((prof_mask >> kind) & 1u)) return launch();
+        if (suppress_prof || !((prof_mask >> kind) & 1u)) return launch();
         ProfRec r{kind, get_event(), get_event(), bytes};
         if (!r.e0 || !r.e1) return hipErrorOutOfMemory;
         hipError_t e = hipEventRecord(r.e0, st);
@@ -325,8 +334,23 @@ public:
         gk.inv01 = to_mont(invmod(p0 % p1, p1), p1);
         gk.inv02 = to_mont(invmod(p0 % p2, p2), p2);
         gk.inv12 = to_mont(invmod(p1 % p2, p2), p2);
+        // set-up of the allocation-free entry point on this device: the scratch array's device
+        // address, and every plan it may use built now (plan() allocates its root tables)
+        if (hipGetSymbolAddress((void**)&noalloc_scratch, HIP_SYMBOL(::scratch)) != hipSuccess || !noalloc_scratch)
+            return BI_HIP_EDEVICE;
+        for (uint32_t lg = 4; lg <= kNoAllocMaxLog; ++lg)
+            if (!plan(lg)) return BI_HIP_ENOMEM;
         ready = true;
         return BI_HIP_OK;
+    }
+
+    uint32_t* noalloc_scratch = nullptr;  // device address of `scratch` on this engine's device
+
+    // An existing plan, never built here (so nothing is allocated).
+    const Plan* find_plan(uint32_t log_n, uint32_t mid_lr = 12) const {
+        if (log_n <= mid_lr) mid_lr = log_n;
+        const auto it = plans.find(log_n * 32u + mid_lr);
+        return it == plans.end() ? nullptr : it->second.get();
     }
 
     // Stage split for a transform of 2^log_n points whose contiguous block pass has 2^mid_lr
@@ -539,12 +563,26 @@ public:
         const uint32_t lg = (uint32_t)lg0;
         Plan* pl = planned ? plan(lg, chosen[0].mid_lr) : plan(lg);
         if (!pl) return BI_HIP_ENOMEM;
-        const uint32_t n = 1u << lg;
+        uint32_t* w = workspace(single_words(lg, la, lb, square));
+        if (!w) return BI_HIP_ENOMEM;
+        if (int rc = order_shared(st)) return rc;
+        if (int rc = launch_single(pl, d_a, la, d_b, lb, square, d_out, w, st)) return rc;
+        return used_shared(st);
+    }
+
+    // Device words one unchunked product needs: the residues of both operands (one for a square)
+    // and the carry-scan status words.
+    static size_t single_words(uint32_t lg, uint32_t la, uint32_t lb, bool square) {
+        return ((size_t)3 << lg) * (square ? 1 : 2) + ((size_t)la + lb + 4095u) / 4096u + 64;
+    }
+
+    // Queue one unchunked product on plan pl with its residues and status words in w (at least
+    // single_words() words): no allocation and no host synchronisation.
+    int launch_single(const Plan* pl, const uint32_t* d_a, uint32_t la, const uint32_t* d_b, uint32_t lb,
+                      bool square, uint32_t* d_out, uint32_t* w, hipStream_t st) {
+        const uint32_t n = 1u << pl->log_n;
         const uint32_t out_len = la + lb;
         const uint32_t ntiles = (out_len + 4095u) / 4096u;
-        const size_t need = (size_t)3 * n * (square ? 1 : 2) + ntiles + 64;
-        uint32_t* w = workspace(need);
-        if (!w) return BI_HIP_ENOMEM;
         uint32_t* res_a = w;
         uint32_t* res_b = square ? nullptr : w + (size_t)3 * n;
         uint32_t* status = w + (size_t)3 * n * (square ? 1 : 2);
@@ -563,12 +601,9 @@ public:
             gj.wrap_la = la;
             gj.wrap_lb = lb;
         }
-        if (int rc = order_shared(st)) return rc;
-        const int rc = launch_products(pl, square ? kSquareKind : kPlain, 1, job, nullptr, 1, gj, nullptr,
-                                       out_len, (double)la + (square ? 0.0 : (double)lb), (double)out_len,
-                                       status, ntiles + 1, st);
-        if (rc) return rc;
-        return used_shared(st);
+        return launch_products(pl, square ? kSquareKind : kPlain, 1, job, nullptr, 1, gj, nullptr, out_len,
+                               (double)la + (square ? 0.0 : (double)lb), (double)out_len, status,
+                               ntiles + 1, st);
     }
 
     // Batches as captured graphs: on by default (bi_hip_set_graphs); the last executable graph is
@@ -1204,6 +1239,28 @@ int bi_hip_mul_device(const uint32_t* d_a, size_t la, const uint32_t* d_b, size_
                         d_out, st);
 }
 
+int bi_hip_mul_device_noalloc(const uint32_t* d_a, size_t la, const uint32_t* d_b, size_t lb,
+                              int is_square, uint32_t* d_out, size_t out_cap, void* stream) {
+    if (!d_a || la == 0 || (!is_square && (!d_b || lb == 0)) || !d_out) return BI_HIP_EINVAL;
+    if (is_square) lb = la;
+    if (out_cap < la + lb) return BI_HIP_ECAP;
+    std::lock_guard<std::mutex> lk(engine().mtx);
+    bi::Engine& e = engine();
+    if (!e.ready || !e.noalloc_scratch) return BI_HIP_ENOINIT;
+    const int lg = bi_hip_transform_log2(la, lb);
+    if (lg < 0 || lg > (int)kNoAllocMaxLog) return BI_HIP_ESCRATCH;
+    if (bi::Engine::single_words((uint32_t)lg, (uint32_t)la, (uint32_t)lb, is_square != 0) > kNoAllocWords)
+        return BI_HIP_ESCRATCH;
+    const bi::Plan* pl = e.find_plan((uint32_t)lg);
+    if (!pl) return BI_HIP_ENOINIT;
+    hipStream_t st = stream ? (hipStream_t)stream : e.stream;
+    e.suppress_prof = true;  // profiling would create events
+    const int rc = e.launch_single(pl, d_a, (uint32_t)la, is_square ? d_a : d_b, (uint32_t)lb, is_square != 0,
+                                   d_out, e.noalloc_scratch, st);
+    e.suppress_prof = false;
+    return rc;
+}
+
 namespace {
 
 // a * b for host limb arrays; limbs [la + lb - keep, la + lb) of the product go to out.
@@ -1705,6 +1762,8 @@ const char* bi_hip_strerror(int code) {
         case BI_HIP_EDEVICE: return "HIP runtime error";
         case BI_HIP_ENODEV: return "no gfx950 device";
         case BI_HIP_ECAP: return "output capacity too small";
+        case BI_HIP_ESCRATCH: return "product too large for the allocation-free scratch (transforms up to 2^20 points)";
+        case BI_HIP_ENOINIT: return "engine not initialised: call bi_hip_init first";
         default: return "unknown status";
     }
 }

@@ -13,7 +13,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbigint_hip.so")
 
-OK, EINVAL, ETOOBIG, ENOMEM, EDEVICE, ENODEV, ECAP = range(7)
+OK, EINVAL, ETOOBIG, ENOMEM, EDEVICE, ENODEV, ECAP, ESCRATCH, ENOINIT = range(9)
 PLAN_FIXED, PLAN_TABLE, PLAN_LEARNED = 0, 1, 2
 
 EXPORTS = ["bi_hip_init", "bi_hip_mul", "bi_hip_mul_top", "bi_hip_mul_device", "bi_hip_mul_batch",
@@ -24,7 +24,7 @@ EXPORTS = ["bi_hip_init", "bi_hip_mul", "bi_hip_mul_top", "bi_hip_mul_device", "
            "bi_hip_set_planner", "bi_hip_load_planner", "bi_hip_mlp_param_count",
            "bi_hip_mlp_forward", "bi_hip_mlp_grad", "bi_hip_mlp_adam", "bi_hip_set_graphs",
            "bi_hip_add_batch_device", "bi_hip_device_alloc", "bi_hip_device_free",
-           "bi_hip_copy_to_device", "bi_hip_copy_to_host"]
+           "bi_hip_copy_to_device", "bi_hip_copy_to_host", "bi_hip_mul_device_noalloc"]
 KERNEL_NAMES = ["fill", "outer_fwd", "mid", "outer_inv", "garner_carry", "outer_first"]
 
 u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -76,6 +76,7 @@ def lib() -> ctypes.CDLL:
         L.bi_hip_mul_device.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
                                         ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p,
                                         ctypes.c_size_t, ctypes.c_void_p]
+        L.bi_hip_mul_device_noalloc.argtypes = L.bi_hip_mul_device.argtypes
         L.bi_hip_mul_batch.argtypes = [ctypes.POINTER(Pair), ctypes.c_size_t, ctypes.c_int]
         L.bi_hip_mul_batch_device.argtypes = [ctypes.POINTER(DPair), ctypes.c_size_t, ctypes.c_int,
                                               ctypes.c_void_p]

@@ -47,7 +47,9 @@ enum {
     BI_HIP_ENOMEM = 3,   /* device or host allocation failed */
     BI_HIP_EDEVICE = 4,  /* HIP runtime error */
     BI_HIP_ENODEV = 5,   /* no gfx950 device */
-    BI_HIP_ECAP = 6      /* output capacity too small */
+    BI_HIP_ECAP = 6,     /* output capacity too small */
+    BI_HIP_ESCRATCH = 7, /* bi_hip_mul_device_noalloc: product beyond its fixed scratch */
+    BI_HIP_ENOINIT = 8   /* bi_hip_mul_device_noalloc: bi_hip_init has not run */
 };
 
 /* Plan modes for bi_hip_mul_batch. */
@@ -94,6 +96,20 @@ int bi_hip_mul_top(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, i
  * installed table, else the built-in cost heuristic). */
 int bi_hip_mul_device(const uint32_t* d_a, size_t la, const uint32_t* d_b, size_t lb,
                       int is_square, uint32_t* d_out, size_t out_cap, void* stream);
+
+/* bi_hip_mul_device without any allocation, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same arguments and result as bi_hip_mul_device, but always the
+ * unchunked plan 0, no profiling, and no hipMalloc / hipFree / event creation in the call: the
+ * residues and carry status live in a fixed device array of 24 MiB (6 * 2^20 + 1024 words) per
+ * device, set up by bi_hip_init (BI_HIP_ENOINIT before that), so a product must fit a transform
+ * of at most 2^20 points, about 2^20 words per operand in a balanced product (BI_HIP_ESCRATCH
+ * beyond).  Calls share that array: calls that overlap on one GPU (on different streams without
+ * ordering between them) race; calls on one stream, or ordered by the caller, do not.  The
+ * array keeps the last call's contents; every call clears the words it needs cleared with a
+ * kernel (no memset). */
+int bi_hip_mul_device_noalloc(const uint32_t* d_a, size_t la, const uint32_t* d_b, size_t lb,
+                              int is_square, uint32_t* d_out, size_t out_cap, void* stream);
 
 /* One product of a device-resident batch: a, b, out are device pointers.  b == NULL or
  * (b == a, lb == la) selects squaring.  Exactly la + lb limbs are written to out. */
