@@ -1,7 +1,7 @@
 // Device-side modular arithmetic for the three NTT primes (gfx950 / CDNA4).
 //
-// The 28-bit limb width is the reference's (include/bigint/ntt_base.h:14-20), the primes are in
-// ntt_primes.h; Montgomery reduction uses R = 2^32 as in include/bigint/ntt_base.h:40-71.  The
+// The 28-bit limb width and the primes are the reference's (include/bigint/ntt_base.h:14-20,
+// ntt_primes.h); Montgomery reduction uses R = 2^32 as in include/bigint/ntt_base.h:40-71.  The
 // prime index is a template parameter so every modulus and its -P^-1 mod 2^32 are immediates in
 // the generated code.
 #pragma once
@@ -44,14 +44,15 @@ __device__ __forceinline__ T load_desc(const T* p) {
 constexpr uint32_t kNegInvP[kNumPrimes] = {kP[0] - 2u, kP[1] - 2u, kP[2] - 2u};
 constexpr uint32_t kDigitBits = 28;
 constexpr uint32_t kDigitMask = (1u << kDigitBits) - 1u;
-constexpr int kMaxLogN = 26;  // smallest 2-adic valuation of P-1 (P0 and P2: 26)
+constexpr int kMaxLogN = 26;  // smallest 2-adic valuation of P-1 (P0: 26; P1, P2: 27)
 
 static_assert(kP[0] * (kP[0] - 2u) == 0xFFFFFFFFu, "P0 Montgomery constant");
 static_assert(kP[1] * (kP[1] - 2u) == 0xFFFFFFFFu, "P1 Montgomery constant");
 static_assert(kP[2] * (kP[2] - 2u) == 0xFFFFFFFFu, "P2 Montgomery constant");
 
-// a*b*2^-32 mod P for a, b < P.  t + u*P < P*(P + 2^32) < 2^64 for all three primes, and the
-// reduced value is below 1.54*P < 2^32, so one conditional subtraction finishes it.
+// a*b*2^-32 mod P for a, b < P.  t + u*P < P*(P + 2^32) < 2^64 for all three primes (P2 is the
+// largest: 1.5e19 < 1.8e19), and the reduced value is below 1.54*P < 2^32, so one conditional
+// subtraction finishes it; r - P wraps above r exactly when r < P, so an unsigned min selects.
 template <int I>
 __device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t b) {
     constexpr uint32_t p = kP[I];
@@ -86,18 +87,32 @@ __device__ __forceinline__ uint32_t fold2p(uint32_t x) {
 template <int I>
 __device__ __forceinline__ uint32_t add_mod(uint32_t a, uint32_t b) {
     constexpr uint32_t p = kP[I];
-    const uint32_t s = a + b;  // < 2p < 2^32
-    const uint32_t s2 = s - p;
-    return s2 < s ? s2 : s;
+    if constexpr (lazy_prime(I)) {
+        const uint32_t s = a + b;  // < 2p < 2^32
+        const uint32_t s2 = s - p;
+        return s2 < s ? s2 : s;
+    } else {
+        // a + b may pass 2^32: subtract P when the sum carried out of 32 bits or is >= P
+        uint32_t s, d;
+        const bool carry = __builtin_add_overflow(a, b, &s);
+        const bool borrow = __builtin_sub_overflow(s, p, &d);
+        return (carry || !borrow) ? d : s;
+    }
 }
 
 // (a - b) mod P for a, b < P.
 template <int I>
 __device__ __forceinline__ uint32_t sub_mod(uint32_t a, uint32_t b) {
     constexpr uint32_t p = kP[I];
-    const uint32_t d = a - b;
-    const uint32_t d2 = d + p;  // < 2p < 2^32 when no borrow
-    return d2 < d ? d2 : d;
+    if constexpr (lazy_prime(I)) {
+        const uint32_t d = a - b;
+        const uint32_t d2 = d + p;  // < 2p < 2^32 when no borrow
+        return d2 < d ? d2 : d;
+    } else {
+        uint32_t d;
+        const bool borrow = __builtin_sub_overflow(a, b, &d);
+        return borrow ? d + p : d;  // d + p wraps back into [0, P) after a borrow
+    }
 }
 
 }  // namespace bi

@@ -145,10 +145,11 @@ __device__ __forceinline__ void group_stages(uint32_t (&x)[kE], const uint32_t* 
                     }
                 } else if (!INV) {
                     const uint32_t vw = mont_mul<I>(v, w);
-                    // P1, P2 < 2^31: outputs that are the multiplied operand ("v") of the next
-                    // stage in this register group stay in [0, 2P) (Montgomery accepts any 32-bit
-                    // input); the rest, and every output of a group's last stage, are reduced.
-                    constexpr bool kLazy = (int)hi + 1 < ST;
+                    // P1 < 2^31: outputs that are the multiplied operand ("v") of the next stage
+                    // in this register group stay in [0, 2P) (Montgomery accepts any 32-bit
+                    // input with P < 2^31); the rest, and every output of a group's last stage,
+                    // are reduced.  P2 > 2^31: every output reduced.
+                    constexpr bool kLazy = lazy_prime(I) && (int)hi + 1 < ST;
                     if (kLazy && (q & (h >> 1)) != 0) {
                         x[sg * EG + q] = u + vw;
                         x[sg * EG + q + h] = u + kP[I] - vw;
@@ -158,8 +159,12 @@ __device__ __forceinline__ void group_stages(uint32_t (&x)[kE], const uint32_t* 
                     }
                 } else {
                     x[sg * EG + q] = add_mod<I>(u, v);
-                    // u - v + P < 2P < 2^32 goes to Montgomery unreduced
-                    x[sg * EG + q + h] = mont_mul<I>(u + kP[I] - v, w);
+                    if constexpr (lazy_prime(I)) {
+                        // u - v + P < 2P < 2^32 goes to Montgomery unreduced
+                        x[sg * EG + q + h] = mont_mul<I>(u + kP[I] - v, w);
+                    } else {
+                        x[sg * EG + q + h] = mont_mul<I>(sub_mod<I>(u, v), w);
+                    }
                 }
             }
         }
@@ -207,7 +212,8 @@ __device__ __forceinline__ uint32_t limb_word(const uint32_t* __restrict__ limbs
 }
 
 // A 32-bit input word as a residue: below P for P1 and P2, below 4P for P0 (the range of the
-// lazy forward butterflies).  2^32 < 9.2 P0 < 2.2 P1 < 2.4 P2.
+// lazy forward butterflies).  2^32 < 9.2 P0 < 2.2 P1 < 1.9 P2.  Each step subtracts when that
+// does not wrap (x - m wraps above x exactly when x < m).
 template <int I>
 __device__ __forceinline__ uint32_t word_residue(uint32_t x) {
     if constexpr (I == 0) {
@@ -216,11 +222,14 @@ __device__ __forceinline__ uint32_t word_residue(uint32_t x) {
         x = y < x ? y : x;  // < 5.2 P0
         y = x - p4;
         return y < x ? y : x;  // < 4 P0
-    } else {
+    } else if constexpr (lazy_prime(I)) {
         constexpr uint32_t p2 = 2u * kP[I];
         uint32_t y = x - p2;
         x = y < x ? y : x;  // < 2 P
         y = x - kP[I];
+        return y < x ? y : x;
+    } else {
+        const uint32_t y = x - kP[I];  // x < 2^32 < 2 P
         return y < x ? y : x;
     }
 }
@@ -575,7 +584,7 @@ __global__ __launch_bounds__(1 << (LR - 4)) void k_mid_pass(PassArgs a) {
 // Garner CRT + carry propagation (decoupled look-back).
 //
 // Coefficient j of the convolution of the operands' 32-bit words is recovered from the three
-// inverse-transform outputs y_p (which carry a factor n * R^-1) as x_j < 2^89 (at most 2^25 terms
+// inverse-transform outputs y_p (which carry a factor n * R^-1) as x_j < 2^90 (at most 2^26 terms
 // below 2^64 each; P0 P1 P2 > 2^90.8) and split into 32-bit pieces w0, w1, w2.  Output word i of
 // the product is w0_i + w1_{i-1} + w2_{i-2} + carry_i: each sum is below 3 * 2^32, so carry_i is
 // in {0, 1, 2} and the carry-in -> carry-out behaviour of any run of words is a map
@@ -612,11 +621,11 @@ __device__ __forceinline__ unsigned __int128 crt_coeff(uint32_t r0, uint32_t r1,
     const uint32_t c1 = mont_mul<1>(r1, ks[1]);
     const uint32_t c2 = mont_mul<2>(r2, ks[2]);
     const uint32_t v0 = c0;
+    // v0 < P0 < P1 < P2 and v1 < P1 < P2 are already residues of the larger primes
+    static_assert(kP[0] < kP[1] && kP[1] < kP[2], "CRT order");
     const uint32_t v1 = mont_mul<1>(sub_mod<1>(c1, v0), k.inv01);
     const uint32_t t = mont_mul<2>(sub_mod<2>(c2, v0), k.inv02);
-    static_assert(kP[1] < 2u * kP[2] && kP[0] < kP[1] && kP[0] < kP[2], "CRT folds");
-    const uint32_t v1r = v1 >= kP[2] ? v1 - kP[2] : v1;  // v1 < P1 < 2 P2, as a residue mod P2
-    const uint32_t v2 = mont_mul<2>(sub_mod<2>(t, v1r), k.inv12);
+    const uint32_t v2 = mont_mul<2>(sub_mod<2>(t, v1), k.inv12);
     const uint64_t inner = (uint64_t)v1 + (uint64_t)kP[1] * v2;  // < 2^62.1
     return (unsigned __int128)inner * kP[0] + v0;
 }

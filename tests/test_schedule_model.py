@@ -13,10 +13,14 @@ import pytest
 
 from bigint_amd import limbs as L
 
-# The device primes: the reference's first two, and 1811939329 = 27 * 2^26 + 1 in place of its
-# 2281701377 so that every prime is below 2^31.
-P = [469762049, 2013265921, 1811939329]
-G = [3, 31, 13]
+# The device primes and generators: the reference's (include/bigint/ntt_base.h:14-16).  P0 and P1
+# are below 2^31 (room for lazy butterfly values); P2 = 2281701377 is above it.
+P = [469762049, 2013265921, 2281701377]
+G = [3, 31, 3]
+
+
+def lazy_prime(I):
+    return P[I] < (1 << 31)
 R32 = 1 << 32
 KE, KTILE = 16, 4096
 KLDS = KTILE + KTILE // 16
@@ -166,8 +170,12 @@ def word_residue(I, x):
     if I == 0:
         x = csub(csub(x, 4 * P[0]), 4 * P[0])
         assert x < 4 * P[0]
-    else:
+    elif lazy_prime(I):
         x = csub(csub(x, 2 * P[I]), P[I])
+        assert x < P[I]
+    else:
+        assert x < 2 * P[I]  # 32-bit word below 2 P2
+        x = csub(x, P[I])
         assert x < P[I]
     return x % P[I]
 
@@ -570,15 +578,32 @@ def k_mont(I, a, b, lazy=False):
 
 
 def k_add_mod(I, a, b):
+    """Kernel add_mod in 32-bit words: for P < 2^31 the sum fits and an unsigned min of s and
+    s - P selects; for P2 the carry out of 32 bits and the borrow of s - P decide."""
     assert 0 <= a < P[I] and 0 <= b < P[I]
-    s = a + b
-    assert s <= M32
-    return s - P[I] if s >= P[I] else s
+    if lazy_prime(I):
+        s = a + b
+        assert s <= M32
+        s2 = (s - P[I]) & M32
+        return min(s, s2)
+    s = (a + b) & M32
+    carry = a + b > M32
+    d = (s - P[I]) & M32
+    borrow = s < P[I]
+    r = d if (carry or not borrow) else s
+    assert r == (a + b) % P[I]
+    return r
 
 
 def k_sub_mod(I, a, b):
     assert 0 <= a < P[I] and 0 <= b < P[I]
-    return a - b if a >= b else a - b + P[I]
+    if lazy_prime(I):
+        d = (a - b) & M32
+        return min(d, (d + P[I]) & M32)
+    d = (a - b) & M32
+    r = (d + P[I]) & M32 if a < b else d
+    assert r == (a - b) % P[I]
+    return r
 
 
 def k_fold2p(x):
@@ -588,10 +613,11 @@ def k_fold2p(x):
 
 def lazy_group(I, inv, ST, x, rng):
     """Kernel group_stages on one register group of 2^ST values with its lazy reductions
-    (P0: Harvey butterflies, forward values in [0, 4P), inverse in [0, 2P); P1 and P2: forward
+    (P0: Harvey butterflies, forward values in [0, 4P), inverse in [0, 2P); P1: forward
     outputs that feed the next stage's Montgomery operand stay in [0, 2P), the inverse difference
-    goes to Montgomery as u - v + P).  Every intermediate must fit 32 bits; the values are checked
-    against the exact butterflies mod P."""
+    goes to Montgomery as u - v + P; P2 > 2^31: every value reduced, additions with a carry test).
+    Every intermediate must fit 32 bits; the values are checked against the exact butterflies
+    mod P."""
     p = P[I]
     rinv = pow(R32, -1, p)
     exact = [v % p for v in x]
@@ -621,7 +647,7 @@ def lazy_group(I, inv, ST, x, rng):
                     x[q + h] = k_mont(0, u + 2 * p - v, w, lazy=True)
             elif not inv:
                 vw = k_mont(I, v, w)
-                if hi + 1 < ST and (q & (h >> 1)):
+                if lazy_prime(I) and hi + 1 < ST and (q & (h >> 1)):
                     assert u < p
                     x[q], x[q + h] = u + vw, u + p - vw
                     assert x[q] < 2 * p <= M32 and x[q + h] < 2 * p
@@ -629,7 +655,10 @@ def lazy_group(I, inv, ST, x, rng):
                     x[q], x[q + h] = k_add_mod(I, u, vw), k_sub_mod(I, u, vw)
             else:
                 x[q] = k_add_mod(I, u, v)
-                x[q + h] = k_mont(I, u + p - v, w)
+                if lazy_prime(I):
+                    x[q + h] = k_mont(I, u + p - v, w)
+                else:
+                    x[q + h] = k_mont(I, k_sub_mod(I, u, v), w)
             assert x[q] % p == exact[q] and x[q + h] % p == exact[q + h]
     return x
 
@@ -653,6 +682,8 @@ def test_word_residue_and_pointwise_ranges(I):
     forward outputs lands in an inverse group's input range."""
     p = P[I]
     for x in [0, 1, p - 1, p, 2 * p - 1, 2 * p, 4 * p - 1 if I == 0 else M32, M32]:
+        if x > M32:  # inputs are 32-bit words
+            continue
         r = word_residue(I, x)
         assert r == x % p
     fwd_top = 4 * p if I == 0 else p
@@ -681,8 +712,8 @@ def kernel_crt(r0, r1, r2, n):
     v0 = k_mont(0, r0, ks[0])
     v1 = k_mont(1, k_sub_mod(1, k_mont(1, r1, ks[1]), v0), inv01)
     t = k_mont(2, k_sub_mod(2, k_mont(2, r2, ks[2]), v0), inv02)
-    v1r = v1 - P[2] if v1 >= P[2] else v1
-    v2 = k_mont(2, k_sub_mod(2, t, v1r), inv12)
+    assert v1 < P[1] < P[2]  # already a residue mod P2
+    v2 = k_mont(2, k_sub_mod(2, t, v1), inv12)
     x = v0 + P[0] * (v1 + P[1] * v2)
     assert x < P[0] * P[1] * P[2]
     return x
