@@ -111,14 +111,29 @@ __device__ __forceinline__ const uint32_t* block_pows(const uint32_t* table, uin
 
 // Forward (Cooley-Tukey) or inverse (Gentleman-Sande) stages of one register group.
 // Butterfly with lower row r at the stage with half-span 2^lt uses tw[2^(LR-lt-1) + (r >> (lt+1))].
-template <int I, bool INV, int LR, int LC, int ST, int TL>
-__device__ __forceinline__ void group_stages(uint32_t (&x)[kE], const uint32_t* __restrict__ tw) {
+// SKIP0 with skip0 set (forward only): the upper operand of every butterfly of the group's first
+// stage is zero, so that stage leaves (u, u): a copy instead of butterflies.
+template <int I, bool INV, int LR, int LC, int ST, int TL, bool SKIP0 = false>
+__device__ __forceinline__ void group_stages(uint32_t (&x)[kE], const uint32_t* __restrict__ tw,
+                                             bool skip0 = false) {
     using Ly = Layout<LR, LC, ST, TL>;
     constexpr int EG = Ly::EG;
     static_for<ST>([&](auto hi) {
         constexpr int lh = INV ? (int)hi : (ST - 1 - (int)hi);  // log2 of the span in q units
         constexpr int h = 1 << lh;
         constexpr int lt = TL + lh;
+        if constexpr (SKIP0 && !INV && (int)hi == 0) {
+            if (skip0) {
+                // u + 0 * w and u - 0 * w: both outputs are u, which lies in the next stage's
+                // input range for every prime (below 4P for P0, below P for P1 and P2)
+#pragma unroll
+                for (int sg = 0; sg < Ly::NSG; ++sg)
+#pragma unroll
+                    for (int q = 0; q < EG; ++q)
+                        if (!(q & h)) x[sg * EG + q + h] = x[sg * EG + q];
+                return;
+            }
+        }
 #pragma unroll
         for (int sg = 0; sg < Ly::NSG; ++sg) {
             const uint32_t rhv = Ly::rh(sg);
@@ -247,6 +262,60 @@ struct LimbSrc {
     uint32_t nlimbs, off, cnt;
 };
 
+// Input words of a forward transform cut from limbs, in a group layout: element idx of the tile
+// (row r, column c: idx = tile_base + c + (r << log_s)) is word src.off + idx of the operand, zero
+// at and past src.cnt.  HALF: only the tile's lower half rows are read (in group 0's layout they
+// are the elements q < EG / 2); the upper half is known to be zero.
+template <int LR, int LC, int ST, int TL, bool HALF>
+__device__ __forceinline__ void load_words(const LimbSrc& src, uint32_t (&x)[kE], uint32_t tile_base,
+                                           uint32_t log_s) {
+    using Ly = Layout<LR, LC, ST, TL>;
+    constexpr int QN = HALF ? Ly::EG / 2 : Ly::EG;
+    // Every limb load is issued before any word is cut, so the thread waits for memory once
+    // (with the branch inside the element loop each word waited for its own two loads).
+    // Limb indices are below 2^32; the loads index with them as 64-bit offsets.
+    uint32_t lo[kE], hi[kE], g[kE];
+    const uint32_t last = src.nlimbs - 1u;
+#pragma unroll
+    for (int sg = 0; sg < Ly::NSG; ++sg) {
+        const uint32_t c = Ly::col(sg);
+#pragma unroll
+        for (int q = 0; q < QN; ++q) {
+            const int e = sg * Ly::EG + q;
+            const uint32_t idx = tile_base + c + (Ly::row(sg, q) << log_s);
+            g[e] = idx < src.cnt ? src.off + idx : ~0u;  // ~0u: past the source, word 0
+            const uint32_t k = word_limb(g[e] == ~0u ? 0u : g[e]);
+            const uint32_t k0 = k < last ? k : last, k1 = k + 1u < last ? k + 1u : last;
+            lo[e] = gmem(src.limbs)[(size_t)k0];
+            hi[e] = gmem(src.limbs)[(size_t)k1];
+        }
+    }
+#pragma unroll
+    for (int sg = 0; sg < Ly::NSG; ++sg) {
+#pragma unroll
+        for (int q = 0; q < Ly::EG; ++q) {
+            const int e = sg * Ly::EG + q;
+            if (q >= QN) {
+                x[e] = 0u;
+                continue;
+            }
+            // bits [32g, 32g + 32) lie in limbs k = floor(32g / 28) and k + 1 from bit
+            // o = 32g - 28k of limb k; limbs at and past nlimbs read as zero
+            const uint32_t gg = g[e] == ~0u ? 0u : g[e];
+            const uint32_t k = word_limb(gg), o = word_shift(gg, k);
+            const uint32_t l = k < src.nlimbs ? lo[e] : 0u, h = k + 1u < src.nlimbs ? hi[e] : 0u;
+            x[e] = g[e] == ~0u ? 0u : ((l >> o) | (h << (28u - o)));
+        }
+    }
+}
+
+// Words of the operand that can be nonzero: the source's words past its offset, at most cnt.
+__device__ __forceinline__ uint32_t source_words(const LimbSrc& src) {
+    const uint32_t w = words_of(src.nlimbs);
+    const uint32_t avail = w > src.off ? w - src.off : 0u;
+    return avail < src.cnt ? avail : src.cnt;
+}
+
 // Global memory <-> registers in a group layout.  Element (r, c) of the tile is at
 // tile_base + c + (r << log_s) of the plane; with a limb source, element idx is input word
 // src.off + idx (zero at and past src.cnt).
@@ -255,34 +324,7 @@ __device__ __forceinline__ void global_io(uint32_t* __restrict__ plane, const Li
                                           uint32_t (&x)[kE], uint32_t tile_base, uint32_t log_s) {
     using Ly = Layout<LR, LC, ST, TL>;
     if (!STORE && src.limbs) {
-        // Every limb load is issued before any word is cut, so the thread waits for memory once
-        // (with the branch inside the element loop each word waited for its own two loads).
-        // Limb indices are below 2^32; the loads index with them as 64-bit offsets.
-        uint32_t lo[kE], hi[kE], g[kE];
-        const uint32_t last = src.nlimbs - 1u;
-#pragma unroll
-        for (int sg = 0; sg < Ly::NSG; ++sg) {
-            const uint32_t c = Ly::col(sg);
-#pragma unroll
-            for (int q = 0; q < Ly::EG; ++q) {
-                const int e = sg * Ly::EG + q;
-                const uint32_t idx = tile_base + c + (Ly::row(sg, q) << log_s);
-                g[e] = idx < src.cnt ? src.off + idx : ~0u;  // ~0u: past the source, word 0
-                const uint32_t k = word_limb(g[e] == ~0u ? 0u : g[e]);
-                const uint32_t k0 = k < last ? k : last, k1 = k + 1u < last ? k + 1u : last;
-                lo[e] = gmem(src.limbs)[(size_t)k0];
-                hi[e] = gmem(src.limbs)[(size_t)k1];
-            }
-        }
-#pragma unroll
-        for (int e = 0; e < kE; ++e) {
-            // bits [32g, 32g + 32) lie in limbs k = floor(32g / 28) and k + 1 from bit
-            // o = 32g - 28k of limb k; limbs at and past nlimbs read as zero
-            const uint32_t gg = g[e] == ~0u ? 0u : g[e];
-            const uint32_t k = word_limb(gg), o = word_shift(gg, k);
-            const uint32_t l = k < src.nlimbs ? lo[e] : 0u, h = k + 1u < src.nlimbs ? hi[e] : 0u;
-            x[e] = g[e] == ~0u ? 0u : ((l >> o) | (h << (28u - o)));
-        }
+        load_words<LR, LC, ST, TL, false>(src, x, tile_base, log_s);
         return;
     }
 #pragma unroll
@@ -298,10 +340,11 @@ __device__ __forceinline__ void global_io(uint32_t* __restrict__ plane, const Li
 }
 
 // Whole tile transform.  Forward: x arrives in group 0's layout and leaves in the last group's;
-// inverse: the reverse (and scales by 2^LR).
-template <int I, bool INV, int LR, int LC>
+// inverse: the reverse (and scales by 2^LR).  SKIP0 / skip0: the tile's upper half rows are zero
+// (see group_stages).
+template <int I, bool INV, int LR, int LC, bool SKIP0 = false>
 __device__ __forceinline__ void tile_transform(uint32_t (&x)[kE], uint32_t* lds,
-                                               const uint32_t* tw) {
+                                               const uint32_t* tw, bool skip0 = false) {
     using S = Sched<LR>;
     static_for<S::NG>([&](auto step) {
         constexpr int gi = INV ? (S::NG - 1 - (int)step) : (int)step;
@@ -312,7 +355,7 @@ __device__ __forceinline__ void tile_transform(uint32_t (&x)[kE], uint32_t* lds,
             __syncthreads();
             lds_io<LR, LC, S::stages(gi), S::trlo(gi), false>(lds, x);
         }
-        group_stages<I, INV, LR, LC, S::stages(gi), S::trlo(gi)>(x, tw);
+        group_stages<I, INV, LR, LC, S::stages(gi), S::trlo(gi), SKIP0 && !INV && (int)step == 0>(x, tw, skip0);
     });
 }
 
@@ -378,8 +421,13 @@ __device__ __forceinline__ void outer_first_body(const PassArgs& a, const PassJo
                                 : LimbSrc{job.limbs_b, job.lb, 0u, ~0u};
     constexpr LimbSrc kPlane{nullptr, 0u, 0u, 0u};
     constexpr int G0 = 0, GL = S::NG - 1;
+    // The first stage pairs rows r and r + R/2 (elements idx and idx + n/2).  When every upper
+    // row of the tile lies past the operand (an operand filling at most half the transform, the
+    // usual case), those rows are not read and the first stage is a copy.
+    const bool half = tile_base + (1u << (LR - 1 + a.log_s)) >= source_words(src);
     uint32_t words[kE];
-    global_io<LR, LC, S::stages(G0), S::trlo(G0), false>(nullptr, src, words, tile_base, a.log_s);
+    if (half) load_words<LR, LC, S::stages(G0), S::trlo(G0), true>(src, words, tile_base, a.log_s);
+    else load_words<LR, LC, S::stages(G0), S::trlo(G0), false>(src, words, tile_base, a.log_s);
     // The three primes' twiddle tables (3 x 2^LR words) are built at once, overlapping the limb
     // loads, so no prime waits for its table behind a barrier.
     static_for<kNumPrimes>([&](auto pi) {
@@ -395,7 +443,7 @@ __device__ __forceinline__ void outer_first_body(const PassArgs& a, const PassJo
         uint32_t x[kE];
 #pragma unroll
         for (int k = 0; k < kE; ++k) x[k] = word_residue<I>(words[k]);
-        tile_transform<I, false, LR, LC>(x, lds, tw + (I << LR));
+        tile_transform<I, false, LR, LC, true>(x, lds, tw + (I << LR), half);
         uint32_t* plane = (op == 0 ? job.res_a : job.res_b) + (size_t)I * n;
         global_io<LR, LC, S::stages(GL), S::trlo(GL), true>(plane, kPlane, x, tile_base, a.log_s);
     });

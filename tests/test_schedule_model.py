@@ -835,3 +835,49 @@ def test_device_add_model(lx, ly):
             assert val(out) == want
             assert swap == (sub and val(y) > val(x))
             assert n == (want.bit_length() + 27) // 28  # 0 for zero
+
+
+def _source_words(nlimbs, off, cnt):
+    w = words_of(nlimbs)
+    return min(max(w - off, 0), cnt)
+
+
+@pytest.mark.parametrize("nlimbs,off,cnt,log_n,log_r", [
+    (9586981 // 64, 0, M32, 18, 7),          # a 2^22-bit operand on its 2^18-point transform
+    (4682, 0, M32, 13, 6),                    # 4097 words: one word into the upper half
+    (2341, 0, M32, 12, 4),                    # 2049 words on 2^12 points (the wrapped case)
+    (1000, 0, M32, 12, 8), (30, 0, M32, 6, 2), (5, 0, M32, 4, 1),
+    (60000, 3 * 4096, 4096, 13, 5),           # a chunk of a longer operand
+    (60000, 13 * 4096, 4096, 13, 7),          # the last, partial chunk
+])
+def test_first_pass_half_skip_reads_only_zero_rows(nlimbs, off, cnt, log_n, log_r):
+    """The first forward pass skips the upper half rows of a tile (and replaces the first stage by
+    a copy) when tile_base + (R/2) S >= the source's word count.  Replayed over every tile: each
+    skipped element really is a zero input word, and the copy equals the butterfly it replaces."""
+    rng = random.Random(nlimbs + off)
+    limbs = [rng.randrange(1 << 28) for _ in range(nlimbs)]
+    limbs[-1] |= 1 << 27
+    log_s = log_n - log_r
+    log_c = 12 - log_r
+    if log_c > log_s:
+        pytest.skip("tile wider than the row stride")
+    sw = _source_words(nlimbs, off, cnt)
+    skipped = 0
+    for t in range(1 << (log_n - 12)):
+        tile_base = t << log_c          # first pass: one block, column groups
+        half = tile_base + (1 << (log_r - 1 + log_s)) >= sw
+        if not half:
+            continue
+        skipped += 1
+        for r in range(1 << (log_r - 1), 1 << log_r):
+            for c in range(1 << log_c):
+                idx = tile_base + c + (r << log_s)
+                assert idx < (1 << log_n)
+                word = limb_word(limbs, off + idx) if idx < cnt else 0
+                assert word == 0, (t, r, c)
+    assert skipped > 0 or sw > (1 << (log_n - 1))
+    # the copy is the butterfly with v = 0 for every prime and twiddle
+    for I in range(3):
+        for u in [0, 1, P[I] - 1]:
+            for w in [to_mont(I, 1), to_mont(I, 12345)]:
+                assert (u + k_mont(I, 0, w)) % P[I] == u and (u - k_mont(I, 0, w)) % P[I] == u
