@@ -64,8 +64,7 @@ uint32_t root(int I, uint32_t k, bool inv) {
 struct OuterPass {
     uint32_t log_r, log_c, log_s;
     uint32_t nblk;
-    uint32_t* beta = nullptr;      // device [3][nblk] (null when nblk == 1)
-    uint32_t* beta_inv = nullptr;
+    uint32_t* beta = nullptr;      // device [3][nblk] (null when nblk == 1), both directions
 };
 
 struct Plan {
@@ -73,45 +72,35 @@ struct Plan {
     std::vector<OuterPass> outer;  // forward order
     uint32_t mid_log_r = 0, mid_nblk = 1;
     uint32_t* mid_beta = nullptr;
-    uint32_t* mid_beta_inv = nullptr;
     uint32_t kscale[3] = {0, 0, 0};
 };
 
 // Per-block roots of a pass whose tiles have 2^log_r rows: for block b,
 // beta_b = w_{n/S}^{bitrev_{log nblk}(b)}, stored as beta_b^(2^e) for e < log_r (Montgomery form,
 // kBetaPowStride words per block), so a tile's twiddles tw[k] = W[k] * beta^tr need one product
-// each and no squaring chain on the device.
-hipError_t upload_beta(uint32_t log_n, uint32_t log_r, uint32_t log_s, uint32_t nblk,
-                       uint32_t** fwd, uint32_t** inv) {
-    *fwd = *inv = nullptr;
+// each and no squaring chain on the device.  The inverse stages use the same forward roots.
+hipError_t upload_beta(uint32_t log_n, uint32_t log_r, uint32_t log_s, uint32_t nblk, uint32_t** fwd) {
+    *fwd = nullptr;
     if (nblk <= 1) return hipSuccess;
     if (log_r > kBetaPowStride) return hipErrorInvalidValue;
     const size_t stride = kBetaPowStride;
-    std::vector<uint32_t> hf(3 * (size_t)nblk * stride, 0u), hi(3 * (size_t)nblk * stride, 0u);
+    std::vector<uint32_t> hf(3 * (size_t)nblk * stride, 0u);
     const uint32_t lb = 31u - (uint32_t)__builtin_clz(nblk);
     for (int I = 0; I < 3; ++I) {
         const uint32_t p = kPh[I];
-        const uint32_t wf = root(I, log_n - log_s, false), wi = root(I, log_n - log_s, true);
+        const uint32_t wf = root(I, log_n - log_s, false);
         for (uint32_t b = 0; b < nblk; ++b) {
-            const uint32_t e = bitrev(b, lb);
-            uint32_t bf = powmod(wf, e, p), bi = powmod(wi, e, p);
+            uint32_t bf = powmod(wf, bitrev(b, lb), p);
             uint32_t* of = &hf[((size_t)I * nblk + b) * stride];
-            uint32_t* oi = &hi[((size_t)I * nblk + b) * stride];
             for (uint32_t k = 0; k < log_r; ++k) {
                 of[k] = to_mont(bf, p);
-                oi[k] = to_mont(bi, p);
                 bf = mulmod(bf, bf, p);
-                bi = mulmod(bi, bi, p);
             }
         }
     }
     hipError_t e = hipMalloc((void**)fwd, hf.size() * 4);
     if (e != hipSuccess) return e;
-    e = hipMalloc((void**)inv, hi.size() * 4);
-    if (e != hipSuccess) return e;
-    e = hipMemcpy(*fwd, hf.data(), hf.size() * 4, hipMemcpyHostToDevice);
-    if (e != hipSuccess) return e;
-    return hipMemcpy(*inv, hi.data(), hi.size() * 4, hipMemcpyHostToDevice);
+    return hipMemcpy(*fwd, hf.data(), hf.size() * 4, hipMemcpyHostToDevice);
 }
 
 struct ProfRec {
@@ -295,7 +284,7 @@ public:
     bool ready = false;
     int device = -1;
     hipStream_t stream = nullptr;
-    uint32_t* wtab = nullptr;  // [2][3][4096]
+    uint32_t* wtab = nullptr;  // [3][4096]
     GarnerConsts gk{};
     std::map<uint32_t, std::unique_ptr<Plan>> plans;
     // workspace (grow-only)
@@ -314,19 +303,19 @@ public:
         device = dev;
         if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
             return BI_HIP_EDEVICE;
-        // root table W[m + i] = w_{2m}^{bitrev_{log m}(i)}, Montgomery form
-        std::vector<uint32_t> h(2 * 3 * (size_t)kTileH);
-        for (int dir = 0; dir < 2; ++dir)
-            for (int I = 0; I < 3; ++I) {
-                uint32_t* t = h.data() + ((size_t)dir * 3 + I) * kTileH;
-                t[0] = to_mont(1, kPh[I]);
-                for (uint32_t lm = 0; (1u << lm) < (uint32_t)kTileH; ++lm) {
-                    const uint32_t m = 1u << lm;
-                    const uint32_t w = root(I, lm + 1, dir == 1);
-                    for (uint32_t i = 0; i < m; ++i)
-                        t[m + i] = to_mont(powmod(w, bitrev(i, lm), kPh[I]), kPh[I]);
-                }
+        // root table W[m + i] = w_{2m}^{bitrev_{log m}(i)}, Montgomery form, one per prime; the
+        // inverse stages use it too (their output order is mirrored, which Garner undoes)
+        std::vector<uint32_t> h(3 * (size_t)kTileH);
+        for (int I = 0; I < 3; ++I) {
+            uint32_t* t = h.data() + (size_t)I * kTileH;
+            t[0] = to_mont(1, kPh[I]);
+            for (uint32_t lm = 0; (1u << lm) < (uint32_t)kTileH; ++lm) {
+                const uint32_t m = 1u << lm;
+                const uint32_t w = root(I, lm + 1, false);
+                for (uint32_t i = 0; i < m; ++i)
+                    t[m + i] = to_mont(powmod(w, bitrev(i, lm), kPh[I]), kPh[I]);
             }
+        }
         if (hipMalloc((void**)&wtab, h.size() * 4) != hipSuccess) return BI_HIP_ENOMEM;
         if (hipMemcpy(wtab, h.data(), h.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
             return BI_HIP_EDEVICE;
@@ -389,14 +378,14 @@ public:
             op.log_c = 12 - r;
             op.log_s = log_n - done - r;
             op.nblk = 1u << done;
-            if (upload_beta(log_n, r, op.log_s, op.nblk, &op.beta, &op.beta_inv) != hipSuccess)
+            if (upload_beta(log_n, r, op.log_s, op.nblk, &op.beta) != hipSuccess)
                 return nullptr;
             pl->outer.push_back(op);
             done += r;
         }
         pl->mid_log_r = mid_lr;
         pl->mid_nblk = n >> mid_lr;
-        if (upload_beta(log_n, mid_lr, 0, pl->mid_nblk, &pl->mid_beta, &pl->mid_beta_inv) != hipSuccess)
+        if (upload_beta(log_n, mid_lr, 0, pl->mid_nblk, &pl->mid_beta) != hipSuccess)
             return nullptr;
         for (int I = 0; I < 3; ++I) {
             const uint32_t p = kPh[I];
@@ -488,7 +477,6 @@ public:
             a.log_s = op.log_s;
             a.flags = base_flags | (k == 0 ? (uint32_t)kFromLimbs : 0u);
             a.beta = op.beta;
-            a.beta_inv = nullptr;
             a.beta_stride = op.nblk;
             // the first pass reads the limbs once; every pass writes 3 planes per operand
             const double bytes = k == 0 ? 4.0 * in_limbs + 12.0 * n * nops_d * nj : 24.0 * n * nops_d * nj;
@@ -505,7 +493,6 @@ public:
                   (kind == kInverse ? (uint32_t)kInvOnly : 0u) |
                   (pl->outer.empty() && fwd_passes ? (uint32_t)kFromLimbs : 0u);
         a.beta = pl->mid_beta;
-        a.beta_inv = pl->mid_beta_inv;
         a.beta_stride = pl->mid_nblk;
         {
             double in_bytes = pl->outer.empty() && fwd_passes ? 4.0 * in_limbs : 12.0 * n * nops_d * nj;
@@ -521,8 +508,7 @@ public:
             a.log_c = op.log_c;
             a.log_s = op.log_s;
             a.flags = 0;
-            a.beta = op.beta_inv;
-            a.beta_inv = nullptr;
+            a.beta = op.beta;  // forward roots (see coeff_pos in the kernels)
             a.beta_stride = op.nblk;
             if (timed(BI_HIP_K_OUTER_INV, 24.0 * n * nj, st,
                       [&] { return launch_outer_pass(a, true, njobs, 1, st); }) != hipSuccess)

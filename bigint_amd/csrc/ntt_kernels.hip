@@ -20,7 +20,9 @@
 //     three-way overlapped carry).
 //
 // Twiddle convention: W[m + i] = w_{2m}^{bitrev_{log2 m}(i)} (Montgomery form), used by the
-// butterfly of block i at the stage with m blocks.  Data stay in plain residue form: a Montgomery
+// butterfly of block i at the stage with m blocks, in the forward and in the inverse stages (the
+// inverse then computes the transform with root w, which leaves coefficient j at position
+// (n - j) mod n: see coeff_pos).  Data stay in plain residue form: a Montgomery
 // product with a Montgomery-form twiddle is an ordinary modular product.  The pointwise product
 // is a Montgomery product, so the inverse output carries n * R^-1, removed in Garner.
 #include <hip/hip_runtime.h>
@@ -376,7 +378,8 @@ __device__ __forceinline__ void outer_body(const PassArgs& a, const PassJob& job
     const uint32_t cg = t & ((1u << log_ncg) - 1u);
     const uint32_t tile_base = (blk << (LR + a.log_s)) + (cg << LC);
 
-    const uint32_t* wt = a.wtab + ((INV ? kNumPrimes : 0) + I) * kTile;
+    // both directions use the forward roots (see coeff_pos)
+    const uint32_t* wt = a.wtab + I * kTile;
     build_twiddles<I, LR>(tw, wt, block_pows<I>(a.beta, a.beta_stride, blk), kThreads);
 
     uint32_t* plane = (op == 0 ? job.res_a : job.res_b) + (size_t)I * n;
@@ -493,12 +496,11 @@ __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, 
     const uint32_t n = 1u << a.log_n;
     const uint32_t blk = blockIdx.x;
     const uint32_t tile_base = blk << LR;
+    // one twiddle table per tile serves the forward and the inverse stages (see coeff_pos)
     const uint32_t* wf = a.wtab + I * kTile;
-    const uint32_t* wi = a.wtab + (kNumPrimes + I) * kTile;
     const uint32_t* bf = block_pows<I>(a.beta, a.beta_stride, blk);
-    const uint32_t* bi_ = block_pows<I>(a.beta_inv, a.beta_stride, blk);
     if (a.flags & kInvOnly) {  // a product spectrum from the diagonal sums: inverse stages only
-        build_twiddles<I, LR>(tw, wi, bi_, T);
+        build_twiddles<I, LR>(tw, wf, bf, T);
         using Sx = Sched<LR>;
         constexpr LimbSrc kPl{nullptr, 0u, 0u, 0u};
         uint32_t* p = job.res_a + (size_t)I * n;
@@ -544,10 +546,6 @@ __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, 
 #pragma unroll
         for (int k = 0; k < kE; ++k) xa[k] = pointwise<I>(xa[k], xa[k]);
     }
-    // the same LDS table now receives the inverse twiddles
-    __syncthreads();
-    build_twiddles<I, LR>(tw, wi, bi_, T);
-    __syncthreads();
     tile_transform<I, true, LR, 0>(xa, lds, tw);
     global_io<LR, 0, S::stages(G0), S::trlo(G0), true>(pa, kPlane, xa, tile_base, 0);
 }
@@ -567,8 +565,7 @@ __device__ __forceinline__ void mid_pair_body(const PassArgs& a, const PassJob& 
     const bool square = (a.flags & kSquare) != 0;
     uint32_t* pa = job.res_a + (size_t)I * n;
     uint32_t* pb = job.res_b + (size_t)I * n;
-    const uint32_t* wf = a.wtab + I * kTile;
-    const uint32_t* wi = a.wtab + (kNumPrimes + I) * kTile;
+    const uint32_t* wf = a.wtab + I * kTile;  // forward roots serve both directions
     const uint32_t blk0 = 2u * blockIdx.x;
     uint32_t xa[kE], xb[kE], na[kE];
     global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pa, kPlane, xa, blk0 << LR, 0);
@@ -595,9 +592,6 @@ __device__ __forceinline__ void mid_pair_body(const PassArgs& a, const PassJob& 
 #pragma unroll
             for (int k = 0; k < kE; ++k) xa[k] = pointwise<I>(xa[k], xa[k]);
         }
-        __syncthreads();
-        build_twiddles<I, LR>(tw, wi, block_pows<I>(a.beta_inv, a.beta_stride, blk), T);
-        __syncthreads();
         tile_transform<I, true, LR, 0>(xa, lds, tw);
         global_io<LR, 0, S::stages(G0), S::trlo(G0), true>(pa, kPlane, xa, blk << LR, 0);
     }
@@ -678,6 +672,24 @@ __device__ __forceinline__ unsigned __int128 crt_coeff(uint32_t r0, uint32_t r1,
     return (unsigned __int128)inner * kP[0] + v0;
 }
 
+// Position of convolution coefficient j (j < n) in an inverse-transform output.  The inverse
+// runs the Gentleman-Sande stages with the forward roots (the forward twiddle tables serve both
+// directions), which computes the transform with root w instead of w^-1: its output holds
+// n * x[(n - j) mod n] at position j, so coefficient j sits at position (n - j) mod n.
+__device__ __forceinline__ uint32_t coeff_pos(uint32_t j, uint32_t n) { return (n - j) & (n - 1u); }
+
+// Words wx and wx - 1 (wx even) of one plane of n inverse-transform outputs, words outside
+// [0, n) reading as zero.  Word w sits at position (n - w) mod n, so for 2 <= wx <= n - 2 the two
+// words are the aligned position pair (n - wx, n - wx + 1); word 0 is at position 0 and word n - 1
+// at position 1.
+__device__ __forceinline__ u32x2 load_pair_rev(const uint32_t* __restrict__ yp, uint32_t n, int64_t wx) {
+    if (wx >= 2 && wx <= (int64_t)n - 2) return *gmem(reinterpret_cast<const u32x2*>(yp + (n - (uint32_t)wx)));
+    u32x2 r{0u, 0u};
+    if (wx == 0) r.x = gmem(yp)[0];
+    else if (wx == (int64_t)n) r.y = gmem(yp)[1];
+    return r;
+}
+
 // The coefficient that wrapped from index n onto index 0: product of the operands' top words.
 __device__ __forceinline__ uint64_t wrap_product(const GarnerJob& jb) {
     const uint32_t ga = words_of(jb.wrap_la) - 1u, gb = words_of(jb.wrap_lb) - 1u;
@@ -698,9 +710,10 @@ __device__ __forceinline__ Pieces garner(const GarnerJob& jb, const GarnerConsts
             const uint64_t w = wrap_product(jb);
             return {(uint32_t)w, (uint32_t)(w >> 32), 0u};
         }
-        r0 = gmem(jb.y)[j];
-        r1 = gmem(jb.y)[n + j];
-        r2 = gmem(jb.y)[2u * n + j];
+        const uint32_t pj = coeff_pos(j, n);
+        r0 = gmem(jb.y)[pj];
+        r1 = gmem(jb.y)[n + pj];
+        r2 = gmem(jb.y)[2u * n + pj];
         if (j == 0 && jb.wrap_a) {
             const unsigned __int128 x = crt_coeff(r0, r1, r2, jb.kscale, k) - wrap_product(jb);
             const uint64_t lo = (uint64_t)x;
@@ -716,9 +729,10 @@ __device__ __forceinline__ Pieces garner(const GarnerJob& jb, const GarnerConsts
             if (off >= n) break;
             if (cc < jb.nchunks) {
                 const uint32_t* y = jb.y + (size_t)cc * jb.chunk_stride;
-                r0 = add_mod<0>(r0, fold1p(gmem(y)[off]));  // P0 outputs lie in [0, 2P)
-                r1 = add_mod<1>(r1, gmem(y)[n + off]);
-                r2 = add_mod<2>(r2, gmem(y)[2u * n + off]);
+                const uint32_t po = coeff_pos(off, n);
+                r0 = add_mod<0>(r0, fold1p(gmem(y)[po]));  // P0 outputs lie in [0, 2P)
+                r1 = add_mod<1>(r1, gmem(y)[n + po]);
+                r2 = add_mod<2>(r2, gmem(y)[2u * n + po]);
             }
         }
     }
@@ -789,33 +803,33 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     uint32_t r0[NQ], r1[NQ], r2[NQ];
     // residues of word base - 3, for the tile's local carry below
     uint32_t h3[3] = {0u, 0u, 0u};
-    // The block's words [base - 4, base + kWordTile) of each plane are read coalesced (lane k of
-    // a wave reads pair k) and staged in LDS one plane at a time, where each thread picks up its
-    // run of 16; read straight from memory, the runs 56 bytes apart would spread every wave's
-    // load over 28 cache lines (CRT+carry at 2^24 points: 112 -> 97 us).  Words outside every
-    // transform read as zero; a chunked product's chunk residues are summed on the way.  The
-    // output limbs go through the same LDS array.
+    // The block's words [base - 4, base + kWordTile) of each plane are read coalesced and staged
+    // in LDS one plane at a time, where each thread picks up its run of 16; read straight from
+    // memory, the runs 56 bytes apart would spread every wave's load over 28 cache lines.  Word w
+    // sits at position (n - w) mod n (coeff_pos), so aligned position pairs hold descending word
+    // pairs: pair i of the window is words base + 2i and base + 2i - 1.  Thread tid loads pairs
+    // i = k * 256 + tid (k < 7: words base - 1 .. base + 3582), threads 0, 1, 2 the window's ends
+    // (pairs -2, -1: words base - 4 .. base - 2; pair 1792: word base + 3583).  Words outside
+    // every transform read as zero; a chunked product's chunk residues are summed on the way.
+    // The output limbs go through the same LDS array.
     constexpr uint32_t kStage = kWordTile + 4u;  // 4 halo words + tile
     constexpr uint32_t kResWords = kStage > kCarryTile ? kStage : kCarryTile;
     __shared__ __align__(16) uint32_t s_res[kResWords];
     {
         constexpr uint32_t kPairs = kWordTile / (2u * kThreads);
         static_assert(kPairs * 2u * kThreads == kWordTile, "pairs per thread");
-        u32x2 v[3][kPairs], h[2] = {{0u, 0u}, {0u, 0u}};
+        // the end pair of threads 0, 1, 2: i = -2, -1, 1792
+        const int64_t ie = tid == 0u ? -2 : (tid == 1u ? -1 : (int64_t)(kWordTile / 2u));
+        const bool has_end = tid < 3u;
+        u32x2 v[3][kPairs], h[3] = {{0u, 0u}, {0u, 0u}, {0u, 0u}};
         if constexpr (!CHUNKED) {
 #pragma unroll
             for (uint32_t p = 0; p < 3u; ++p) {
                 const uint32_t* yp = jb.y + (size_t)p * n;
 #pragma unroll
-                for (uint32_t k = 0; k < kPairs; ++k) {
-                    const uint32_t w = base + 2u * (k * kThreads + tid);
-                    v[p][k] = w < n ? *gmem(reinterpret_cast<const u32x2*>(yp + w)) : u32x2{0u, 0u};
-                }
-            }
-            if (tid < 3u && base >= 4u) {  // thread p loads plane p's halo
-                const uint32_t* yp = jb.y + (size_t)tid * n;
-                if (base - 4u < n) h[0] = *gmem(reinterpret_cast<const u32x2*>(yp + base - 4u));
-                if (base - 2u < n) h[1] = *gmem(reinterpret_cast<const u32x2*>(yp + base - 2u));
+                for (uint32_t k = 0; k < kPairs; ++k)
+                    v[p][k] = load_pair_rev(yp, n, (int64_t)base + 2 * (int64_t)(k * kThreads + tid));
+                if (has_end) h[p] = load_pair_rev(yp, n, (int64_t)base + 2 * ie);
             }
         } else {
             // chunk cc covers words [cc L, cc L + n); those reaching [lo, hi] are
@@ -832,26 +846,18 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
             const uint32_t c_lo = lo >= n ? (p2 ? ((lo - n) >> jb.log_chunk) : (lo - n) / L) + 1u : 0u;
             for (uint32_t cc = c_lo; cc <= c_hi; ++cc) {
                 const uint32_t* yc = jb.y + (size_t)cc * jb.chunk_stride;
-                const uint32_t c0 = cc * L;  // first word of chunk cc
+                const int64_t c0 = (int64_t)cc * L;  // first word of chunk cc
 #pragma unroll
                 for (uint32_t p = 0; p < 3u; ++p) {
+                    const uint32_t* yp = yc + (size_t)p * n;
 #pragma unroll
                     for (uint32_t k = 0; k < kPairs; ++k) {
-                        const uint32_t w = base + 2u * (k * kThreads + tid);
-                        if (w >= c0 && w - c0 < n) {
-                            const u32x2 x = *gmem(reinterpret_cast<const u32x2*>(yc + (size_t)p * n + (w - c0)));
-                            v[p][k] = u32x2{add_residue(p, v[p][k].x, x.x), add_residue(p, v[p][k].y, x.y)};
-                        }
+                        const u32x2 x = load_pair_rev(yp, n, (int64_t)base + 2 * (int64_t)(k * kThreads + tid) - c0);
+                        v[p][k] = u32x2{add_residue(p, v[p][k].x, x.x), add_residue(p, v[p][k].y, x.y)};
                     }
-                }
-                if (tid < 3u && base >= 4u) {
-#pragma unroll
-                    for (uint32_t e = 0; e < 2u; ++e) {
-                        const uint32_t w = base - 4u + 2u * e;
-                        if (w >= c0 && w - c0 < n) {
-                            const u32x2 x = *gmem(reinterpret_cast<const u32x2*>(yc + (size_t)tid * n + (w - c0)));
-                            h[e] = u32x2{add_residue(tid, h[e].x, x.x), add_residue(tid, h[e].y, x.y)};
-                        }
+                    if (has_end) {
+                        const u32x2 x = load_pair_rev(yp, n, (int64_t)base + 2 * ie - c0);
+                        h[p] = u32x2{add_residue(p, h[p].x, x.x), add_residue(p, h[p].y, x.y)};
                     }
                 }
             }
@@ -860,13 +866,19 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
 #pragma unroll
         for (uint32_t p = 0; p < 3u; ++p) {
             if (p > 0u) __syncthreads();  // the previous plane has been read
+            // pair i: word base + 2i at staged index 2i + 4, word base + 2i - 1 at 2i + 3
 #pragma unroll
-            for (uint32_t k = 0; k < kPairs; ++k)
-                *reinterpret_cast<u32x2*>(s_res + 4u + 2u * (k * kThreads + tid)) = v[p][k];
-            if (tid == p) {
-                *reinterpret_cast<u32x2*>(s_res) = h[0];
-                *reinterpret_cast<u32x2*>(s_res + 2u) = h[1];
+            for (uint32_t k = 0; k < kPairs; ++k) {
+                const uint32_t t = 2u * (k * kThreads + tid) + 3u;
+                s_res[t] = v[p][k].y;
+                s_res[t + 1u] = v[p][k].x;
             }
+            if (tid == 0u) s_res[0] = h[p].x;  // pair -2: word base - 4 (base - 5 is outside)
+            if (tid == 1u) {                   // pair -1: words base - 2, base - 3
+                s_res[1] = h[p].y;
+                s_res[2] = h[p].x;
+            }
+            if (tid == 2u) s_res[kStage - 1u] = h[p].y;  // pair 1792: word base + 3583
             __syncthreads();
             if (tid == 0u) h3[p] = s_res[1];
             if (fast) {

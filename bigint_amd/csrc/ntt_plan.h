@@ -39,11 +39,10 @@ struct PassArgs {
     uint32_t log_r, log_c;  // tile: R rows x C columns
     uint32_t log_s;         // row stride S
     uint32_t flags;
-    const uint32_t* wtab;      // root table: [fwd/inv][prime][4096], Montgomery form
+    const uint32_t* wtab;      // forward root table: [prime][4096], Montgomery form
     // per-block roots of this pass: beta^(2^e) for e < log_r, [prime][beta_stride][kBetaPowStride],
-    // or null
+    // or null; forward and inverse stages use the same roots
     const uint32_t* beta;
-    const uint32_t* beta_inv;  // inverse roots (block pass only)
     uint32_t beta_stride;      // blocks per prime
 };
 
@@ -56,7 +55,8 @@ constexpr uint32_t kCarryTileLimbs = 4096;
 constexpr uint32_t kCarryTileWords = 3584;
 
 struct GarnerJob {
-    const uint32_t* y;   // 3 planes of n: inverse transform output (times n * R^-1)
+    const uint32_t* y;   // 3 planes of n: inverse transform output (times n * R^-1), coefficient j
+                         // at position (n - j) mod n
     uint32_t* out;       // out_len limbs
     uint32_t n, out_len;
     uint32_t kscale[3];  // n^-1 * R^2 mod P_i, so mont_mul(y, kscale) = y * n^-1 * R

@@ -289,6 +289,61 @@ def garner_carry(coeff, out_len):
     return out
 
 
+def coeff_pos(j, n):
+    """Kernel coeff_pos: position of coefficient j in an inverse output computed with the forward
+    roots (which leaves n x[(n - j) mod n] at position j)."""
+    assert 0 <= j < n
+    return (n - j) & (n - 1)
+
+
+def load_pair_rev(plane, n, wx):
+    """Kernel load_pair_rev: words wx and wx - 1 (wx even) of a plane, zero outside [0, n); every
+    read index is bounds-checked and the pair load must be an aligned position pair."""
+    assert wx % 2 == 0
+    if 2 <= wx <= n - 2:
+        q = n - wx
+        assert q % 2 == 0 and 0 <= q and q + 1 < n
+        return plane[q], plane[q + 1]
+    if wx == 0:
+        return plane[0], 0
+    if wx == n:
+        return 0, plane[1]
+    return 0, 0
+
+
+def garner_stage(plane, n, base, chunks=None):
+    """Kernel k_garner_carry's staging of the window [base - 4, base + 3584) of one plane: thread
+    tid loads pairs i = k * 256 + tid (k < 7), threads 0, 1, 2 the pairs -2, -1, 1792; pair i
+    holds words base + 2i and base + 2i - 1 and lands at staged indices 2i + 4 and 2i + 3.
+    chunks: list of (plane, first word) summed per word (chunked products), else one plane at 0."""
+    KST = KWT + 4
+    s_res = [None] * KST
+    srcs = chunks if chunks is not None else [(plane, 0)]
+
+    def pair(i):
+        x = y_ = 0
+        for pl, c0 in srcs:
+            a, b = load_pair_rev(pl, n, base + 2 * i - c0)
+            x, y_ = x + a, y_ + b
+        return x, y_
+    for tid in range(256):
+        for k in range(7):
+            i = k * 256 + tid
+            x, y_ = pair(i)
+            t = 2 * i + 3
+            assert s_res[t] is None and s_res[t + 1] is None
+            s_res[t], s_res[t + 1] = y_, x
+    x, _ = pair(-2)
+    s_res[0] = x
+    x, y_ = pair(-1)
+    s_res[1], s_res[2] = y_, x
+    _, y_ = pair(KWT // 2)
+    assert s_res[KST - 1] is None
+    s_res[KST - 1] = y_
+    assert all(v is not None for v in s_res)
+    return s_res
+
+
 def plan(log_n):
     if log_n <= 12:
         return [], log_n
@@ -307,7 +362,7 @@ def model_product(a, b, log_n):
     res = {0: [[0] * n for _ in range(3)], 1: [[0] * n for _ in range(3)]}
     ops = [(0, a), (1, b)]
     for I in range(3):
-        Wf, Wi = wtab(I, False), wtab(I, True)
+        Wf = wtab(I, False)
         for k, (log_r, log_c, log_s, nblk) in enumerate(outer):
             bt = beta_table(I, log_n, log_s, nblk, False, log_r)
             ntiles = 1 << (log_n - log_r - log_c)
@@ -323,16 +378,14 @@ def model_product(a, b, log_n):
                         False, limbs if k == 0 else None, I)
                     tile_transform(I, False, X, log_r, log_c, tw, 256)
                     gio(res[op][I], n, X, 1 << gs[-1][0], log_c, 0, tile_base, log_s, 256, True)
-        # middle pass
+        # middle pass: one forward table per tile serves both directions
         nblk = n >> mid_log_r
         bf = beta_table(I, log_n, 0, nblk, False, mid_log_r)
-        bi = beta_table(I, log_n, 0, nblk, True, mid_log_r)
         nthreads = (1 << mid_log_r) // KE
         gs = groups(mid_log_r)
         for blk in range(nblk):
             tb = blk << mid_log_r
             twf = build_tw(I, Wf, None if blk == 0 else bf[blk], mid_log_r)
-            twi = build_tw(I, Wi, None if blk == 0 else bi[blk], mid_log_r)
             XA, XB = [None] * nthreads, [None] * nthreads
             gio(res[0][I], n, XA, 1 << gs[0][0], 0, gs[0][1], tb, 0, nthreads, False,
                 a if not outer else None, I)
@@ -342,29 +395,37 @@ def model_product(a, b, log_n):
             tile_transform(I, False, XB, mid_log_r, 0, twf, nthreads)
             for t in range(nthreads):
                 XA[t] = [mont(I, u, v) for u, v in zip(XA[t], XB[t])]
-            tile_transform(I, True, XA, mid_log_r, 0, twi, nthreads)
+            tile_transform(I, True, XA, mid_log_r, 0, twf, nthreads)
             gio(res[0][I], n, XA, 1 << gs[0][0], 0, gs[0][1], tb, 0, nthreads, True)
         for k in range(len(outer) - 1, -1, -1):
             log_r, log_c, log_s, nblk = outer[k]
-            bt = beta_table(I, log_n, log_s, nblk, True, log_r)
+            bt = beta_table(I, log_n, log_s, nblk, False, log_r)  # forward roots
             ntiles = 1 << (log_n - log_r - log_c)
             log_ncg = log_s - log_c
             gs = groups(log_r)
             for t in range(ntiles):
                 blk, cg = t >> log_ncg, t & ((1 << log_ncg) - 1)
                 tile_base = (blk << (log_r + log_s)) + (cg << log_c)
-                tw = build_tw(I, Wi, None if blk == 0 else bt[blk], log_r)
+                tw = build_tw(I, Wf, None if blk == 0 else bt[blk], log_r)
                 X = [None] * 256
                 gio(res[0][I], n, X, 1 << gs[-1][0], log_c, 0, tile_base, log_s, 256, False)
                 tile_transform(I, True, X, log_r, log_c, tw, 256)
                 gio(res[0][I], n, X, 1 << gs[0][0], log_c, gs[0][1], tile_base, log_s, 256, True)
-    # Garner with the n^-1 R factor, 32-bit pieces and the tile carry scan
+    # Garner with the n^-1 R factor, 32-bit pieces and the tile carry scan.  The inverse ran with
+    # the forward roots, so coefficient j sits at position (n - j) mod n; each tile's window is
+    # read through the kernel's pair staging.
     y = res[0]
     out_len = len(a) + len(b)
+    for tile in range((out_len + KLT - 1) // KLT):
+        for p in range(3):
+            stage = garner_stage(y[p], n, tile * KWT)
+            assert stage == [y[p][coeff_pos(w, n)] if 0 <= w < n else 0
+                             for w in range(tile * KWT - 4, tile * KWT + KWT)]
     xs = []
     for j in range(n):
-        x = kernel_crt(y[0][j], y[1][j], y[2][j], n)
-        assert x < 1 << 89
+        pj = coeff_pos(j, n)
+        x = kernel_crt(y[0][pj], y[1][pj], y[2][pj], n)
+        assert x < 1 << 90
         xs.append(x)
     wa, wb = words_of(len(a)), words_of(len(b))
     wrap = limb_word(a, wa - 1) * limb_word(b, wb - 1) if wa + wb - 1 > n else 0
@@ -881,3 +942,29 @@ def test_first_pass_half_skip_reads_only_zero_rows(nlimbs, off, cnt, log_n, log_
         for u in [0, 1, P[I] - 1]:
             for w in [to_mont(I, 1), to_mont(I, 12345)]:
                 assert (u + k_mont(I, 0, w)) % P[I] == u and (u - k_mont(I, 0, w)) % P[I] == u
+
+
+@pytest.mark.parametrize("n,L,nch", [(16, 8, 5), (64, 32, 3), (4096, 2048, 4), (8192, 3584 * 2, 3),
+                                     (8192, 4096, 5)])
+def test_garner_staging_chunked_reversed(n, L, nch):
+    """Chunked Garner staging: chunk cc's transform holds its word wl at position (n - wl) mod n;
+    the kernel sums, per window word w, the chunks with 0 <= w - cc L < n, loading aligned
+    position pairs.  Replayed for every tile of the product against the direct sum."""
+    rng = random.Random(n * 7 + L + nch)
+    planes = [[rng.randrange(1 << 20) for _ in range(n)] for _ in range(nch)]
+    total = (nch - 1) * L + n
+    for base in range(0, total + KWT, KWT):
+        lo, hi = max(base - 4, 0), base + KWT - 1
+        c_hi = min(hi // L, nch - 1)
+        c_lo = (lo - n) // L + 1 if lo >= n else 0
+        chunks = [(planes[cc], cc * L) for cc in range(c_lo, c_hi + 1)]
+        got = garner_stage(None, n, base, chunks)
+        exp = []
+        for w in range(base - 4, base + KWT):
+            v = 0
+            for cc in range(nch):
+                wl = w - cc * L
+                if 0 <= wl < n:
+                    v += planes[cc][coeff_pos(wl, n)]
+            exp.append(v)
+        assert got == exp, base
