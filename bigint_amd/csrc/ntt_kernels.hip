@@ -382,18 +382,13 @@ __device__ __forceinline__ void outer_body(const PassArgs& a, const PassJob& job
     const uint32_t* wt = a.wtab + I * kTile;
     build_twiddles<I, LR>(tw, wt, block_pows<I>(a.beta, a.beta_stride, blk), kThreads);
 
+    // residue planes in and out: a first pass, which reads limbs, runs as k_outer_first
     uint32_t* plane = (op == 0 ? job.res_a : job.res_b) + (size_t)I * n;
-    LimbSrc src{nullptr, 0u, 0u, 0u};
-    if (!INV && (a.flags & kFromLimbs)) {
-        src = op == 0 ? LimbSrc{job.limbs_a, job.la, job.a_off, job.a_cnt}
-                      : LimbSrc{job.limbs_b, job.lb, 0u, ~0u};
-    }
     constexpr LimbSrc kPlane{nullptr, 0u, 0u, 0u};
     constexpr int G0 = 0, GL = S::NG - 1;
     uint32_t x[kE];
     if (!INV) {
-        global_io<LR, LC, S::stages(G0), S::trlo(G0), false>(plane, src, x, tile_base, a.log_s);
-        if (src.limbs) words_to_residues<I>(x);
+        global_io<LR, LC, S::stages(G0), S::trlo(G0), false>(plane, kPlane, x, tile_base, a.log_s);
         __syncthreads();
         tile_transform<I, false, LR, LC>(x, lds, tw);
         global_io<LR, LC, S::stages(GL), S::trlo(GL), true>(plane, kPlane, x, tile_base, a.log_s);
@@ -488,7 +483,9 @@ __device__ __forceinline__ uint32_t pointwise(uint32_t a, uint32_t b) {
     else return mont_mul<I>(a, b);
 }
 
-template <int I, int LR>
+// LIMBS: the pass reads the operands' limbs (a transform of at most one tile, no strided
+// passes); its own instantiation keeps the limb path's registers out of the plane-reading one.
+template <int I, int LR, bool LIMBS>
 __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, uint32_t* lds,
                                          uint32_t* tw) {
     using S = Sched<LR>;
@@ -513,7 +510,7 @@ __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, 
     }
     build_twiddles<I, LR>(tw, wf, bf, T);
 
-    const bool from_limbs = (a.flags & kFromLimbs) != 0;
+    constexpr bool from_limbs = LIMBS;
     const bool square = (a.flags & kSquare) != 0;
     uint32_t* pa = job.res_a + (size_t)I * n;
     constexpr int G0 = 0;
@@ -610,15 +607,15 @@ __global__ __launch_bounds__(1 << (LR - 4)) void k_mid_pair(PassArgs a) {
     }
 }
 
-template <int LR>
+template <int LR, bool LIMBS>
 __global__ __launch_bounds__(1 << (LR - 4)) void k_mid_pass(PassArgs a) {
     __shared__ uint32_t lds[(1 << LR) + (1 << LR) / 16];
     __shared__ uint32_t tw[1 << LR];
     const PassJob job = a.jobs ? load_desc(a.jobs + blockIdx.y) : a.job;
     switch (blockIdx.z) {
-        case 0: mid_body<0, LR>(a, job, lds, tw); break;
-        case 1: mid_body<1, LR>(a, job, lds, tw); break;
-        default: mid_body<2, LR>(a, job, lds, tw); break;
+        case 0: mid_body<0, LR, LIMBS>(a, job, lds, tw); break;
+        case 1: mid_body<1, LR, LIMBS>(a, job, lds, tw); break;
+        default: mid_body<2, LR, LIMBS>(a, job, lds, tw); break;
     }
 }
 
@@ -1141,7 +1138,11 @@ constexpr auto first_table(std::integer_sequence<int, LRs...>) {
 }
 template <int... LRs>
 constexpr auto mid_table(std::integer_sequence<int, LRs...>) {
-    return std::array<KernelFn, sizeof...(LRs)>{&k_mid_pass<LRs + 4>...};
+    return std::array<KernelFn, sizeof...(LRs)>{&k_mid_pass<LRs + 4, false>...};
+}
+template <int... LRs>
+constexpr auto mid_limb_table(std::integer_sequence<int, LRs...>) {
+    return std::array<KernelFn, sizeof...(LRs)>{&k_mid_pass<LRs + 4, true>...};
 }
 template <int... LRs>
 constexpr auto pair_table(std::integer_sequence<int, LRs...>) {
@@ -1172,6 +1173,7 @@ hipError_t launch_outer_pass(const PassArgs& a, bool inv, uint32_t njobs, uint32
 
 hipError_t launch_mid_pass(const PassArgs& a, uint32_t njobs, hipStream_t st) {
     static constexpr auto tab = mid_table(std::make_integer_sequence<int, 9>{});
+    static constexpr auto ltab = mid_limb_table(std::make_integer_sequence<int, 9>{});
     static constexpr auto ptab = pair_table(std::make_integer_sequence<int, 9>{});
     if (a.log_r < 4 || a.log_r > 12) return hipErrorInvalidValue;
     const uint32_t nblk = 1u << (a.log_n - a.log_r);
@@ -1185,7 +1187,8 @@ hipError_t launch_mid_pass(const PassArgs& a, uint32_t njobs, hipStream_t st) {
         const uint32_t nj = std::min(kMaxGridY, njobs - j0);
         const bool pair = (a.flags & ~kSquare) == 0 && nblk > 1 && (nblk / 2u) * nj >= 256u;
         dim3 grid(pair ? nblk / 2 : nblk, nj, kNumPrimes);
-        hipLaunchKernelGGL((pair ? ptab : tab)[a.log_r - 4], grid, dim3(threads), 0, st, b);
+        const auto& t = pair ? ptab : ((a.flags & kFromLimbs) ? ltab : tab);
+        hipLaunchKernelGGL(t[a.log_r - 4], grid, dim3(threads), 0, st, b);
     }
     return hipGetLastError();
 }
