@@ -629,7 +629,7 @@ __global__ __launch_bounds__(1 << (LR - 4)) void k_mid_pass(PassArgs a) {
 // in {0, 1, 2} and the carry-in -> carry-out behaviour of any run of words is a map
 // {0..3} -> {0..3}, stored as four 2-bit fields.  Maps compose associatively, which makes the
 // carry chain a scan.  The product's words are finally cut back into 28-bit limbs: a thread owns
-// 14 words = 16 limbs, a tile 3584 words = 4096 limbs, so limbs never straddle threads.
+// 7 words = 8 limbs, a tile 3584 words = 4096 limbs, so limbs never straddle threads.
 
 __device__ __forceinline__ uint32_t map_apply(uint32_t m, uint32_t c) { return (m >> (2u * c)) & 3u; }
 // (second after first): apply `first`, then `second`
@@ -693,61 +693,31 @@ __device__ __forceinline__ uint64_t wrap_product(const GarnerJob& jb) {
     return (uint64_t)limb_word(jb.wrap_a, jb.wrap_la, ga) * limb_word(jb.wrap_b, jb.wrap_lb, gb);
 }
 
-// Convolution coefficient j of the job, split into 32-bit pieces.  For a chunked product the
-// coefficient is the sum of the chunk products covering j.  CRT is linear, so the chunk
-// residues are summed mod each prime first and reconstructed once: the true sum is a
-// coefficient of the whole product, below 2^89 < P0*P1*P2.
-__device__ __forceinline__ Pieces garner(const GarnerJob& jb, const GarnerConsts& k, uint32_t j,
-                                         uint32_t tile_chunk) {
-    uint32_t r0, r1, r2;
-    const uint32_t n = jb.n;
-    if (jb.nchunks <= 1) {
-        if (j >= n) {
-            if (j != n || !jb.wrap_a) return {0u, 0u, 0u};
-            const uint64_t w = wrap_product(jb);
-            return {(uint32_t)w, (uint32_t)(w >> 32), 0u};
-        }
-        const uint32_t pj = coeff_pos(j, n);
-        r0 = gmem(jb.y)[pj];
-        r1 = gmem(jb.y)[n + pj];
-        r2 = gmem(jb.y)[2u * n + pj];
-        if (j == 0 && jb.wrap_a) {
-            const unsigned __int128 x = crt_coeff(r0, r1, r2, jb.kscale, k) - wrap_product(jb);
-            const uint64_t lo = (uint64_t)x;
-            return Pieces{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)(x >> 64)};
-        }
-    } else {
-        r0 = r1 = r2 = 0;
-        // primary chunk of j; earlier chunks whose products reach j add in
-        const uint32_t c = jb.log_chunk < 32u ? (j >> jb.log_chunk) : tile_chunk;
-        for (uint32_t d = 0; d <= c; ++d) {
-            const uint32_t cc = c - d;
-            const uint32_t off = j - cc * jb.chunk_len;
-            if (off >= n) break;
-            if (cc < jb.nchunks) {
-                const uint32_t* y = jb.y + (size_t)cc * jb.chunk_stride;
-                const uint32_t po = coeff_pos(off, n);
-                r0 = add_mod<0>(r0, fold1p(gmem(y)[po]));  // P0 outputs lie in [0, 2P)
-                r1 = add_mod<1>(r1, gmem(y)[n + po]);
-                r2 = add_mod<2>(r2, gmem(y)[2u * n + po]);
-            }
-        }
-    }
-    const unsigned __int128 x = crt_coeff(r0, r1, r2, jb.kscale, k);
-    const uint64_t lo = (uint64_t)x;
-    return Pieces{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)(x >> 64)};
-}
-
 // acc + x mod P_p for a residue x of prime p's inverse output (P0 outputs lie in [0, 2 P0)).
 __device__ __forceinline__ uint32_t add_residue(uint32_t p, uint32_t acc, uint32_t x) {
     return p == 0u ? add_mod<0>(acc, fold1p(x)) : (p == 1u ? add_mod<1>(acc, x) : add_mod<2>(acc, x));
 }
 
-constexpr uint32_t kCarryLimbsPerThread = 16;
-constexpr uint32_t kCarryWordsPerThread = 14;  // 14 * 32 = 16 * 28 bits
-constexpr uint32_t kCarryTile = kThreads * kCarryLimbsPerThread;  // 4096 limbs per tile
-constexpr uint32_t kWordTile = kThreads * kCarryWordsPerThread;   // 3584 words per tile
+constexpr uint32_t kGThreads = 512;           // threads per Garner + carry tile
+constexpr uint32_t kCarryWordsPerThread = 7;   // 7 * 32 = 8 * 28 bits
+constexpr uint32_t kCarryLimbsPerThread = 8;
+constexpr uint32_t kCarryTile = kGThreads * kCarryLimbsPerThread;  // 4096 limbs per tile
+constexpr uint32_t kWordTile = kGThreads * kCarryWordsPerThread;   // 3584 words per tile
 static_assert(kCarryTile == kCarryTileLimbs && kWordTile == kCarryTileWords, "tile geometry");
+
+// Coefficient j of a job from its three (chunk-summed) residues, split into 32-bit pieces.  An
+// unchunked product whose top coefficient wrapped from index n onto index 0 has it taken out of
+// coefficient 0 and put back at index n (whose residues read as zero).
+__device__ __forceinline__ Pieces coeff_pieces(uint32_t r0, uint32_t r1, uint32_t r2, int64_t j,
+                                               const GarnerJob& jb, const GarnerConsts& k) {
+    unsigned __int128 x = crt_coeff(r0, r1, r2, jb.kscale, k);
+    if (jb.wrap_a) {
+        if (j == 0) x -= wrap_product(jb);
+        else if (j == (int64_t)jb.n) x = wrap_product(jb);
+    }
+    const uint64_t lo = (uint64_t)x;
+    return Pieces{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)(x >> 64)};
+}
 
 // Status words carry their whole payload (flag << 8 | map or carry), so they are published and
 // polled with relaxed agent-scope atomics: no acquire/release fence, hence no L2 write-back or
@@ -760,22 +730,23 @@ __device__ __forceinline__ void status_store(uint32_t* p, uint32_t v) {
 }
 constexpr uint32_t kAgg = 1u, kInc = 2u;
 
-// Pieces of an unchunked product's coefficient from its residues (no wrap at this index).
-__device__ __forceinline__ Pieces split_coeff(uint32_t r0, uint32_t r1, uint32_t r2,
-                                              const GarnerJob& jb, const GarnerConsts& k) {
-    const unsigned __int128 x = crt_coeff(r0, r1, r2, jb.kscale, k);
-    const uint64_t lo = (uint64_t)x;
-    return Pieces{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)(x >> 64)};
-}
-
 // CHUNKED: the jobs are chunked products (nchunks > 1); otherwise every job is unchunked.  Two
 // instantiations keep the chunk loop's register demand out of the common kernel.
+//
+// A tile of 3584 words (4096 limbs) per block of 512 threads; thread t owns words
+// j0 = base + 7t .. j0 + 6.  Its word sums s_q = w0(j0+q) + w1(j0+q-1) + w2(j0+q-2) take the
+// pieces of coefficients j0 - 1 and j0 - 2 from its left neighbour (a wave shuffle, across waves
+// through LDS); thread 0 takes them from coefficients base - 1 and base - 2, which threads 0..2
+// reconstruct together with base - 3 (for the tile's local carry rule).
 template <bool CHUNKED>
-__global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
+__global__ __launch_bounds__(kGThreads) void k_garner_carry(GarnerArgs a) {
     const GarnerJob jb = a.jobs ? load_desc(a.jobs + blockIdx.y) : a.job;
+    constexpr uint32_t kWaves = kGThreads / 64;
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_wave[kThreads / 64];
+    __shared__ uint32_t s_wave[kWaves];
     __shared__ uint32_t s_carry_in;
+    __shared__ uint32_t s_edge[kWaves][3];  // lane 63 of each wave: w1 and w2 of its last word, w2 of the one before
+    __shared__ uint32_t s_halo[3][3];       // pieces of words base - 3, base - 2, base - 1
     const uint32_t tid = threadIdx.x;
     if (tid == 0) s_tile = __hip_atomic_fetch_add(gmem(jb.counter), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
@@ -783,58 +754,45 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     const uint32_t ntiles = (jb.out_len + kCarryTile - 1u) / kCarryTile;
     if (tile >= ntiles) return;
     const uint32_t base = tile * kWordTile;  // first word (coefficient) of the tile
-    const uint32_t j0 = base + tid * kCarryWordsPerThread;  // this thread's first word
-
-    // 1. pieces of coefficients j0-2 .. j0+13: the two before the thread's words feed its
-    //    first two sums (recomputed here rather than exchanged).
-    constexpr uint32_t NQ = kCarryWordsPerThread + 2;
-    // word sums s_q = w0_{q+2} + w1_{q+1} + w2_q < 3 * 2^32 of the thread's 14 words (pieces
-    // indexed from word j0 - 2), and w0_1, w1_0 for the tile's local carry
-    uint32_t s_lo[kCarryWordsPerThread], s_hi[kCarryWordsPerThread];
-    uint32_t w0_1 = 0u, w1_0 = 0u;
     const uint32_t n = jb.n;
-    const bool wrap_here = jb.wrap_a && j0 - 2u <= n && n < j0 + kCarryWordsPerThread;
-    const bool fast = j0 >= 2u && !wrap_here;
-    // residues of the thread's run, words j0 - 2 .. j0 + 13 of each plane.  j0 - 2, n and chunk
-    // lengths are even, so aligned pairs never straddle a transform's end.
-    uint32_t r0[NQ], r1[NQ], r2[NQ];
-    // residues of word base - 3, for the tile's local carry below
-    uint32_t h3[3] = {0u, 0u, 0u};
-    // The block's words [base - 4, base + kWordTile) of each plane are read coalesced and staged
-    // in LDS one plane at a time, where each thread picks up its run of 16; read straight from
-    // memory, the runs 56 bytes apart would spread every wave's load over 28 cache lines.  Word w
-    // sits at position (n - w) mod n (coeff_pos), so aligned position pairs hold descending word
-    // pairs: pair i of the window is words base + 2i and base + 2i - 1.  Thread tid loads pairs
-    // i = k * 256 + tid (k < 7: words base - 1 .. base + 3582), threads 0, 1, 2 the window's ends
-    // (pairs -2, -1: words base - 4 .. base - 2; pair 1792: word base + 3583).  Words outside
-    // every transform read as zero; a chunked product's chunk residues are summed on the way.
-    // The output limbs go through the same LDS array.
+
+    // 1. Residues.  The block's words [base - 4, base + kWordTile) of each plane are read coalesced
+    //    and staged in LDS one plane at a time, where each thread picks up its run of 7.  Word w
+    //    sits at position (n - w) mod n (coeff_pos), so aligned position pairs hold descending
+    //    word pairs: pair i of the window is words base + 2i and base + 2i - 1.  Thread tid loads
+    //    pairs i = k * 512 + tid (k < 4, i < 1792: words base - 1 .. base + 3582), threads 0, 1, 2
+    //    the window's ends (pairs -2, -1: words base - 4 .. base - 2; pair 1792: word base + 3583).
+    //    Words outside every transform read as zero; a chunked product's chunk residues are summed
+    //    on the way.  The output limbs go through the same LDS array.
     constexpr uint32_t kStage = kWordTile + 4u;  // 4 halo words + tile
     constexpr uint32_t kResWords = kStage > kCarryTile ? kStage : kCarryTile;
     __shared__ __align__(16) uint32_t s_res[kResWords];
+    constexpr uint32_t kNPairs = kWordTile / 2u;                       // 1792
+    constexpr uint32_t kPairLoads = (kNPairs + kGThreads - 1) / kGThreads;  // 4
+    const int64_t ie = tid == 0u ? -2 : (tid == 1u ? -1 : (int64_t)kNPairs);
+    const bool has_end = tid < 3u;
+    uint32_t r[3][kCarryWordsPerThread];  // residues of the thread's words, per plane
+    uint32_t hr[3] = {0u, 0u, 0u};        // threads 0..2: residues of word base - 3 + tid
     {
-        constexpr uint32_t kPairs = kWordTile / (2u * kThreads);
-        static_assert(kPairs * 2u * kThreads == kWordTile, "pairs per thread");
-        // the end pair of threads 0, 1, 2: i = -2, -1, 1792
-        const int64_t ie = tid == 0u ? -2 : (tid == 1u ? -1 : (int64_t)(kWordTile / 2u));
-        const bool has_end = tid < 3u;
-        u32x2 v[3][kPairs], h[3] = {{0u, 0u}, {0u, 0u}, {0u, 0u}};
+        u32x2 v[3][kPairLoads], h[3] = {{0u, 0u}, {0u, 0u}, {0u, 0u}};
+#pragma unroll
+        for (uint32_t p = 0; p < 3u; ++p)
+#pragma unroll
+            for (uint32_t k = 0; k < kPairLoads; ++k) v[p][k] = u32x2{0u, 0u};
         if constexpr (!CHUNKED) {
 #pragma unroll
             for (uint32_t p = 0; p < 3u; ++p) {
                 const uint32_t* yp = jb.y + (size_t)p * n;
 #pragma unroll
-                for (uint32_t k = 0; k < kPairs; ++k)
-                    v[p][k] = load_pair_rev(yp, n, (int64_t)base + 2 * (int64_t)(k * kThreads + tid));
+                for (uint32_t k = 0; k < kPairLoads; ++k) {
+                    const uint32_t i = k * kGThreads + tid;
+                    if (i < kNPairs) v[p][k] = load_pair_rev(yp, n, (int64_t)base + 2 * (int64_t)i);
+                }
                 if (has_end) h[p] = load_pair_rev(yp, n, (int64_t)base + 2 * ie);
             }
         } else {
             // chunk cc covers words [cc L, cc L + n); those reaching [lo, hi] are
             // (lo - n) / L < cc <= hi / L
-#pragma unroll
-            for (uint32_t p = 0; p < 3u; ++p)
-#pragma unroll
-                for (uint32_t k = 0; k < kPairs; ++k) v[p][k] = u32x2{0u, 0u};
             const uint32_t L = jb.chunk_len;
             const bool p2 = jb.log_chunk < 32u;
             const uint32_t lo = base >= 4u ? base - 4u : 0u, hi = base + kWordTile - 1u;
@@ -848,9 +806,12 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
                 for (uint32_t p = 0; p < 3u; ++p) {
                     const uint32_t* yp = yc + (size_t)p * n;
 #pragma unroll
-                    for (uint32_t k = 0; k < kPairs; ++k) {
-                        const u32x2 x = load_pair_rev(yp, n, (int64_t)base + 2 * (int64_t)(k * kThreads + tid) - c0);
-                        v[p][k] = u32x2{add_residue(p, v[p][k].x, x.x), add_residue(p, v[p][k].y, x.y)};
+                    for (uint32_t k = 0; k < kPairLoads; ++k) {
+                        const uint32_t i = k * kGThreads + tid;
+                        if (i < kNPairs) {
+                            const u32x2 x = load_pair_rev(yp, n, (int64_t)base + 2 * (int64_t)i - c0);
+                            v[p][k] = u32x2{add_residue(p, v[p][k].x, x.x), add_residue(p, v[p][k].y, x.y)};
+                        }
                     }
                     if (has_end) {
                         const u32x2 x = load_pair_rev(yp, n, (int64_t)base + 2 * ie - c0);
@@ -859,16 +820,18 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
                 }
             }
         }
-        const uint32_t o = tid * kCarryWordsPerThread + 2u;  // staged index of word j0 - 2
+        const uint32_t o = tid * kCarryWordsPerThread + 4u;  // staged index of word j0
 #pragma unroll
         for (uint32_t p = 0; p < 3u; ++p) {
             if (p > 0u) __syncthreads();  // the previous plane has been read
             // pair i: word base + 2i at staged index 2i + 4, word base + 2i - 1 at 2i + 3
 #pragma unroll
-            for (uint32_t k = 0; k < kPairs; ++k) {
-                const uint32_t t = 2u * (k * kThreads + tid) + 3u;
-                s_res[t] = v[p][k].y;
-                s_res[t + 1u] = v[p][k].x;
+            for (uint32_t k = 0; k < kPairLoads; ++k) {
+                const uint32_t i = k * kGThreads + tid;
+                if (i < kNPairs) {
+                    s_res[2u * i + 3u] = v[p][k].y;
+                    s_res[2u * i + 4u] = v[p][k].x;
+                }
             }
             if (tid == 0u) s_res[0] = h[p].x;  // pair -2: word base - 4 (base - 5 is outside)
             if (tid == 1u) {                   // pair -1: words base - 2, base - 3
@@ -877,67 +840,68 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
             }
             if (tid == 2u) s_res[kStage - 1u] = h[p].y;  // pair 1792: word base + 3583
             __syncthreads();
-            if (tid == 0u) h3[p] = s_res[1];
-            if (fast) {
-                uint32_t* r = p == 0u ? r0 : (p == 1u ? r1 : r2);
 #pragma unroll
-                for (uint32_t i = 0; i < NQ; i += 2) {
-                    const u32x2 x = *reinterpret_cast<const u32x2*>(s_res + o + i);
-                    r[i] = x.x;
-                    r[i + 1] = x.y;
-                }
+            for (uint32_t q = 0; q < kCarryWordsPerThread; ++q) r[p][q] = s_res[o + q];
+            if (has_end) hr[p] = s_res[1u + tid];  // words base - 3, base - 2, base - 1
+        }
+    }
+
+    // 2. Pieces and word sums.  s_q (64-bit: lo, hi) collects w0(j0+q) + w1(j0+q-1) + w2(j0+q-2)
+    //    of the thread's own coefficients; the neighbour's share of s_0 and s_1 follows below.
+    const int64_t j0 = (int64_t)base + (int64_t)tid * kCarryWordsPerThread;
+    uint32_t s_lo[kCarryWordsPerThread], s_hi[kCarryWordsPerThread];
+    uint32_t e_w1 = 0u, e_w2 = 0u, e_w2p = 0u;  // w1, w2 of word j0 + 6, w2 of word j0 + 5
+    {
+        uint64_t acc1 = 0, acc2 = 0;  // partial sums of the next word and the one after
+#pragma unroll
+        for (uint32_t q = 0; q < kCarryWordsPerThread; ++q) {
+            const Pieces pc = coeff_pieces(r[0][q], r[1][q], r[2][q], j0 + q, jb, a.k);
+            const uint64_t sum = acc1 + pc.w0;
+            s_lo[q] = (uint32_t)sum;
+            s_hi[q] = (uint32_t)(sum >> 32);
+            acc1 = acc2 + pc.w1;
+            acc2 = pc.w2;
+            if (q == kCarryWordsPerThread - 2u) e_w2p = pc.w2;
+            if (q == kCarryWordsPerThread - 1u) {
+                e_w1 = pc.w1;
+                e_w2 = pc.w2;
             }
         }
     }
-    if (fast) {
-        const uint32_t jl = j0 - 2u;
-        uint64_t acc2 = 0, acc1 = 0;
-#pragma unroll
-        for (uint32_t i = 0; i < NQ; ++i) {
-            // coefficients outside every transform are zero: no CRT (whole threads skip it in
-            // the tiles of short products)
-            const bool in = CHUNKED || jl + i < n;
-            const Pieces p = in ? split_coeff(r0[i], r1[i], r2[i], jb, a.k) : Pieces{0u, 0u, 0u};
-            // summed as they come (acc2, acc1: partial sums of words i - 2 and i - 1), so the
-            // pieces of the run are never all live at once
-            if (i == 0) w1_0 = p.w1;
-            if (i == 1) w0_1 = p.w0;
-            if (i >= 2) {
-                const uint64_t sum = acc2 + p.w0;
-                s_lo[i - 2] = (uint32_t)sum;
-                s_hi[i - 2] = (uint32_t)(sum >> 32);
-            }
-            acc2 = acc1 + p.w1;
-            acc1 = p.w2;
+    if (has_end) {
+        const Pieces pc = coeff_pieces(hr[0], hr[1], hr[2], (int64_t)base - 3 + (int64_t)tid, jb, a.k);
+        s_halo[tid][0] = pc.w0;
+        s_halo[tid][1] = pc.w1;
+        s_halo[tid][2] = pc.w2;
+    }
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
+    if (lane == 63u) {
+        s_edge[wave][0] = e_w1;
+        s_edge[wave][1] = e_w2;
+        s_edge[wave][2] = e_w2p;
+    }
+    // the left neighbour's w1(j0 - 1), w2(j0 - 1), w2(j0 - 2)
+    uint32_t n_w1 = __shfl_up(e_w1, 1, 64), n_w2 = __shfl_up(e_w2, 1, 64), n_w2p = __shfl_up(e_w2p, 1, 64);
+    __syncthreads();
+    if (lane == 0u) {
+        if (wave > 0u) {
+            n_w1 = s_edge[wave - 1u][0];
+            n_w2 = s_edge[wave - 1u][1];
+            n_w2p = s_edge[wave - 1u][2];
+        } else {
+            n_w1 = s_halo[2][1];   // word base - 1
+            n_w2 = s_halo[2][2];
+            n_w2p = s_halo[1][2];  // word base - 2
         }
-    } else {
-        // the wrapped coefficient and the product's first two words
-        const uint32_t tile_chunk =
-            (jb.nchunks > 1 && jb.log_chunk == 0xFFu) ? base / jb.chunk_len : 0u;
-        uint32_t w0[NQ], w1[NQ], w2[NQ];
-#pragma unroll 2
-        for (uint32_t i = 0; i < NQ; ++i) {
-            Pieces p{0u, 0u, 0u};
-            if (j0 + i >= 2u) {
-                const uint32_t j = j0 + i - 2u;
-                // a chunk length that is a multiple of the tile gives the tile one primary
-                // chunk; the two coefficients before the tile may lie in the previous one
-                const uint32_t c = (jb.nchunks > 1 && jb.log_chunk == 0xFFu && j < base)
-                                       ? j / jb.chunk_len : tile_chunk;
-                p = garner(jb, a.k, j, c);
-            }
-            w0[i] = p.w0;
-            w1[i] = p.w1;
-            w2[i] = p.w2;
-        }
-#pragma unroll
-        for (uint32_t q = 0; q < kCarryWordsPerThread; ++q) {
-            const uint64_t sum = (uint64_t)w0[q + 2] + w1[q + 1] + w2[q];
-            s_lo[q] = (uint32_t)sum;
-            s_hi[q] = (uint32_t)(sum >> 32);
-        }
-        w0_1 = w0[1];
-        w1_0 = w1[0];
+    }
+    {
+        // s_0 += w1(j0 - 1) + w2(j0 - 2), s_1 += w2(j0 - 1): below 3 * 2^32 in total
+        uint64_t t0 = ((uint64_t)s_hi[0] << 32 | s_lo[0]) + n_w1 + n_w2p;
+        uint64_t t1 = ((uint64_t)s_hi[1] << 32 | s_lo[1]) + n_w2;
+        s_lo[0] = (uint32_t)t0;
+        s_hi[0] = (uint32_t)(t0 >> 32);
+        s_lo[1] = (uint32_t)t1;
+        s_hi[1] = (uint32_t)(t1 >> 32);
     }
 
     // The carry into the tile, resolved locally when possible: the previous tile's last word has
@@ -946,19 +910,11 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     // needs nothing from its predecessors (0xFF: not resolved, use the look-back).
     uint32_t local = 0xFFu;
     if (tid == 0 && tile != 0) {
-        const uint32_t j = base - 3u;
-        Pieces p3;
-        if (!(jb.wrap_a && j == n)) {
-            p3 = split_coeff(h3[0], h3[1], h3[2], jb, a.k);  // staged (zero past the transform)
-        } else {
-            const uint32_t c = (jb.nchunks > 1 && jb.log_chunk == 0xFFu) ? j / jb.chunk_len : 0u;
-            p3 = garner(jb, a.k, j, c);
-        }
-        const uint64_t sl = (uint64_t)w0_1 + w1_0 + p3.w2;
+        const uint64_t sl = (uint64_t)s_halo[2][0] + s_halo[1][1] + s_halo[0][2];
         if ((uint32_t)sl <= 0xFFFFFFFDu) local = (uint32_t)(sl >> 32);
     }
 
-    // 2. this thread's carry map (carry-out for each carry-in 0..3, four chains run side by
+    // 3. this thread's carry map (carry-out for each carry-in 0..3, four chains run side by
     //    side), then an inclusive scan of maps across the block
     uint32_t m;
     {
@@ -973,7 +929,6 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
         }
         m = c[0] | (c[1] << 2) | (c[2] << 4) | (c[3] << 6);
     }
-    const uint32_t lane = tid & 63u, wave = tid >> 6;
     uint32_t inc = m;
 #pragma unroll
     for (uint32_t d = 1; d < 64; d <<= 1) {
@@ -988,11 +943,11 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     if (lane == 0) excl = kMapIdentity;
     excl = map_compose(excl, wave_prefix);
 
-    // 3. carry into the tile: resolved locally, or by a decoupled look-back of the first wave
+    // 4. carry into the tile: resolved locally, or by a decoupled look-back of the first wave
     //    (64 predecessor tiles read at once).  Every tile publishes its status either way.
     if (wave == 0) {
         uint32_t agg = kMapIdentity;
-        for (uint32_t w = 0; w < kThreads / 64; ++w) agg = map_compose(s_wave[w], agg);
+        for (uint32_t w = 0; w < kWaves; ++w) agg = map_compose(s_wave[w], agg);
         uint32_t carry_in = 0;
         const uint32_t loc = __shfl(local, 0, 64);
         if (tile == 0) {
@@ -1038,7 +993,7 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     }
     __syncthreads();
 
-    // 4. apply the carries, cut the 14 words into 16 limbs and write them
+    // 5. apply the carries, cut the 7 words into 8 limbs and write them
     uint32_t carry = map_apply(excl, s_carry_in);
     uint32_t wv[kCarryWordsPerThread];
 #pragma unroll
@@ -1067,7 +1022,7 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
     if ((reinterpret_cast<uintptr_t>(out) & 15u) == 0) {
 #pragma unroll
         for (uint32_t k = 0; k < kCarryLimbsPerThread / 4; ++k) {
-            const uint32_t idx = 4u * (k * kThreads + tid);
+            const uint32_t idx = 4u * (k * kGThreads + tid);
             const u32x4 x = *reinterpret_cast<const u32x4*>(s_res + idx);
             if (idx + 4u <= lim) {
                 *gmem(reinterpret_cast<u32x4*>(out + idx)) = x;
@@ -1079,8 +1034,8 @@ __global__ __launch_bounds__(kThreads) void k_garner_carry(GarnerArgs a) {
         }
     } else {
 #pragma unroll
-        for (uint32_t m = 0; m < kCarryLimbsPerThread; ++m) {
-            const uint32_t idx = m * kThreads + tid;
+        for (uint32_t m2 = 0; m2 < kCarryLimbsPerThread; ++m2) {
+            const uint32_t idx = m2 * kGThreads + tid;
             if (idx < lim) gmem(out)[idx] = s_res[idx];
         }
     }
@@ -1210,7 +1165,7 @@ hipError_t launch_garner_carry(const GarnerArgs& a, uint32_t max_out_len, uint32
         GarnerArgs b = a;
         if (a.jobs) b.jobs = a.jobs + j0;
         dim3 grid(ntiles, std::min(kMaxGridY, njobs - j0), 1);
-        hipLaunchKernelGGL(chunked ? k_garner_carry<true> : k_garner_carry<false>, grid, dim3(kThreads),
+        hipLaunchKernelGGL(chunked ? k_garner_carry<true> : k_garner_carry<false>, grid, dim3(kGThreads),
                            0, st, b);
     }
     return hipGetLastError();

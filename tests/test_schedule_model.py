@@ -227,6 +227,7 @@ def words_of(nlimbs):
 
 
 KWT, KLT = 3584, 4096  # words / limbs per carry tile
+GT, GW, GL = 512, 7, 8  # Garner threads per tile, words and limbs per thread
 
 
 def map_apply(m, c):
@@ -253,34 +254,49 @@ def garner_carry(coeff, out_len):
             s = piece(base + o, 0) + piece(base + o - 1, 1) + piece(base + o - 2, 2)
             assert s < 3 << 32
             sums.append(s)
+        # the kernel's sums: each thread's own coefficients j0 .. j0 + 6, plus the left
+        # neighbour's w1(j0 - 1), w2(j0 - 1), w2(j0 - 2) (lane shuffle, LDS across waves; the
+        # tile's halo coefficients base - 1, base - 2 for thread 0)
+        for t in range(GT):
+            j0 = base + GW * t
+            own = []
+            acc1 = acc2 = 0
+            for q in range(GW):
+                w0, w1, w2 = (piece(j0 + q, k) for k in range(3))
+                own.append(acc1 + w0)
+                acc1, acc2 = acc2 + w1, w2
+            nb = j0 - 1  # the neighbour's last word (thread t - 1, or the halo for t = 0)
+            own[0] += piece(nb, 1) + piece(nb - 1, 2)
+            own[1] += piece(nb, 2)
+            assert own == sums[GW * t:GW * t + GW], (tile, t)
         if tile:  # kernel's local resolution of the carry into the tile, checked against the scan
             sl = piece(base - 1, 0) + piece(base - 2, 1) + piece(base - 3, 2)
             if sl & 0xFFFFFFFF <= 0xFFFFFFFD:
                 assert sl >> 32 == carry_in
         maps = []
-        for t in range(256):
+        for t in range(GT):
             c = [0, 1, 2, 3]
-            for q in range(14):
-                s = sums[14 * t + q]
+            for q in range(GW):
+                s = sums[GW * t + q]
                 c = [(s + ck) >> 32 for ck in c]
             assert max(c) <= 3
             maps.append(c[0] | c[1] << 2 | c[2] << 4 | c[3] << 6)
         excl = 0b11100100
-        for t in range(256):
+        for t in range(GT):
             carry = map_apply(excl, carry_in)
             assert carry <= 2
             words = []
-            for q in range(14):
-                v = sums[14 * t + q] + carry
+            for q in range(GW):
+                v = sums[GW * t + q] + carry
                 words.append(v & 0xFFFFFFFF)
                 carry = v >> 32
-            for q in range(16):
+            for q in range(GL):
                 bit = 28 * q
                 wi, bo = bit >> 5, bit & 31
                 v = words[wi] >> bo
                 if bo > 4:
                     v |= words[wi + 1] << (32 - bo)
-                i = tile * KLT + 16 * t + q
+                i = tile * KLT + GL * t + q
                 if i < out_len:
                     out[i] = v & MASK
             excl = map_compose(maps[t], excl)
@@ -313,8 +329,8 @@ def load_pair_rev(plane, n, wx):
 
 def garner_stage(plane, n, base, chunks=None):
     """Kernel k_garner_carry's staging of the window [base - 4, base + 3584) of one plane: thread
-    tid loads pairs i = k * 256 + tid (k < 7), threads 0, 1, 2 the pairs -2, -1, 1792; pair i
-    holds words base + 2i and base + 2i - 1 and lands at staged indices 2i + 4 and 2i + 3.
+    tid loads pairs i = k * 512 + tid (k < 4, i < 1792), threads 0, 1, 2 the pairs -2, -1, 1792;
+    pair i holds words base + 2i and base + 2i - 1 and lands at staged indices 2i + 4 and 2i + 3.
     chunks: list of (plane, first word) summed per word (chunked products), else one plane at 0."""
     KST = KWT + 4
     s_res = [None] * KST
@@ -326,9 +342,11 @@ def garner_stage(plane, n, base, chunks=None):
             a, b = load_pair_rev(pl, n, base + 2 * i - c0)
             x, y_ = x + a, y_ + b
         return x, y_
-    for tid in range(256):
-        for k in range(7):
-            i = k * 256 + tid
+    for tid in range(GT):
+        for k in range(4):
+            i = k * GT + tid
+            if i >= KWT // 2:
+                continue
             x, y_ = pair(i)
             t = 2 * i + 3
             assert s_res[t] is None and s_res[t + 1] is None
