@@ -43,9 +43,12 @@ auto words64(std::span<const uint32_t> x) -> size_type {
 // loop vectorizes), and each block sum is split at bit 28 into the column's own part and the
 // part carried into the next column.  The host route only takes products of <= 16,384 bits
 // (at most 586 terms per column), so the carried part stays below 2^40.  No scratch array: the
-// product is written as it goes.  Built for AVX2 with a generic fallback chosen at load time.
-__attribute__((target_clones("avx2", "default")))
-void schoolbook_cols(const uint32_t* a, size_type na, const uint32_t* b, size_type nb, uint32_t* r) {
+// product is written as it goes.  Built twice, for AVX2 and generic x86-64, and chosen once at
+// run time (an explicit choice rather than an ifunc, whose resolver runs before sanitizer
+// runtimes start).
+__attribute__((always_inline)) inline void schoolbook_cols_body(const uint32_t* a, size_type na,
+                                                                const uint32_t* b, size_type nb,
+                                                                uint32_t* r) {
     uint64_t carry = 0;  // carry into column k
     for (size_type k = 0; k + 1 < na + nb; ++k) {
         const size_type lo = k + 1 > nb ? k + 1 - nb : 0, hi = std::min(k, na - 1);
@@ -63,6 +66,26 @@ void schoolbook_cols(const uint32_t* a, size_type na, const uint32_t* b, size_ty
     r[na + nb - 1] = (uint32_t)carry;
 }
 
+__attribute__((target("avx2"))) void schoolbook_cols_avx2(const uint32_t* a, size_type na,
+                                                          const uint32_t* b, size_type nb, uint32_t* r) {
+    schoolbook_cols_body(a, na, b, nb, r);
+}
+
+void schoolbook_cols_generic(const uint32_t* a, size_type na, const uint32_t* b, size_type nb, uint32_t* r) {
+    schoolbook_cols_body(a, na, b, nb, r);
+}
+
+bool cpu_has_avx2() {
+    __builtin_cpu_init();
+    return __builtin_cpu_supports("avx2");
+}
+
+void schoolbook_cols(const uint32_t* a, size_type na, const uint32_t* b, size_type nb, uint32_t* r) {
+    static const bool avx2 = cpu_has_avx2();
+    if (avx2) schoolbook_cols_avx2(a, na, b, nb, r);
+    else schoolbook_cols_generic(a, na, b, nb, r);
+}
+
 auto schoolbook(std::span<const uint32_t> a, std::span<const uint32_t> b) -> Digits {
     Digits r(a.size() + b.size());
     schoolbook_cols(a.data(), a.size(), b.data(), b.size(), r.data());
@@ -78,8 +101,15 @@ void add_engine_time_ns(uint64_t ns) noexcept { g_backend_ns.fetch_add(ns, std::
 
 auto load_planner(const char* path) -> bool { return backend::load_planner(path); }
 
+// A product the reference would send to its NTT creates the thread pool at its default size
+// (first call wins), as the reference's get_pool() does inside ntt::mul.
+static void note_reference_route(size_type total_bits) {
+    if (total_bits > REF_FFT_MAX_TOTAL_BITS && total_bits <= REF_NTT_MAX_TOTAL_BITS) (void)thread_pool_size();
+}
+
 auto mul_limbs(std::span<const uint32_t> a, std::span<const uint32_t> b, bool square) -> Digits {
     if (square) b = a;
+    note_reference_route((words64(a) + words64(b)) * 64);
     if ((words64(a) + words64(b)) * 64 <= HOST_MAX_TOTAL_BITS) {
         g_host_products.fetch_add(1, std::memory_order_relaxed);
         Digits r = schoolbook(a, b);
@@ -123,6 +153,7 @@ auto mul_digits(const Digits& a, const Digits& b, size_type output_precision,
         while (la > 1 && sa[la - 1] == 0) --la;
         while (lb > 1 && sb[lb - 1] == 0) --lb;
         const size_type keep = output_precision + 1;
+        note_reference_route((words64(sa) + words64(sb)) * 64);
         if ((words64(sa) + words64(sb)) * 64 > HOST_MAX_TOTAL_BITS && la + lb > keep) {
             Digits top(keep);
             const auto t0 = std::chrono::steady_clock::now();
@@ -202,6 +233,7 @@ auto mul_digits_batch(const std::vector<std::pair<const Digits*, const Digits*>>
                 res[i] = std::move(r);
                 ++host_counts[t];
             } else {
+                note_reference_route((words64(sa) + words64(sb)) * 64);
                 big_parts[t].push_back(Item{i, za, zb, sa, sb, same});
             }
         }

@@ -19,6 +19,14 @@ namespace bigint::mul {
 // Largest total size (bits, counted as the reference counts them) multiplied on the host.
 constexpr size_type HOST_MAX_TOTAL_BITS = 2 * 128 * 64;
 
+// The reference's route boundaries (mul.h:153, 192, 208-220): it multiplies products of
+// REF_FFT_MAX_TOTAL_BITS < (|A| + |B|) * 64 <= REF_NTT_MAX_TOTAL_BITS with its threaded NTT, whose
+// first call creates the thread pool at its default size (get_pool(),
+// src/mul/ntt_multithread.cpp:287-290), after which init_thread_pool has no effect.  Products in
+// that band fix the pool size here the same way, whatever route computes them.
+constexpr size_type REF_FFT_MAX_TOTAL_BITS = 10485760;
+constexpr size_type REF_NTT_MAX_TOTAL_BITS = ((size_type(1) << 26) - 2) * 28;
+
 // Raised when the GPU engine reports an error other than out-of-memory.
 class engine_error : public std::runtime_error {
 public:

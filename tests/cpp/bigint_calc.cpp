@@ -11,6 +11,7 @@
 //   frecip A pa prec | fprint A pa digits | ftodouble A pa | ffromdouble X (hex bits)
 //   fround A pa mode prec rel
 //   counts                   -> "<gpu products> <host products>"
+//   poolinit N | pool        -> init_thread_pool(N) ("ok") | thread_pool_size()
 //   decthr N                 -> sets DEC_STRING_BRUTE_THRESHOLDS[1] (limbs) and prints "ok"
 //   devmuladd A B C D        -> A*B + C*D and A*A through DeviceInt (backend memory): "S SQ"
 //   batch N SEED MAXBITS     -> checksum of BigInt::mul_batch over N SplitMix64 pairs (see below)
@@ -221,6 +222,13 @@ int main() {
                 in >> t;
                 BigInt::DEC_STRING_BRUTE_THRESHOLDS[1] = t;
                 out << "ok";
+            } else if (op == "poolinit") {  // init_thread_pool(N)
+                uint32_t k;
+                in >> k;
+                init_thread_pool(k);
+                out << "ok";
+            } else if (op == "pool") {  // the pool's thread count (fixes it at the default if unset)
+                out << thread_pool_size();
             } else if (op == "counts") {
                 out << mul::gpu_product_count() << " " << mul::host_product_count();
             } else {

@@ -303,6 +303,35 @@ def test_large_reciprocal_matches_model(calc):
         assert calc("frecip", hx(a), 650, prec) == xf(BF.reciprocal(f, prec))
 
 
+def _calc_session(lines):
+    """Run bigint_calc in a fresh process (the thread pool is process state) over the lines."""
+    if not os.path.exists(CALC):
+        pytest.skip("bigint_calc not built")
+    out = subprocess.run([CALC], input="\n".join(lines) + "\n", capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stderr
+    return out.stdout.strip().split("\n")
+
+
+def test_init_thread_pool_semantics():
+    """init_thread_pool keeps the reference's semantics (bigint.h:35-38; ntt_multithread.cpp:
+    287-290, 407-409): the first call wins, 0 and 1 both mean no worker threads, and a product in
+    the reference's NTT band (more than 10,485,760 total bits) made before any call creates the pool
+    at its default size, after which init_thread_pool has no effect.  A smaller product does not.
+    (Without a GPU the large products here fail after the pool is fixed; the pool is the point.)"""
+    hw = os.cpu_count()
+    assert _calc_session(["poolinit 3", "poolinit 5", "pool"]) == ["ok", "ok", "3"]
+    assert _calc_session(["poolinit 0", "pool"]) == ["ok", "1"]
+    assert _calc_session(["poolinit 1", "poolinit 6", "pool"]) == ["ok", "ok", "1"]
+    assert _calc_session(["pool", "poolinit 2", "pool"]) == [str(hw), "ok", str(hw)]
+    big = hx((1 << 6_000_000) - 12345)        # 2 x 6 Mbit: the reference's NTT band
+    mid = hx((1 << 3_000_000) - 777)          # 2 x 3 Mbit: its FFT band
+    r = _calc_session([f"mul {big} {big}", "poolinit 3", "pool"])
+    assert r[1:] == ["ok", str(hw)], r[1:]
+    r = _calc_session([f"mul {mid} {mid}", "poolinit 3", "pool"])
+    assert r[1:] == ["ok", "3"], r[1:]
+
+
 def _newton_bound_ok(a, pa, res, prec):
     """|x * b - 1| < 2^(-28 (prec - 2)) for x = D 2^(-28 pp) and b = a 2^(-28 pa): the error bound
     of the doubling Newton iteration (src/bigint.cpp:1044-1071) at its final precision."""
