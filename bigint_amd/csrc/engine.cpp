@@ -7,6 +7,7 @@
 #include <cmath>
 #include <tuple>
 #include <cstdint>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -36,6 +37,20 @@ constexpr uint32_t kPh[3] = {kP[0], kP[1], kP[2]};  // the device primes and gen
 constexpr uint32_t kGh[3] = {kG[0], kG[1], kG[2]};
 constexpr int kTileH = 4096;
 constexpr uint32_t kMaxLog = 26;
+
+// Device allocations made while serving calls (workspaces, staging, descriptors, caller buffers).
+// With BI_HIP_TEST_FAIL_ALLOC set in the environment when the engine initialises, every such
+// allocation fails, so tests can force the out-of-memory paths (BI_HIP_ENOMEM and the host
+// library's std::bad_alloc) from the host; set-up allocations (root tables, plans) are not
+// affected.
+bool g_fail_alloc = false;
+hipError_t call_malloc(void** p, size_t bytes) {
+    if (g_fail_alloc) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc(p, bytes);
+}
 
 uint32_t mulmod(uint64_t a, uint64_t b, uint32_t p) { return (uint32_t)((a * b) % p); }
 uint32_t powmod(uint32_t b, uint64_t e, uint32_t p) {
@@ -301,6 +316,7 @@ public:
         if (dev < 0 || dev >= count) return BI_HIP_EINVAL;
         if (hipSetDevice(dev) != hipSuccess) return BI_HIP_EDEVICE;
         device = dev;
+        g_fail_alloc = std::getenv("BI_HIP_TEST_FAIL_ALLOC") != nullptr;
         if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
             return BI_HIP_EDEVICE;
         // root table W[m + i] = w_{2m}^{bitrev_{log m}(i)}, Montgomery form, one per prime; the
@@ -435,7 +451,7 @@ public:
             if (ws) (void)hipFree(ws);
             ws = nullptr;
             ws_words = 0;
-            if (hipMalloc((void**)&ws, words * 4) != hipSuccess) return nullptr;
+            if (call_malloc((void**)&ws, words * 4) != hipSuccess) return nullptr;
             ws_words = words;
         }
         return ws;
@@ -794,7 +810,7 @@ public:
             if (d_scratch) (void)hipFree(d_scratch);
             d_scratch = nullptr;
             d_scratch_words = 0;
-            if (hipMalloc((void**)&d_scratch, words * 4) != hipSuccess) return nullptr;
+            if (call_malloc((void**)&d_scratch, words * 4) != hipSuccess) return nullptr;
             d_scratch_words = words;
         }
         return d_scratch;
@@ -978,7 +994,7 @@ public:
             if (desc) (void)hipFree(desc);
             desc = nullptr;
             desc_bytes = 0;
-            if (hipMalloc((void**)&desc, desc_need) != hipSuccess) return BI_HIP_ENOMEM;
+            if (call_malloc((void**)&desc, desc_need) != hipSuccess) return BI_HIP_ENOMEM;
             desc_bytes = desc_need;
         }
         std::vector<uint8_t> hdesc(desc_need);
@@ -1145,7 +1161,7 @@ public:
             if (desc) (void)hipFree(desc);
             desc = nullptr;
             desc_bytes = 0;
-            if (hipMalloc((void**)&desc, need) != hipSuccess) return BI_HIP_ENOMEM;
+            if (call_malloc((void**)&desc, need) != hipSuccess) return BI_HIP_ENOMEM;
             desc_bytes = need;
         }
         if (wait_shared() || hipStreamSynchronize(st) != hipSuccess ||
@@ -1263,7 +1279,7 @@ int mul_host(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, int is_
         if (e.io) (void)hipFree(e.io);
         e.io = nullptr;
         e.io_words = 0;
-        if (hipMalloc((void**)&e.io, words * 4) != hipSuccess) return BI_HIP_ENOMEM;
+        if (bi::call_malloc((void**)&e.io, words * 4) != hipSuccess) return BI_HIP_ENOMEM;
         e.io_words = words;
     }
     uint32_t* da = e.io;
@@ -1381,7 +1397,7 @@ int bi_hip_mul_batch(bi_hip_pair* pairs, size_t count, int plan_mode) {
         if (e.stage) (void)hipFree(e.stage);
         e.stage = nullptr;
         e.stage_words = 0;
-        if (hipMalloc((void**)&e.stage, (in_words + out_words) * 4) != hipSuccess) return BI_HIP_ENOMEM;
+        if (bi::call_malloc((void**)&e.stage, (in_words + out_words) * 4) != hipSuccess) return BI_HIP_ENOMEM;
         e.stage_words = in_words + out_words;
     }
     uint32_t* dbuf = e.stage;
@@ -1657,7 +1673,7 @@ int bi_hip_add_batch_device(bi_hip_dadd* jobs, size_t count, void* stream) {
         if (e.add_scratch) (void)hipFree(e.add_scratch);
         e.add_scratch = nullptr;
         e.add_bytes = 0;
-        if (hipMalloc((void**)&e.add_scratch, need) != hipSuccess) return BI_HIP_ENOMEM;
+        if (bi::call_malloc((void**)&e.add_scratch, need) != hipSuccess) return BI_HIP_ENOMEM;
         e.add_bytes = need;
     }
     uint32_t* words = (uint32_t*)(e.add_scratch + desc_bytes);
@@ -1692,7 +1708,7 @@ int bi_hip_device_alloc(size_t bytes, void** ptr) {
         if (int rc = engine().init(0)) return rc;
     }
     if (bytes == 0) return BI_HIP_OK;
-    return hipMalloc(ptr, bytes) == hipSuccess ? BI_HIP_OK : BI_HIP_ENOMEM;
+    return bi::call_malloc(ptr, bytes) == hipSuccess ? BI_HIP_OK : BI_HIP_ENOMEM;
 }
 
 int bi_hip_device_free(void* ptr) {

@@ -332,6 +332,52 @@ def test_init_thread_pool_semantics():
     assert r[1:] == ["ok", "3"], r[1:]
 
 
+def _calc_env_session(lines, env):
+    if not os.path.exists(CALC):
+        pytest.skip("bigint_calc not built")
+    e = dict(os.environ)
+    e.update(env)
+    out = subprocess.run([CALC], input="\n".join(lines) + "\n", capture_output=True, text=True,
+                         timeout=300, env=e)
+    assert out.returncode == 0, out.stderr
+    return out.stdout.strip().split("\n")
+
+
+def test_engine_failures_map_to_exceptions_without_gpu():
+    """Engine failures reach the caller as exceptions, as the reference's do (bad_alloc from its
+    allocations, bigint.h:115): without a GPU an engine-sized product raises
+    bigint::mul::engine_error with the engine's status text (no CPU fallback), while host-sized
+    products still work in the same process."""
+    import shutil
+    if shutil.which("rocm-smi") and os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU may be present: the no-device path is not reachable")
+    big = hx((1 << 20000) - 3)
+    r = _calc_env_session([f"mul {big} {big}", "mul 7 6", f"fmul {big} 0 {big} 0 1000"], {})
+    assert r[0].startswith("EXC ") and "bad_alloc" not in r[0], r[0]
+    assert r[1] == "2a"
+    assert r[2].startswith("EXC "), r[2]
+
+
+@pytest.mark.gpu
+def test_engine_out_of_memory_maps_to_bad_alloc():
+    """With device allocations forced to fail from the host (BI_HIP_TEST_FAIL_ALLOC, read when the
+    engine initialises), every engine route of the host library raises std::bad_alloc: a product,
+    a precision-truncated product, a batch and a DeviceInt upload; host-sized products are
+    unaffected.  The same commands succeed without the flag."""
+    rng = random.Random(115)
+    a, b = rng.getrandbits(200000) | 1, rng.getrandbits(150000) | 1
+    lines = [f"mul {hx(a)} {hx(b)}", "mul 7 6", f"fmul {hx(a)} 3 {hx(b)} 5 1000",
+             f"devmuladd {hx(a)} {hx(b)} {hx(b)} {hx(a)}", "batch 40 3 400000"]
+    r = _calc_env_session(lines, {"BI_HIP_TEST_FAIL_ALLOC": "1"})
+    assert r[0] == "EXC std::bad_alloc", r[0]
+    assert r[1] == "2a"
+    assert r[2] == "EXC std::bad_alloc", r[2]
+    assert r[3] == "EXC std::bad_alloc", r[3]
+    assert r[4] == "EXC std::bad_alloc", r[4]
+    ok = _calc_env_session(lines, {})
+    assert unhx(ok[0]) == a * b and ok[1] == "2a" and not any(x.startswith("EXC") for x in ok), ok
+
+
 def _newton_bound_ok(a, pa, res, prec):
     """|x * b - 1| < 2^(-28 (prec - 2)) for x = D 2^(-28 pp) and b = a 2^(-28 pa): the error bound
     of the doubling Newton iteration (src/bigint.cpp:1044-1071) at its final precision."""
