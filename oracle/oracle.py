@@ -137,6 +137,8 @@ def mt_lib() -> ctypes.CDLL:
         L.cpu_mul_digits.argtypes = [u32p, ctypes.c_size_t, u32p, ctypes.c_size_t, ctypes.c_int,
                                      u32p, szp]
         L.cpu_pool_threads.restype = ctypes.c_uint32
+        L.cpu_route.restype = ctypes.c_int
+        L.cpu_route.argtypes = [ctypes.c_size_t, ctypes.c_size_t]
         _MT = L
     return _MT
 

@@ -35,3 +35,47 @@ def test_trailing_zero_limbs_and_all_ones():
     a = L.int_to_limbs(x)
     b = np.concatenate([np.zeros(7, np.uint32), L.random_operand(40000, 3)])
     assert L.limbs_to_int(mt_mul(a, b)) == x * L.limbs_to_int(b)
+
+
+def route(la, lb):
+    return oracle.mt_lib().cpu_route(la, lb)
+
+
+def test_dispatcher_routes_like_the_reference():
+    """(|A| + |B|) * 64 bits in u64 words after the 28-bit -> u64 repack (mul.h:208-220, 245-268):
+    <= 16,384 brute, <= 10,485,760 FFT, <= 1,879,048,136 NTT, else SSA."""
+    words = lambda limbs: (28 * limbs + 63) // 64  # noqa: E731
+    limbs_for_words = lambda w: (64 * w) // 28     # noqa: E731
+    assert route(limbs_for_words(128), limbs_for_words(128)) == 0
+    assert route(limbs_for_words(128), limbs_for_words(129)) == 1
+    assert route(limbs_for_words(81920), limbs_for_words(81920)) == 1
+    assert route(limbs_for_words(81920), limbs_for_words(81921)) == 2
+    assert route(149797, 149797) == 1  # 2^22-bit operands: the FFT route (the CPU oracle config)
+    assert route(599187, 599187) == 2  # 2^24-bit operands: NTT
+    assert route(38347923, 38347923) == 3
+    assert words(599187) == 262145
+
+
+@pytest.mark.parametrize("bits", range(10, 21))
+def test_fft_route_exact_at_each_digit_size_limit(bits):
+    """All-ones squares, every convolution coefficient at its maximum (the reference's worst case,
+    tests/measure_digit_bits.cpp:65-85), as large as the capacity table allows for each digit
+    size, and an all-ones product of unequal operands: exact against the scalar oracle."""
+    cap = [0] * 8 + [16531472, 11048310, 10485760, 2827308, 1433736, 851968, 799288, 334830, 262144,
+                     128826, 18432, 19456, 18200]
+    words = cap[bits] // 128                    # both operands: 2 * words * 64 <= capacity
+    limbs = (64 * words) // 28
+    if (28 * limbs + 63) // 64 * 2 * 64 <= 16384:
+        pytest.skip("below the FFT band")
+    a = L.int_to_limbs((1 << (28 * limbs)) - 1)
+    assert route(a.size, a.size) == 1
+    assert np.array_equal(mt_mul(a, None), oracle.mul_limbs(a, None))
+    b = L.int_to_limbs((1 << (28 * (limbs // 3))) - 1)
+    assert np.array_equal(mt_mul(a, b), oracle.mul_limbs(a, b))
+
+
+@pytest.mark.parametrize("ba,bb", [(100, 200), (28 * 292, 28 * 292), (5000, 300000), (1 << 22, 1 << 22)])
+def test_every_route_random(ba, bb):
+    a = L.random_operand(ba, 7)
+    b = L.random_operand(bb, 8)
+    assert np.array_equal(mt_mul(a, b), oracle.mul_limbs(a, b))
