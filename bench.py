@@ -77,6 +77,25 @@ def cpu_baseline(a: np.ndarray, b: np.ndarray, bits: int, max_s: float = 25.0) -
                       f"{info.get('cores_per_socket')} cores x {info.get('threads_per_core')} threads"}
 
 
+def batch_cpu_baseline(pairs_count: int, cores: int, stride: int = 16) -> dict:
+    """The batched workload on the CPU restatement, on the reference's routes (brute and FFT
+    single-threaded, NTT on the threaded pool already created by cpu_baseline): every 16th pair
+    of the same size draw, SplitMix64 operands, multiplied one after another as the reference's
+    operator* loop does; a bounded sample (about 10 s)."""
+    from oracle import cpu_time
+    r = cpu_time.batch(cores, pairs_count, stride)
+    return {"value": r["sequential_Gbit_s"], "unit": "Gbit/s", "cores": r["threads"], "kind": "port",
+            "ms": r["sequential_ms"], "routes": r["routes"],
+            "sample": f"{r['pairs_timed']} of {pairs_count} pairs (every {stride}th, {r['sample_bits']} "
+                      f"input bits), sequential loop on the reference's routes"}
+
+
+def c1_cpu() -> dict:
+    """2^22 x 2^22 bits and the square of A on the CPU restatement (the reference's FFT route)."""
+    from oracle import cpu_time
+    return cpu_time.c1(0)
+
+
 def batch_leg(hip, engine, L, pairs_count: int, steps: int) -> dict:
     """The batched product-tree workload: `pairs_count` independent products whose operand sizes
     are log-uniform in [2^12, 2^24] bits (SplitMix64 from the fixed seed), uniform random limbs
@@ -415,6 +434,11 @@ def main() -> int:
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a, b, bits)
         line["speedup_vs_cpu"] = round(value / line["cpu_baseline"]["value"], 2)
+        if args.batch_pairs:
+            line["batch"]["cpu_baseline"] = batch_cpu_baseline(args.batch_pairs, line["cpu_baseline"]["cores"])
+            cb = line["batch"]["cpu_baseline"]["value"]
+            line["batch"]["learned_vs_cpu"] = round(line["batch"]["learned"]["Gbit_s"] / cb, 1) if cb else None
+        line["cpu_c1"] = c1_cpu()
     print(json.dumps(line), flush=True)
     return 0 if exact and line.get("batch", {}).get("exact", True) else 1
 

@@ -4,11 +4,12 @@ thread count, because the pool follows init_thread_pool's first-call-wins rule
 
   python -m oracle.cpu_time --threads 8 --bits 28          # 2^28 x 2^28, warm-up + timed runs
   python -m oracle.cpu_time --threads 16 --batch 4096 --stride 16
+  python -m oracle.cpu_time --threads 8 --c1                # 2^22 x 2^22 and the square of A
 
 --batch times a sample of the batched product-tree workload (every `stride`-th pair of the
-4096-pair log-uniform draw) in two ways: a sequential loop of multiplies on the threaded pool
-(the reference's `operator*` loop), and pairs spread across `threads` workers each running a
-one-thread multiply.  Prints one JSON line.
+4096-pair log-uniform draw) in two ways: a sequential loop of multiplies on the reference's routes
+(brute and FFT single-threaded, NTT on the threaded pool: the reference's `operator*` loop), and
+pairs spread across `threads` workers each running a one-thread multiply.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -57,27 +58,64 @@ def balanced(threads: int, log_bits: int, max_s: float) -> dict:
             "Gbit_s": 2 * bits / med / 1e9}
 
 
+ROUTES = ("brute", "fft", "ntt", "ssa")
+
+
+def batch_ops(count: int, stride: int):
+    """Every `stride`-th pair of the batched workload: the same SplitMix64 operands the GPU runs
+    (limbs.batch_bits, limbs.batch_seeds)."""
+    draw = L.batch_bits(count)
+    seeds = L.batch_seeds(count)
+    ops = []
+    for i in range(0, count, stride):
+        (ba, bb), (sa, sb) = draw[i], seeds[i]
+        a, b = L.random_operand(ba, sa), L.random_operand(bb, sb)
+        ops.append((a, b, np.zeros(a.size + b.size + 4, np.uint32)))
+    return draw, ops
+
+
 def batch(threads: int, count: int, stride: int) -> dict:
+    """The sample as a sequential loop of multiplies (the reference's operator* loop) on the
+    reference's routes: brute and FFT single-threaded, the NTT on the threaded pool."""
     M = oracle.mt_lib()
     M.cpu_init_thread_pool(threads)
-    draw = L.batch_bits(count)
-    sample = draw[::stride]
-    rng = np.random.default_rng(7)
-    ops = []
-    for ba, bb in sample:
-        a = L.random_operand(ba, int(rng.integers(1 << 62)))
-        b = L.random_operand(bb, int(rng.integers(1 << 62)))
-        ops.append((a, b, np.zeros(a.size + b.size + 4, np.uint32)))
-    bits = sum(ba + bb for ba, bb in sample)
+    draw, ops = batch_ops(count, stride)
+    bits = sum(ba + bb for ba, bb in draw[::stride])
+    routes = {r: 0 for r in ROUTES}
+    for a, b, _ in ops:
+        routes[ROUTES[M.cpu_route(a.size, b.size)]] += 1
     for a, b, o in ops[:8]:  # warm-up: pool, pages
         _mul(M, a, b, o)
     t0 = time.perf_counter()
     for a, b, o in ops:
         _mul(M, a, b, o)
     seq = time.perf_counter() - t0
-    return {"threads": int(M.cpu_pool_threads()), "pairs_timed": len(sample), "stride": stride,
-            "sample_bits": int(bits), "sequential_ms": seq * 1e3, "sequential_Gbit_s": bits / seq / 1e9,
-            "full_batch_bits": int(sum(ba + bb for ba, bb in draw))}
+    return {"threads": int(M.cpu_pool_threads()), "pairs_timed": len(ops), "stride": stride,
+            "routes": routes, "sample_bits": int(bits), "sequential_ms": seq * 1e3,
+            "sequential_Gbit_s": bits / seq / 1e9, "full_batch_bits": int(sum(ba + bb for ba, bb in draw))}
+
+
+def c1(threads: int, reps: int = 5) -> dict:
+    """2^22 x 2^22 bits (A seed, B seed + 1) and the square of A, as the reference routes them
+    (its FFT: 8,388,608 total bits), median of `reps` after a warm-up."""
+    M = oracle.mt_lib()
+    M.cpu_init_thread_pool(threads)
+    a = L.random_operand(1 << 22, L.SEED)
+    b = L.random_operand(1 << 22, L.SEED + 1)
+    out = np.zeros(a.size + b.size + 4, np.uint32)
+    res = {"threads": int(M.cpu_pool_threads()), "route": ROUTES[M.cpu_route(a.size, b.size)]}
+    for name, y, same in (("product", b, 0), ("square", a, 1)):
+        n = ctypes.c_size_t()
+        p = ctypes.POINTER(ctypes.c_uint32)
+        times = []
+        for r in range(reps + 1):
+            t0 = time.perf_counter()
+            M.cpu_mul_digits(a.ctypes.data_as(p), a.size, y.ctypes.data_as(p), y.size, same,
+                             out.ctypes.data_as(p), ctypes.byref(n))
+            if r:
+                times.append(time.perf_counter() - t0)
+        res[name + "_ms"] = float(np.median(times)) * 1e3
+    return res
 
 
 def batch_across(threads: int, count: int, stride: int) -> dict:
@@ -85,14 +123,8 @@ def batch_across(threads: int, count: int, stride: int) -> dict:
     ctypes call releases the GIL)."""
     M = oracle.mt_lib()
     M.cpu_init_thread_pool(1)
-    draw = L.batch_bits(count)[::stride]
-    rng = np.random.default_rng(7)
-    ops = []
-    for ba, bb in draw:
-        a = L.random_operand(ba, int(rng.integers(1 << 62)))
-        b = L.random_operand(bb, int(rng.integers(1 << 62)))
-        ops.append((a, b, np.zeros(a.size + b.size + 4, np.uint32)))
-    bits = sum(ba + bb for ba, bb in draw)
+    draw, ops = batch_ops(count, stride)
+    bits = sum(ba + bb for ba, bb in draw[::stride])
     nxt = [0]
     lock = threading.Lock()
 
@@ -123,8 +155,11 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=0, help="time the batched workload of this many pairs")
     ap.add_argument("--stride", type=int, default=16)
     ap.add_argument("--across", action="store_true", help="pairs across threads (one-thread pool)")
+    ap.add_argument("--c1", action="store_true", help="2^22 x 2^22 bits and the square of A")
     args = ap.parse_args()
-    if args.batch and args.across:
+    if args.c1:
+        r = c1(args.threads)
+    elif args.batch and args.across:
         r = batch_across(args.threads, args.batch, args.stride)
     elif args.batch:
         r = batch(args.threads, args.batch, args.stride)
