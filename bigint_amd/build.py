@@ -68,6 +68,10 @@ def build(verbose: bool = False) -> str:
     _run([CXX] + HOST_FLAGS + [os.path.join(ROOT, "tests", "cpp", "bigint_calc.cpp"), "-o", calc,
                                "-L" + HERE, "-lbigint", "-lbigint_hip",
                                "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/.."])
+    # host schoolbook against the engine, end to end (the host route's size limit)
+    _run([CXX] + HOST_FLAGS + [os.path.join(HERE, "examples", "crossover.cpp"), "-o",
+                               os.path.join(bdir, "crossover"), "-L" + HERE, "-lbigint", "-lbigint_hip",
+                               "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/.."])
     # π example on the engine
     _run([CXX] + HOST_FLAGS + [os.path.join(HERE, "examples", "pi.cpp"), "-o", os.path.join(bdir, "pi"),
                                "-L" + HERE, "-lbigint", "-lbigint_hip", "-Wl,-rpath," + HERE,

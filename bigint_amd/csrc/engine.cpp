@@ -471,10 +471,13 @@ public:
                         double out_limbs, uint32_t* status, uint32_t status_words, hipStream_t st) {
         const uint32_t n = 1u << pl->log_n;
         const double nj = (double)njobs;
-        if (kind != kBForward &&
-            timed(BI_HIP_K_FILL, 4.0 * status_words, st,
-                  [&] { return launch_fill(status, 0u, status_words, st); }) != hipSuccess)
-            return BI_HIP_EDEVICE;
+        // the first kernel of the sequence clears the Garner launch's status words
+        bool clear_pending = kind != kBForward && status && status_words;
+        auto take_clear = [&](PassArgs& x) {
+            x.clear = clear_pending ? status : nullptr;
+            x.clear_words = clear_pending ? status_words : 0u;
+            clear_pending = false;
+        };
         const bool one_op = kind != kPlain;
         const double nops_d = one_op ? 1.0 : 2.0;
         const bool fwd_passes = kind != kInverse;  // kInverse: res_a already holds a spectrum
@@ -494,6 +497,7 @@ public:
             a.flags = base_flags | (k == 0 ? (uint32_t)kFromLimbs : 0u);
             a.beta = op.beta;
             a.beta_stride = op.nblk;
+            take_clear(a);
             // the first pass reads the limbs once; every pass writes 3 planes per operand
             const double bytes = k == 0 ? 4.0 * in_limbs + 12.0 * n * nops_d * nj : 24.0 * n * nops_d * nj;
             if (timed(k == 0 ? BI_HIP_K_OUTER_FIRST : BI_HIP_K_OUTER_FWD, bytes, st,
@@ -510,6 +514,7 @@ public:
                   (pl->outer.empty() && fwd_passes ? (uint32_t)kFromLimbs : 0u);
         a.beta = pl->mid_beta;
         a.beta_stride = pl->mid_nblk;
+        take_clear(a);
         {
             double in_bytes = pl->outer.empty() && fwd_passes ? 4.0 * in_limbs : 12.0 * n * nops_d * nj;
             if (kind == kChunk) in_bytes += 12.0 * n * nj;  // the shared transformed B
@@ -526,6 +531,7 @@ public:
             a.flags = 0;
             a.beta = op.beta;  // forward roots (see coeff_pos in the kernels)
             a.beta_stride = op.nblk;
+            take_clear(a);
             if (timed(BI_HIP_K_OUTER_INV, 24.0 * n * nj, st,
                       [&] { return launch_outer_pass(a, true, njobs, 1, st); }) != hipSuccess)
                 return BI_HIP_EDEVICE;

@@ -447,6 +447,15 @@ __device__ __forceinline__ void outer_first_body(const PassArgs& a, const PassJo
     });
 }
 
+// Zero the launch sequence's carry-status words (PassArgs::clear) with every thread of the grid:
+// the first kernel of a sequence does it, long before the sequence's Garner launch reads them.
+__device__ __forceinline__ void clear_status(const PassArgs& a) {
+    if (!a.clear) return;
+    const uint32_t nthreads = blockDim.x * gridDim.x * gridDim.y * gridDim.z;
+    const uint32_t id = ((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
+    for (uint32_t i = id; i < a.clear_words; i += nthreads) gmem(a.clear)[i] = 0u;
+}
+
 // Its own kernel, so the register demand of the three-prime loop does not lower the occupancy of
 // the other strided passes.
 template <int LR>
@@ -454,6 +463,7 @@ __global__ __launch_bounds__(kThreads) void k_outer_first(PassArgs a) {
     __shared__ uint32_t lds[kLdsWords];
     __shared__ uint32_t tw[kNumPrimes << LR];
     const PassJob job = a.jobs ? load_desc(a.jobs + blockIdx.y) : a.job;
+    clear_status(a);
     outer_first_body<LR>(a, job, blockIdx.z, lds, tw);
 }
 
@@ -462,6 +472,7 @@ __global__ __launch_bounds__(kThreads) void k_outer_pass(PassArgs a) {
     __shared__ uint32_t lds[kLdsWords];
     __shared__ uint32_t tw[1 << LR];
     const PassJob job = a.jobs ? load_desc(a.jobs + blockIdx.y) : a.job;
+    clear_status(a);
     const uint32_t nops = INV ? 1u : ((a.flags & (kSquare | kSingle)) ? 1u : 2u);
     const uint32_t prime = blockIdx.z / nops, op = blockIdx.z % nops;
     switch (prime) {
@@ -600,6 +611,7 @@ __global__ __launch_bounds__(1 << (LR - 4)) void k_mid_pair(PassArgs a) {
     __shared__ uint32_t lds[(1 << LR) + (1 << LR) / 16];
     __shared__ uint32_t tw[1 << LR];
     const PassJob job = a.jobs ? load_desc(a.jobs + blockIdx.y) : a.job;
+    clear_status(a);
     switch (blockIdx.z) {
         case 0: mid_pair_body<0, LR>(a, job, lds, tw); break;
         case 1: mid_pair_body<1, LR>(a, job, lds, tw); break;
@@ -612,6 +624,7 @@ __global__ __launch_bounds__(1 << (LR - 4)) void k_mid_pass(PassArgs a) {
     __shared__ uint32_t lds[(1 << LR) + (1 << LR) / 16];
     __shared__ uint32_t tw[1 << LR];
     const PassJob job = a.jobs ? load_desc(a.jobs + blockIdx.y) : a.job;
+    clear_status(a);
     switch (blockIdx.z) {
         case 0: mid_body<0, LR, LIMBS>(a, job, lds, tw); break;
         case 1: mid_body<1, LR, LIMBS>(a, job, lds, tw); break;
@@ -1070,10 +1083,6 @@ __global__ __launch_bounds__(256) void k_diag_accumulate(const uint32_t* sa, con
     }
 }
 
-__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t count) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) p[i] = v;
-}
 
 }  // namespace bi
 
@@ -1120,6 +1129,7 @@ hipError_t launch_outer_pass(const PassArgs& a, bool inv, uint32_t njobs, uint32
     for (uint32_t j0 = 0; j0 < njobs; j0 += kMaxGridY) {  // one grid row per job
         PassArgs b = a;
         if (a.jobs) b.jobs = a.jobs + j0;
+        if (j0) b.clear = nullptr;  // the first launch clears
         dim3 grid(ntiles, std::min(kMaxGridY, njobs - j0), fused ? nops : kNumPrimes * nops);
         hipLaunchKernelGGL((fused ? first : inv ? bwd : fwd)[a.log_r - 1], grid, dim3(kThreads), 0, st, b);
     }
@@ -1137,6 +1147,7 @@ hipError_t launch_mid_pass(const PassArgs& a, uint32_t njobs, hipStream_t st) {
     for (uint32_t j0 = 0; j0 < njobs; j0 += kMaxGridY) {
         PassArgs b = a;
         if (a.jobs) b.jobs = a.jobs + j0;
+        if (j0) b.clear = nullptr;  // the first launch clears
         // plain products and squares: two tiles per block when the launch still has at least
         // 256 tile pairs per prime (small launches keep one tile per block for parallelism)
         const uint32_t nj = std::min(kMaxGridY, njobs - j0);
@@ -1171,11 +1182,5 @@ hipError_t launch_garner_carry(const GarnerArgs& a, uint32_t max_out_len, uint32
     return hipGetLastError();
 }
 
-hipError_t launch_fill(uint32_t* p, uint32_t v, uint32_t count, hipStream_t st) {
-    if (count == 0) return hipSuccess;
-    (void)hipGetLastError();  // report this launch's error, not an earlier ignored one
-    hipLaunchKernelGGL(k_fill_u32, dim3((count + 255u) / 256u), dim3(256), 0, st, p, v, count);
-    return hipGetLastError();
-}
 
 }  // namespace bi

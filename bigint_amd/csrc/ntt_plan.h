@@ -44,6 +44,10 @@ struct PassArgs {
     // or null; forward and inverse stages use the same roots
     const uint32_t* beta;
     uint32_t beta_stride;      // blocks per prime
+    // The first kernel of a launch sequence zeroes these words (the carry-scan status words and
+    // tile counters of the sequence's Garner launch) before anything else; null elsewhere.
+    uint32_t* clear;
+    uint32_t clear_words;
 };
 
 struct GarnerConsts {
@@ -88,7 +92,6 @@ hipError_t launch_outer_pass(const PassArgs& a, bool inv, uint32_t njobs, uint32
 hipError_t launch_mid_pass(const PassArgs& a, uint32_t njobs, hipStream_t st);
 hipError_t launch_garner_carry(const GarnerArgs& a, uint32_t max_out_len, uint32_t njobs,
                                bool chunked, hipStream_t st);
-hipError_t launch_fill(uint32_t* p, uint32_t v, uint32_t count, hipStream_t st);
 // Diagonal sums of chunk spectra: diag[s] = sum over i + k = s of spec_a[i] * spec_b[k]
 // (pointwise Montgomery products mod each prime), s = 0 .. ca + cb - 2.  Spectra and diagonals
 // are 3 planes of n words each, consecutive.

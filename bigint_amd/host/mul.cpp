@@ -101,6 +101,17 @@ void add_engine_time_ns(uint64_t ns) noexcept { g_backend_ns.fetch_add(ns, std::
 
 auto load_planner(const char* path) -> bool { return backend::load_planner(path); }
 
+// End-to-end crossover of single products between the host schoolbook and the engine (bits, as
+// the reference counts them), measured on the MI355X box by bigint_amd/examples/crossover.cpp:
+// the schoolbook takes 48 us at 46,464 total bits and 99 us at 65,664, the engine 54-59 us there.
+constexpr size_type kHostCrossoverBits = 49152;
+static std::atomic<size_type> g_host_max_bits{kHostCrossoverBits};
+
+auto host_max_total_bits() noexcept -> size_type { return g_host_max_bits.load(std::memory_order_relaxed); }
+void set_host_max_total_bits(size_type bits) noexcept {
+    g_host_max_bits.store(std::max(bits, HOST_MAX_TOTAL_BITS), std::memory_order_relaxed);
+}
+
 // A product the reference would send to its NTT creates the thread pool at its default size
 // (first call wins), as the reference's get_pool() does inside ntt::mul.
 static void note_reference_route(size_type total_bits) {
@@ -110,7 +121,7 @@ static void note_reference_route(size_type total_bits) {
 auto mul_limbs(std::span<const uint32_t> a, std::span<const uint32_t> b, bool square) -> Digits {
     if (square) b = a;
     note_reference_route((words64(a) + words64(b)) * 64);
-    if ((words64(a) + words64(b)) * 64 <= HOST_MAX_TOTAL_BITS) {
+    if ((words64(a) + words64(b)) * 64 <= host_max_total_bits()) {
         g_host_products.fetch_add(1, std::memory_order_relaxed);
         Digits r = schoolbook(a, b);
         while (r.size() > 1 && r.back() == 0) r.pop_back();
@@ -154,7 +165,7 @@ auto mul_digits(const Digits& a, const Digits& b, size_type output_precision,
         while (lb > 1 && sb[lb - 1] == 0) --lb;
         const size_type keep = output_precision + 1;
         note_reference_route((words64(sa) + words64(sb)) * 64);
-        if ((words64(sa) + words64(sb)) * 64 > HOST_MAX_TOTAL_BITS && la + lb > keep) {
+        if ((words64(sa) + words64(sb)) * 64 > host_max_total_bits() && la + lb > keep) {
             Digits top(keep);
             const auto t0 = std::chrono::steady_clock::now();
             const size_type n = backend::multiply_top(sa.first(la), sb.first(lb), same, keep, top.data());

@@ -16,8 +16,17 @@
 
 namespace bigint::mul {
 
-// Largest total size (bits, counted as the reference counts them) multiplied on the host.
+// The reference's brute-force limit (mul.h:80): products of at most this many total bits
+// (counted in 64-bit words, as the reference counts them) are schoolbook products there.
 constexpr size_type HOST_MAX_TOTAL_BITS = 2 * 128 * 64;
+
+// Single products (mul_digits, operator*) of at most host_max_total_bits() total bits are
+// multiplied by the host schoolbook, larger ones by the engine.  The default is the crossover
+// measured end to end on the MI355X box (host schoolbook against the engine with its host copies);
+// set_host_max_total_bits changes it (never below HOST_MAX_TOTAL_BITS).  Batches
+// (mul_digits_batch) keep HOST_MAX_TOTAL_BITS: a batch shares its launch costs among its products.
+auto host_max_total_bits() noexcept -> size_type;
+void set_host_max_total_bits(size_type bits) noexcept;
 
 // The reference's route boundaries (mul.h:153, 192, 208-220): it multiplies products of
 // REF_FFT_MAX_TOTAL_BITS < (|A| + |B|) * 64 <= REF_NTT_MAX_TOTAL_BITS with its threaded NTT, whose

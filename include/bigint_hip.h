@@ -185,7 +185,8 @@ int bi_hip_transform_log2(size_t la, size_t lb);
 /* Per-kernel-class timing, for benchmarks: while enabled, every engine launch is bracketed by
  * HIP events on the stream it runs on, and its algorithmic HBM bytes are accumulated. */
 enum {
-    BI_HIP_K_FILL = 0,      /* status/counter clearing */
+    BI_HIP_K_FILL = 0,      /* status/counter clearing: done by the first kernel of each launch
+                               sequence, so no launch of this class is recorded */
     BI_HIP_K_OUTER_FWD = 1, /* strided forward passes after the first */
     BI_HIP_K_MID = 2,       /* contiguous block pass: forward + pointwise + inverse */
     BI_HIP_K_OUTER_INV = 3, /* strided inverse passes */

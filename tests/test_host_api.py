@@ -245,6 +245,13 @@ def test_products_stay_on_host_below_16384_bits(calc):
         assert unhx(calc("mul", hx(x), hx(y))) == x * y
     g2, _ = map(int, calc("counts").split())
     assert g2 == g1
+    # single products stay on the host up to the measured crossover, 49,152 total bits (in 64-bit
+    # words as the reference counts them); bigger ones go to the engine
+    for bits_a, bits_b in [(24576, 24576), (40000, 9152)]:
+        x, y = (1 << bits_a) - 1, (1 << bits_b) - 1
+        assert unhx(calc("mul", hx(x), hx(y))) == x * y
+    g3, _ = map(int, calc("counts").split())
+    assert g3 == g1
 
 
 @pytest.mark.gpu
@@ -351,8 +358,8 @@ def test_engine_failures_map_to_exceptions_without_gpu():
     import shutil
     if shutil.which("rocm-smi") and os.path.exists("/dev/kfd"):
         pytest.skip("a GPU may be present: the no-device path is not reachable")
-    big = hx((1 << 20000) - 3)
-    r = _calc_env_session([f"mul {big} {big}", "mul 7 6", f"fmul {big} 0 {big} 0 1000"], {})
+    big = hx((1 << 40000) - 3)  # 80,000 total bits: above the host route's limit
+    r = _calc_env_session([f"mul {big} {big}", "mul 7 6", f"fmul {big} 0 {big} 0 2000"], {})
     assert r[0].startswith("EXC ") and "bad_alloc" not in r[0], r[0]
     assert r[1] == "2a"
     assert r[2].startswith("EXC "), r[2]
