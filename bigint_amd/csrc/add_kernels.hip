@@ -85,13 +85,31 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t m, uint32_t* s_wave, uin
     return amap_compose(excl, prefix);
 }
 
-__device__ __forceinline__ void load_tile(const AddJob& j, size_t i0, uint32_t (&a)[kLimbsPerThread],
-                                          uint32_t (&b)[kLimbsPerThread]) {
+// LDS index of tile limb i: one pad word per 32 limbs, so the 16-limb runs that threads read
+// (thread t at 16 t) fall in distinct banks across each half-wave.
+__device__ __forceinline__ uint32_t tpad(uint32_t i) { return i + (i >> 5); }
+constexpr uint32_t kTileLds = kTileA + kTileA / 32;
+
+// Limbs [t0, t0 + 4096) of x (zero past len) into the thread's 16-limb run a[]: read from global
+// memory lane-contiguous (limb k * 256 + t) and regrouped through LDS.  Leaves s free on return.
+__device__ __forceinline__ void load_run(const uint32_t* x, size_t len, size_t t0, uint32_t* s,
+                                         uint32_t (&a)[kLimbsPerThread]) {
+    const uint32_t tid = threadIdx.x;
 #pragma unroll
-    for (uint32_t q = 0; q < kLimbsPerThread; ++q) {
-        a[q] = limb_at(j.x, j.lx, i0 + q);
-        b[q] = limb_at(j.y, j.ly, i0 + q);
+    for (uint32_t k = 0; k < kLimbsPerThread; ++k) {
+        const uint32_t i = k * kThreadsA + tid;
+        s[tpad(i)] = limb_at(x, len, t0 + i);
     }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t q = 0; q < kLimbsPerThread; ++q) a[q] = s[tpad(tid * kLimbsPerThread + q)];
+    __syncthreads();
+}
+
+__device__ __forceinline__ void load_tile(const AddJob& j, size_t t0, uint32_t* s, uint32_t (&a)[kLimbsPerThread],
+                                          uint32_t (&b)[kLimbsPerThread]) {
+    load_run(j.x, j.lx, t0, s, a);
+    load_run(j.y, j.ly, t0, s, b);
 }
 
 // grid.x = tile, grid.y = job.  maps[tile_base + t] = map (add) or xy | yx << 2 (subtract).
@@ -99,9 +117,9 @@ __global__ __launch_bounds__(kThreadsA) void k_add_maps(const AddJob* __restrict
     const AddJob j = load_desc(jobs + blockIdx.y);
     if (blockIdx.x >= j.ntiles) return;
     __shared__ uint32_t s_wave[kThreadsA / 64];
-    const size_t i0 = (size_t)blockIdx.x * kTileA + (size_t)threadIdx.x * kLimbsPerThread;
+    __shared__ uint32_t s_limbs[kTileLds];
     uint32_t a[kLimbsPerThread], b[kLimbsPerThread];
-    load_tile(j, i0, a, b);
+    load_tile(j, (size_t)blockIdx.x * kTileA, s_limbs, a, b);
     uint32_t total;
     if (!j.subtract) {
         block_scan(thread_map(a, b, false), s_wave, &total);
@@ -146,26 +164,32 @@ __global__ __launch_bounds__(kThreadsA) void k_add_apply(const AddJob* __restric
     if (blockIdx.x >= j.ntiles) return;
     __shared__ uint32_t s_wave[kThreadsA / 64];
     __shared__ uint32_t s_top;
+    __shared__ uint32_t s_limbs[kTileLds];
     if (threadIdx.x == 0) s_top = 0u;
     const bool swap = j.subtract && gmem(j.flags)[0] != 0u;
-    const size_t i0 = (size_t)blockIdx.x * kTileA + (size_t)threadIdx.x * kLimbsPerThread;
+    const size_t t0 = (size_t)blockIdx.x * kTileA;
     uint32_t a[kLimbsPerThread], b[kLimbsPerThread];
-    if (swap) load_tile(j, i0, b, a);
-    else load_tile(j, i0, a, b);
+    if (swap) load_tile(j, t0, s_limbs, b, a);
+    else load_tile(j, t0, s_limbs, a, b);
     uint32_t total;
     const uint32_t excl = block_scan(thread_map(a, b, j.subtract), s_wave, &total);
     const uint32_t cin_tile = (gmem(j.carry_in)[blockIdx.x] >> (swap ? 1u : 0u)) & 1u;
     uint32_t c = amap_apply(excl, cin_tile);
-    uint32_t top = 0u;  // 1 + index within the tile of the highest nonzero limb written
+    uint32_t top = 0u;  // 1 + index within the tile of the highest nonzero limb
 #pragma unroll
     for (uint32_t q = 0; q < kLimbsPerThread; ++q) {
         uint32_t o;
         c = step(a[q], b[q], c, j.subtract, o);
-        const size_t i = i0 + q;
-        if (i < j.width) {
-            gmem(j.out)[i] = o;
-            if (o) top = threadIdx.x * kLimbsPerThread + q + 1u;
-        }
+        const uint32_t i = threadIdx.x * kLimbsPerThread + q;
+        s_limbs[tpad(i)] = o;
+        if (o && t0 + i < j.width) top = i + 1u;
+    }
+    __syncthreads();
+    // the tile's limbs below width, written lane-contiguous
+#pragma unroll
+    for (uint32_t k = 0; k < kLimbsPerThread; ++k) {
+        const uint32_t i = k * kThreadsA + threadIdx.x;
+        if (t0 + i < j.width) gmem(j.out)[t0 + i] = s_limbs[tpad(i)];
     }
     if (top) atomicMax(&s_top, top);
     __syncthreads();

@@ -554,6 +554,7 @@ public:
             uint32_t* d_out, hipStream_t st, bool planned = true) {
         const int lg0 = bi_hip_transform_log2(la, lb);
         if (!product_supported(la, lb)) return BI_HIP_ETOOBIG;
+        if (needs_pieces(la, lb)) return run_pieces(d_a, la, d_b, lb, d_out, st);
         // the table (or the heuristic) may pick a chunked or re-split plan, and products beyond
         // one transform need one: those run through the batch path
         const bi_hip_dpair p{d_a, la, square ? nullptr : d_b, lb, d_out, (size_t)la + lb};
@@ -703,9 +704,16 @@ public:
     static constexpr uint64_t kMaxProductLimbs = 0xFFFFFFFFull - 8192u;
     static bool product_supported(size_t la, size_t lb) {
         if (la == 0 || lb == 0 || la > kMaxProductLimbs || lb > kMaxProductLimbs) return false;
-        if (la + lb > kMaxProductLimbs) return false;
-        if (bi_hip_transform_log2(la, lb) >= 0) return true;
-        return std::min(words_of((uint32_t)la), words_of((uint32_t)lb)) <= (1u << 26);
+        return la + lb <= kMaxProductLimbs;
+    }
+    // Products whose shorter operand has more than 2^26 words are beyond one set of diagonal sums
+    // (their coefficients would pass P0 P1 P2): the shorter operand is cut into limb-aligned
+    // pieces of kPieceLimbs limbs (2^26 - 4 words), each piece's product with the longer operand
+    // runs as a product of its own, and the partial products are added into the output at their
+    // limb offsets (run_pieces; the reference's SSA tier takes these sizes, mul.h:199-205).
+    static constexpr uint32_t kPieceLimbs = 76695840u;  // 8 * 9586980: words_of() = 67108860
+    static bool needs_pieces(size_t la, size_t lb) {
+        return std::min(words_of((uint32_t)la), words_of((uint32_t)lb)) > (1u << 26);
     }
     static constexpr int kNumPlanIds = 8;     // the plan space of the table and the MLP
     static constexpr int kSplitPlanId = 8;    // both operands chunked (explicit, or beyond the rest)
@@ -722,7 +730,7 @@ public:
     // are allowed only where their transform is shorter than the unchunked one.
     static bool plan_for_id(uint32_t la, uint32_t lb, bool sq, int id, PairPlan& out) {
         const int lg0 = bi_hip_transform_log2(la, lb);  // -1: beyond one transform
-        if (id == kSplitPlanId && product_supported(la, lb)) {
+        if (id == kSplitPlanId && product_supported(la, lb) && !needs_pieces(la, lb)) {
             out = PairPlan{26, kSplitChunk, 12, true};
             return true;
         }
@@ -950,7 +958,8 @@ public:
         using Key = std::tuple<int, uint32_t, uint32_t, int>;  // phase, lg, mid_lr, kind
         std::map<Key, std::vector<uint32_t>> groups;           // -> item indices
         std::vector<Item> items(count);
-        std::vector<uint32_t> split;  // products run one by one with both operands chunked
+        std::vector<uint32_t> split;   // products run one by one with both operands chunked
+        std::vector<uint32_t> pieces;  // products run one by one in pieces (run_pieces)
         size_t bhat_words = 0;
         for (size_t i = 0; i < count; ++i) {
             const bi_hip_dpair& p = pairs[i];
@@ -961,7 +970,9 @@ public:
             if (!product_supported(p.la, lb)) return BI_HIP_ETOOBIG;
             Item it{(uint32_t)i, (uint32_t)p.la, (uint32_t)lb, sq, false, chosen[i], 1, 0};
             const uint32_t lg = it.pp.lg, mid = it.pp.mid_lr;
-            if (it.pp.split_both) {
+            if (needs_pieces(p.la, lb)) {
+                pieces.push_back((uint32_t)i);
+            } else if (it.pp.split_both) {
                 split.push_back((uint32_t)i);
             } else if (it.pp.chunk_len) {
                 it.swap = it.la < it.lb;
@@ -1127,7 +1138,58 @@ public:
                                           sq, p.out, st);
             if (rc) return rc;
         }
+        for (uint32_t i : pieces) {
+            const bi_hip_dpair& p = pairs[i];
+            const bool sq = p.b == nullptr || (p.b == p.a && p.lb == p.la);
+            const int rc = run_pieces(p.a, (uint32_t)p.la, sq ? p.a : p.b, (uint32_t)(sq ? p.la : p.lb), p.out, st);
+            if (rc) return rc;
+        }
         return BI_HIP_OK;
+    }
+
+    // One product whose shorter operand has more than 2^26 words (see needs_pieces): piece i of
+    // the shorter operand B (limbs [i K, i K + K), K = kPieceLimbs) times A is computed by run()
+    // and added into out at limb offset i K, where the partial sum of the pieces before it has
+    // exactly la significant limbs; piece 0's product is written to out directly.  The partial
+    // products and the additions' scratch live in one allocation freed before returning, so this
+    // tier synchronises st.
+    int run_pieces(const uint32_t* A, uint32_t la, const uint32_t* B, uint32_t lb, uint32_t* out, hipStream_t st) {
+        if (la < lb) {
+            std::swap(A, B);
+            std::swap(la, lb);
+        }
+        const uint32_t np = (uint32_t)(((uint64_t)lb + kPieceLimbs - 1) / kPieceLimbs);
+        const size_t tmp_limbs = (size_t)la + kPieceLimbs;
+        const size_t ntiles = (tmp_limbs + kAddTileLimbs - 1) / kAddTileLimbs;
+        const size_t jobs_off = (tmp_limbs * 4 + 255) & ~(size_t)255;
+        const size_t words_off = jobs_off + (((size_t)np * sizeof(AddJob) + 255) & ~(size_t)255);
+        uint8_t* buf = nullptr;
+        if (call_malloc((void**)&buf, words_off + (2 * ntiles + 2) * 4) != hipSuccess) return BI_HIP_ENOMEM;
+        uint32_t* tmp = reinterpret_cast<uint32_t*>(buf);
+        uint32_t* maps = reinterpret_cast<uint32_t*>(buf + words_off);
+        std::vector<AddJob> jobs(np);
+        for (uint32_t i = 1; i < np; ++i) {
+            const size_t off = (size_t)i * kPieceLimbs, lp = std::min<size_t>(kPieceLimbs, lb - off);
+            const size_t width = (size_t)la + lp;  // the sum is below 2^(28 (la + lp)) at offset off
+            jobs[i] = AddJob{out + off, la, tmp, width, out + off, width, maps, maps + ntiles, maps + 2 * ntiles,
+                             0u, (uint32_t)((width + kAddTileLimbs - 1) / kAddTileLimbs)};
+        }
+        auto* d_jobs = reinterpret_cast<AddJob*>(buf + jobs_off);
+        int rc = hipMemcpyAsync(d_jobs, jobs.data(), np * sizeof(AddJob), hipMemcpyHostToDevice, st) == hipSuccess
+                     ? BI_HIP_OK
+                     : BI_HIP_EDEVICE;
+        for (uint32_t i = 0; i < np && rc == BI_HIP_OK; ++i) {
+            const size_t off = (size_t)i * kPieceLimbs;
+            const uint32_t lp = (uint32_t)std::min<size_t>(kPieceLimbs, lb - off);
+            rc = run(A, la, B + off, lp, false, i == 0 ? out : tmp, st);
+            // the add kernels read the partial product written by run()'s last kernel on st
+            if (rc == BI_HIP_OK && i > 0 && launch_add_batch(d_jobs + i, 1, jobs[i].ntiles, st) != hipSuccess)
+                rc = BI_HIP_EDEVICE;
+        }
+        // the buffer is read by kernels queued on st: they finish before it is freed
+        if (hipStreamSynchronize(st) != hipSuccess && rc == BI_HIP_OK) rc = BI_HIP_EDEVICE;
+        (void)hipFree(buf);
+        return rc;
     }
 
     // One product with both operands cut into kSplitChunk-word chunks (see PairPlan): forward
@@ -1765,7 +1827,7 @@ const char* bi_hip_strerror(int code) {
     switch (code) {
         case BI_HIP_OK: return "ok";
         case BI_HIP_EINVAL: return "invalid argument";
-        case BI_HIP_ETOOBIG: return "product needs a transform longer than 2^26 points";
+        case BI_HIP_ETOOBIG: return "product of 2^32 - 8192 limbs or more";
         case BI_HIP_ENOMEM: return "out of memory";
         case BI_HIP_EDEVICE: return "HIP runtime error";
         case BI_HIP_ENODEV: return "no gfx950 device";

@@ -136,18 +136,26 @@ RESIDUE_PRIMES = (2147483629, 2147483587, 2147483579, 2147483563, 2147483549, 21
 
 
 def residue_mod(limbs: np.ndarray, q: int) -> int:
-    """sum(limb_i * 2^(28 i)) mod q for q < 2^31, vectorised (O(n))."""
-    limbs = np.asarray(limbs, dtype=np.uint64)
+    """sum(limb_i * 2^(28 i)) mod q for q < 2^31, vectorised (O(n)), in chunks of 2^22 limbs so
+    that products of billions of bits need little host memory."""
+    limbs = np.asarray(limbs)
     n = limbs.size
     if n == 0:
         return 0
     qq = np.uint64(q)
+    chunk = min(n, 1 << 22)
     w = np.ones(1, dtype=np.uint64)
     step = (1 << 28) % q  # 2^(28*len(w)) mod q
-    while w.size < n:
+    while w.size < chunk:
         w = np.concatenate([w, (w * np.uint64(step)) % qq])
         step = step * step % q
-    w = w[:n]
-    terms = (limbs % qq) * w % qq  # < 2^31 * 2^31 = 2^62, fits
-    # reduce the sum in chunks to avoid overflow: each term < 2^31, so 2^32 terms fit in 2^63
-    return int(terms.sum(dtype=np.uint64) % qq) if n < (1 << 32) else int(sum(int(t) for t in terms) % q)
+    w = w[:chunk]
+    shift = pow(2, 28 * chunk, q)  # weight of a chunk relative to the one below it
+    r = 0
+    for start in range(((n - 1) // chunk) * chunk, -1, -chunk):  # top chunk first (Horner)
+        part = limbs[start:start + chunk].astype(np.uint64)
+        if q <= 1 << 28:  # a limb (below 2^28) may exceed q
+            part %= qq
+        terms = part * w[:part.size] % qq  # each < 2^31: 2^22 of them sum below 2^53
+        r = (r * shift + int(terms.sum(dtype=np.uint64))) % q
+    return r

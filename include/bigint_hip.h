@@ -17,8 +17,11 @@
  *                          (src/mul/ntt_multithread.cpp:287-290)
  * Preconditions follow the reference: operands are nonzero with a nonzero top limb
  * (mul.h:222, 232).  Where the reference hits unreachable() (bigint_base.h:40-46) this ABI
- * returns BI_HIP_EINVAL instead.  A product needing a transform longer than 2^26 points
- * (ntt_base.h:22-27; the reference sends it to SSA, mul.h:199-219) returns BI_HIP_ETOOBIG.
+ * returns BI_HIP_EINVAL instead.  Products beyond one 2^26-point transform (ntt_base.h:22-27; the
+ * reference sends them to SSA, mul.h:199-219) are computed from chunked transforms, and past a
+ * shorter operand of 2^26 words from pieces of it added on the device.  Such products first wait
+ * on the host for the work already queued on the stream (the pieces also for their own work, before
+ * returning).  Only a product of 2^32 - 8192 limbs or more returns BI_HIP_ETOOBIG.
  *
  * Thread safety: every entry point may be called from any thread; calls are serialised by an
  * internal mutex (the reference serialises pool phases the same way,
@@ -43,7 +46,7 @@ extern "C" {
 enum {
     BI_HIP_OK = 0,
     BI_HIP_EINVAL = 1,   /* bad argument: null pointer, zero length, top limb zero (host paths) */
-    BI_HIP_ETOOBIG = 2,  /* product beyond a 2^26-point transform (see bi_hip_transform_log2) */
+    BI_HIP_ETOOBIG = 2,  /* product of 2^32 - 8192 limbs or more */
     BI_HIP_ENOMEM = 3,   /* device or host allocation failed */
     BI_HIP_EDEVICE = 4,  /* HIP runtime error */
     BI_HIP_ENODEV = 5,   /* no gfx950 device */
@@ -217,7 +220,8 @@ int bi_hip_profile_read(bi_hip_kstat* stats, size_t cap);
  *   6, 7     longer operand cut into 2 / 3 chunks (lengths multiples of the 3584-word carry tile)
  *   8        both operands cut into 2^25-word chunks on 2^26-point transforms, chunk-pair spectra
  *            summed per diagonal (outside the learned plan space; used for products beyond one
- *            transform that no chunk plan fits, and accepted as an explicit id)
+ *            transform that no chunk plan fits, and accepted as an explicit id), for a shorter
+ *            operand of at most 2^26 words; longer ones run in pieces of that size
  * A chunked product transforms the shorter operand once and multiplies every chunk with that
  * transform; chunk plans are valid only where their transform is shorter than the unchunked one.
  * These replace the reference's hand-set size thresholds (include/bigint/mul.h:80, 153, 192). */

@@ -13,7 +13,7 @@ import random
 import numpy as np
 import pytest
 
-from bigint_amd import engine
+from bigint_amd import engine, hip
 from bigint_amd import limbs as L
 from oracle import oracle
 
@@ -127,11 +127,13 @@ def test_error_paths():
     with pytest.raises(engine.EngineError) as e:
         engine.mul(np.array([1, 0], np.uint32), a)  # top limb zero
     assert e.value.code == engine.EINVAL
-    # beyond what three primes can reconstruct: both operands above 2^26 words
-    big = np.ones(76695851, np.uint32)
-    with pytest.raises(engine.EngineError) as e:
-        engine.mul(big, big)
-    assert e.value.code == engine.ETOOBIG
+    # a product of 2^32 - 8192 limbs or more is rejected on the host before any device work (the
+    # lengths are checked before the pointers are used)
+    buf = hip.Buffer(64)
+    for la, lb in [(1 << 31, 1 << 31), ((1 << 32) - 8192, 1), (1, (1 << 32) + 5)]:
+        with pytest.raises(engine.EngineError) as e:
+            engine.mul_device(buf.ptr, la, buf.ptr, lb, False, buf.ptr, la + lb, None)
+        assert e.value.code == engine.ETOOBIG, (la, lb)
 
 
 def test_mul_top_matches_full_product():
@@ -356,6 +358,59 @@ def test_beyond_one_transform_long_times_short():
     b = L.random_operand(28 * lb, 42)
     p = engine.mul(a, b)
     check_residues3(a, b, p)
+
+
+def _fast_operand(n: int, seed: int) -> np.ndarray:
+    """n random 28-bit limbs with a nonzero top limb (numpy's generator: SplitMix64 repacking
+    takes seconds per 10^8 limbs, and the residue checks do not depend on the generator)."""
+    x = np.random.default_rng(seed).integers(0, 1 << 28, size=n, dtype=np.uint32)
+    x[-1] |= 1 << 27
+    return x
+
+
+def test_beyond_three_primes_in_pieces_two():
+    """Both operands above 2^26 words (76,695,851 limbs, 2.15 Gbit): one set of diagonal sums
+    would exceed P0 P1 P2, so the shorter operand runs in two pieces (76,695,840 limbs and 11)
+    whose products are added on the device; also the square, whose pieces are cut from A."""
+    n = 76_695_851
+    assert engine.transform_log2(n, n) == -1
+    assert not engine.lib().bi_hip_plan_allowed(n, n, 0, 8)
+    a = _fast_operand(n, 61)
+    b = _fast_operand(n, 62)
+    p = engine.mul(a, b)
+    assert p.size in (2 * n, 2 * n - 1)
+    check_residues3(a, b, p)
+    check_residues3(a, None, engine.mul(a, None))
+
+
+def test_beyond_three_primes_in_pieces_three():
+    """4.3 Gbit x 4.4 Gbit: the shorter operand (153,396,680 limbs, after the operands are
+    swapped) is cut into three pieces, the last of 5,000 limbs; two device additions carry the
+    partial products into the output."""
+    la, lb = 2 * 76_695_840 + 5_000, 2 * 76_695_840 + 3_000_000
+    a = _fast_operand(la, 63)
+    b = _fast_operand(lb, 64)
+    p = engine.mul(a, b)
+    assert p.size in (la + lb, la + lb - 1)
+    check_residues3(a, b, p)
+
+
+def test_pieces_product_inside_a_batch():
+    """A pieces product in a grouped batch (run one by one after the batch's launch sets) next to
+    small products, in table and learned modes: every product exact."""
+    n = 76_695_851 + 700
+    big_a, big_b = _fast_operand(n, 65), _fast_operand(n - 100, 66)
+    rng = random.Random(67)
+    pairs = [(L.random_operand(rng.randrange(3000, 90000), rng.getrandbits(63)),
+              L.random_operand(rng.randrange(3000, 90000), rng.getrandbits(63))) for _ in range(6)]
+    pairs.insert(3, (big_a, big_b))
+    for mode in (engine.PLAN_TABLE, engine.PLAN_LEARNED):
+        got = engine.mul_batch(pairs, mode)
+        for i, ((a, b), p) in enumerate(zip(pairs, got)):
+            if i == 3:
+                check_residues3(a, b, p)
+            else:
+                assert np.array_equal(p, oracle.mul_limbs(a, b)), (mode, i)
 
 
 def test_batch_graph_replays_match_eager():
