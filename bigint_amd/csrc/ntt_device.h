@@ -64,6 +64,19 @@ __device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t b) {
     return r2 < r ? r2 : r;  // unsigned min: r - p wraps above r exactly when r < p
 }
 
+// t * 2^-32 mod P (Montgomery reduction of a 64-bit value) for t < 2^32 * P and
+// t < 2^64 - 2^32 * P: then t + u*P does not overflow and (t + u*P) / 2^32 < 2P, so one
+// conditional subtraction finishes it.  The caller guarantees the bound (P2: t < 8.6e18).
+template <int I>
+__device__ __forceinline__ uint32_t mont_redc(uint64_t t) {
+    constexpr uint32_t p = kP[I];
+    const uint32_t u = (uint32_t)t * kNegInvP[I];
+    const uint64_t s = t + (uint64_t)u * p;
+    const uint32_t r = (uint32_t)(s >> 32);
+    const uint32_t r2 = r - p;
+    return r2 < r ? r2 : r;
+}
+
 // Montgomery product without the final subtraction: for a < 4P, b < P and 4P < 2^32 (P0 only)
 // the result lies in [0, 2P).
 template <int I>

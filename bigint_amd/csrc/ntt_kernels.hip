@@ -664,20 +664,32 @@ __device__ __forceinline__ uint32_t fold1p(uint32_t x) {
     return y < x ? y : x;
 }
 
+// Garner constants of one job as the CRT uses them.  ks[i] = n^-1 R^2 mod P_i is the job's scale
+// (mont_mul(y, ks[i]) = y n^-1 R); a1 = ks[1] inv01 R^-1 and a2 = ks[2] inv02 R^-1 fold it into the
+// first Garner products, and m01 = P1 - inv01, m02 = P2 - inv02 are their negated factors.
+struct CrtConsts {
+    uint32_t ks0, a1, a2, m01, m02, inv12;
+};
+
+__device__ __forceinline__ CrtConsts crt_consts(const uint32_t (&ks)[3], const GarnerConsts& k) {
+    return CrtConsts{ks[0], mont_mul<1>(ks[1], k.inv01), mont_mul<2>(ks[2], k.inv02), kP[1] - k.inv01,
+                     kP[2] - k.inv02, k.inv12};
+}
+
 // Coefficient from its three inverse-transform outputs r0, r1, r2 (each still carrying the
-// factor n * R^-1, removed by the Garner constants ks); r0 may lie in [0, 2 P0).
+// factor n * R^-1); r0 may lie in [0, 2 P0), r1 < P1, r2 < P2.  Garner's
+//   v0 = c0,  v1 = (c1 - v0) inv01 mod P1,  t = (c2 - v0) inv02 mod P2,  v2 = (t - v1) inv12 mod P2
+// with c_i = r_i n^-1 (mod P_i) computes v1 and t each as one Montgomery reduction of a two-product
+// sum: r1 a1 + v0 (P1 - inv01) < P1^2 + P0 P1 < 4.1e18 + 9.5e17 and r2 a2 + v0 (P2 - inv02)
+// < P2^2 + P0 P2 < 5.3e18 + 1.1e18, both inside mont_redc's bound because v0 < P0 is small.
 __device__ __forceinline__ unsigned __int128 crt_coeff(uint32_t r0, uint32_t r1, uint32_t r2,
-                                                       const uint32_t (&ks)[3],
-                                                       const GarnerConsts& k) {
-    const uint32_t c0 = mont_mul<0>(r0, ks[0]);
-    const uint32_t c1 = mont_mul<1>(r1, ks[1]);
-    const uint32_t c2 = mont_mul<2>(r2, ks[2]);
-    const uint32_t v0 = c0;
+                                                       const CrtConsts& c) {
+    const uint32_t v0 = mont_mul<0>(r0, c.ks0);
     // v0 < P0 < P1 < P2 and v1 < P1 < P2 are already residues of the larger primes
     static_assert(kP[0] < kP[1] && kP[1] < kP[2], "CRT order");
-    const uint32_t v1 = mont_mul<1>(sub_mod<1>(c1, v0), k.inv01);
-    const uint32_t t = mont_mul<2>(sub_mod<2>(c2, v0), k.inv02);
-    const uint32_t v2 = mont_mul<2>(sub_mod<2>(t, v1), k.inv12);
+    const uint32_t v1 = mont_redc<1>((uint64_t)r1 * c.a1 + (uint64_t)v0 * c.m01);
+    const uint32_t t = mont_redc<2>((uint64_t)r2 * c.a2 + (uint64_t)v0 * c.m02);
+    const uint32_t v2 = mont_mul<2>(sub_mod<2>(t, v1), c.inv12);
     const uint64_t inner = (uint64_t)v1 + (uint64_t)kP[1] * v2;  // < 2^62.1
     return (unsigned __int128)inner * kP[0] + v0;
 }
@@ -720,11 +732,12 @@ static_assert(kCarryTile == kCarryTileLimbs && kWordTile == kCarryTileWords, "ti
 
 // Coefficient j of a job from its three (chunk-summed) residues, split into 32-bit pieces.  An
 // unchunked product whose top coefficient wrapped from index n onto index 0 has it taken out of
-// coefficient 0 and put back at index n (whose residues read as zero).
+// coefficient 0 and put back at index n (whose residues read as zero); `wrap` is false where the
+// caller knows j is neither (a tile away from both ends).
 __device__ __forceinline__ Pieces coeff_pieces(uint32_t r0, uint32_t r1, uint32_t r2, int64_t j,
-                                               const GarnerJob& jb, const GarnerConsts& k) {
-    unsigned __int128 x = crt_coeff(r0, r1, r2, jb.kscale, k);
-    if (jb.wrap_a) {
+                                               const GarnerJob& jb, const CrtConsts& c, bool wrap) {
+    unsigned __int128 x = crt_coeff(r0, r1, r2, c);
+    if (wrap) {
         if (j == 0) x -= wrap_product(jb);
         else if (j == (int64_t)jb.n) x = wrap_product(jb);
     }
@@ -756,7 +769,9 @@ __global__ __launch_bounds__(kGThreads) void k_garner_carry(GarnerArgs a) {
     const GarnerJob jb = a.jobs ? load_desc(a.jobs + blockIdx.y) : a.job;
     constexpr uint32_t kWaves = kGThreads / 64;
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_wave[kWaves];
+    __shared__ uint32_t s_wave[kWaves];     // slow path: each wave's inclusive carry map
+    __shared__ uint32_t s_wc[kWaves];       // lane 63 of each wave: carry-out of its run for carry-in 0
+    __shared__ uint32_t s_wconst[kWaves];   // each wave: all its runs pass on a fixed carry
     __shared__ uint32_t s_carry_in;
     __shared__ uint32_t s_edge[kWaves][3];  // lane 63 of each wave: w1 and w2 of its last word, w2 of the one before
     __shared__ uint32_t s_halo[3][3];       // pieces of words base - 3, base - 2, base - 1
@@ -768,6 +783,11 @@ __global__ __launch_bounds__(kGThreads) void k_garner_carry(GarnerArgs a) {
     if (tile >= ntiles) return;
     const uint32_t base = tile * kWordTile;  // first word (coefficient) of the tile
     const uint32_t n = jb.n;
+    // An interior tile reads only aligned position pairs inside the plane (its window
+    // [base - 4, base + 3584] lies in [2, n - 2]) and none of its coefficients is index 0 or n.
+    const bool interior = !CHUNKED && base >= 6u && (uint64_t)base + kWordTile + 2u <= n;
+    const CrtConsts crt = crt_consts(jb.kscale, a.k);
+    const bool wrap = jb.wrap_a != nullptr && !interior;
 
     // 1. Residues.  The block's words [base - 4, base + kWordTile) of each plane are read coalesced
     //    and staged in LDS one plane at a time, where each thread picks up its run of 7.  Word w
@@ -792,7 +812,19 @@ __global__ __launch_bounds__(kGThreads) void k_garner_carry(GarnerArgs a) {
         for (uint32_t p = 0; p < 3u; ++p)
 #pragma unroll
             for (uint32_t k = 0; k < kPairLoads; ++k) v[p][k] = u32x2{0u, 0u};
-        if constexpr (!CHUNKED) {
+        if (interior) {
+            // pair i (words base + 2i, base + 2i - 1) at positions n - base - 2i, +1
+#pragma unroll
+            for (uint32_t p = 0; p < 3u; ++p) {
+                const uint32_t* yp = jb.y + (size_t)p * n + (n - base);
+#pragma unroll
+                for (uint32_t k = 0; k < kPairLoads; ++k) {
+                    const uint32_t i = k * kGThreads + tid;
+                    if (i < kNPairs) v[p][k] = *gmem(reinterpret_cast<const u32x2*>(yp - 2u * i));
+                }
+                if (has_end) h[p] = *gmem(reinterpret_cast<const u32x2*>(yp - 2 * ie));
+            }
+        } else if constexpr (!CHUNKED) {
 #pragma unroll
             for (uint32_t p = 0; p < 3u; ++p) {
                 const uint32_t* yp = jb.y + (size_t)p * n;
@@ -868,7 +900,7 @@ __global__ __launch_bounds__(kGThreads) void k_garner_carry(GarnerArgs a) {
         uint64_t acc1 = 0, acc2 = 0;  // partial sums of the next word and the one after
 #pragma unroll
         for (uint32_t q = 0; q < kCarryWordsPerThread; ++q) {
-            const Pieces pc = coeff_pieces(r[0][q], r[1][q], r[2][q], j0 + q, jb, a.k);
+            const Pieces pc = coeff_pieces(r[0][q], r[1][q], r[2][q], j0 + q, jb, crt, wrap);
             const uint64_t sum = acc1 + pc.w0;
             s_lo[q] = (uint32_t)sum;
             s_hi[q] = (uint32_t)(sum >> 32);
@@ -882,7 +914,7 @@ __global__ __launch_bounds__(kGThreads) void k_garner_carry(GarnerArgs a) {
         }
     }
     if (has_end) {
-        const Pieces pc = coeff_pieces(hr[0], hr[1], hr[2], (int64_t)base - 3 + (int64_t)tid, jb, a.k);
+        const Pieces pc = coeff_pieces(hr[0], hr[1], hr[2], (int64_t)base - 3 + (int64_t)tid, jb, crt, wrap);
         s_halo[tid][0] = pc.w0;
         s_halo[tid][1] = pc.w1;
         s_halo[tid][2] = pc.w2;
@@ -927,40 +959,68 @@ __global__ __launch_bounds__(kGThreads) void k_garner_carry(GarnerArgs a) {
         if ((uint32_t)sl <= 0xFFFFFFFDu) local = (uint32_t)(sl >> 32);
     }
 
-    // 3. this thread's carry map (carry-out for each carry-in 0..3, four chains run side by
-    //    side), then an inclusive scan of maps across the block
-    uint32_t m;
-    {
-        uint32_t c[4] = {0u, 1u, 2u, 3u};
+    // 3. Carries inside the tile.  The carry into a word is at most 2, and a run's carry-out is a
+    //    non-decreasing function of its carry-in, so a run whose carry-outs for carry-ins 0 and 2
+    //    agree passes on that carry whatever it receives (in random data every run does: only a
+    //    word sum within 2 of a multiple of 2^32 can propagate).  When all runs of the tile are
+    //    such, the carry into run t is the carry-out of run t - 1 and the tile's own carry-out is
+    //    that of its last run.  Otherwise each run's carry map (carry-out for each carry-in 0..3,
+    //    four chains side by side) is scanned across the block.
+    uint32_t co0 = 0u, co2 = 2u;
 #pragma unroll
-        for (uint32_t q = 0; q < kCarryWordsPerThread; ++q) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const uint32_t v = s_lo[q] + c[k];
-                c[k] = s_hi[q] + (v < c[k] ? 1u : 0u);
-            }
-        }
-        m = c[0] | (c[1] << 2) | (c[2] << 4) | (c[3] << 6);
+    for (uint32_t q = 0; q < kCarryWordsPerThread; ++q) {
+        const uint32_t v0 = s_lo[q] + co0;
+        co0 = s_hi[q] + (v0 < co0 ? 1u : 0u);
+        const uint32_t v2 = s_lo[q] + co2;
+        co2 = s_hi[q] + (v2 < co2 ? 1u : 0u);
     }
-    uint32_t inc = m;
-#pragma unroll
-    for (uint32_t d = 1; d < 64; d <<= 1) {
-        const uint32_t o = __shfl_up(inc, d, 64);
-        if (lane >= d) inc = map_compose(inc, o);
+    const bool wconst = __ballot(co0 == co2) == ~0ull;
+    if (lane == 63u) {
+        s_wc[wave] = co0;
+        s_wconst[wave] = wconst ? 1u : 0u;
     }
-    if (lane == 63) s_wave[wave] = inc;
     __syncthreads();
-    uint32_t wave_prefix = kMapIdentity;
-    for (uint32_t w = 0; w < wave; ++w) wave_prefix = map_compose(s_wave[w], wave_prefix);
-    uint32_t excl = __shfl_up(inc, 1, 64);
-    if (lane == 0) excl = kMapIdentity;
-    excl = map_compose(excl, wave_prefix);
+    bool fixed_runs = true;
+#pragma unroll
+    for (uint32_t w = 0; w < kWaves; ++w) fixed_runs = fixed_runs && s_wconst[w] != 0u;
+    uint32_t excl = kMapIdentity;  // slow path: composition of the maps of runs 0 .. t-1
+    uint32_t agg;                  // the tile's carry map
+    if (fixed_runs) {
+        agg = s_wc[kWaves - 1u] * 0x55u;  // a constant map
+    } else {
+        uint32_t m;
+        {
+            uint32_t c[4] = {0u, 1u, 2u, 3u};
+#pragma unroll
+            for (uint32_t q = 0; q < kCarryWordsPerThread; ++q) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t v = s_lo[q] + c[k];
+                    c[k] = s_hi[q] + (v < c[k] ? 1u : 0u);
+                }
+            }
+            m = c[0] | (c[1] << 2) | (c[2] << 4) | (c[3] << 6);
+        }
+        uint32_t inc = m;
+#pragma unroll
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(inc, d, 64);
+            if (lane >= d) inc = map_compose(inc, o);
+        }
+        if (lane == 63) s_wave[wave] = inc;
+        __syncthreads();
+        uint32_t wave_prefix = kMapIdentity;
+        for (uint32_t w = 0; w < wave; ++w) wave_prefix = map_compose(s_wave[w], wave_prefix);
+        excl = __shfl_up(inc, 1, 64);
+        if (lane == 0) excl = kMapIdentity;
+        excl = map_compose(excl, wave_prefix);
+        agg = kMapIdentity;
+        for (uint32_t w = 0; w < kWaves; ++w) agg = map_compose(s_wave[w], agg);
+    }
 
     // 4. carry into the tile: resolved locally, or by a decoupled look-back of the first wave
     //    (64 predecessor tiles read at once).  Every tile publishes its status either way.
     if (wave == 0) {
-        uint32_t agg = kMapIdentity;
-        for (uint32_t w = 0; w < kWaves; ++w) agg = map_compose(s_wave[w], agg);
         uint32_t carry_in = 0;
         const uint32_t loc = __shfl(local, 0, 64);
         if (tile == 0) {
@@ -1004,10 +1064,13 @@ __global__ __launch_bounds__(kGThreads) void k_garner_carry(GarnerArgs a) {
         }
         if (lane == 0) s_carry_in = carry_in;
     }
+    // the carry-out of run t - 1 (fast path), read before the barrier so every lane takes part
+    uint32_t prev = __shfl_up(co0, 1, 64);
+    if (lane == 0u) prev = wave > 0u ? s_wc[wave - 1u] : 0u;
     __syncthreads();
 
     // 5. apply the carries, cut the 7 words into 8 limbs and write them
-    uint32_t carry = map_apply(excl, s_carry_in);
+    uint32_t carry = fixed_runs ? (tid == 0u ? s_carry_in : prev) : map_apply(excl, s_carry_in);
     uint32_t wv[kCarryWordsPerThread];
 #pragma unroll
     for (uint32_t q = 0; q < kCarryWordsPerThread; ++q) {

@@ -279,12 +279,18 @@ def garner_carry(coeff, out_len):
             for q in range(GW):
                 s = sums[GW * t + q]
                 c = [(s + ck) >> 32 for ck in c]
-            assert max(c) <= 3
+            assert max(c) <= 3 and c[0] <= c[1] <= c[2]  # carry-out non-decreasing in carry-in
             maps.append(c[0] | c[1] << 2 | c[2] << 4 | c[3] << 6)
+        # the kernel's fast path: every run passes on a fixed carry (carry-outs for carry-ins 0
+        # and 2 agree), so run t receives run t - 1's carry-out for carry-in 0
+        co0 = [map_apply(m, 0) for m in maps]
+        fixed_runs = all(map_apply(m, 0) == map_apply(m, 2) for m in maps)
         excl = 0b11100100
         for t in range(GT):
             carry = map_apply(excl, carry_in)
             assert carry <= 2
+            if fixed_runs:
+                assert carry == (carry_in if t == 0 else co0[t - 1])
             words = []
             for q in range(GW):
                 v = sums[GW * t + q] + carry
@@ -656,6 +662,20 @@ def k_mont(I, a, b, lazy=False):
     return r if lazy or r < P[I] else r - P[I]
 
 
+def k_redc(I, t):
+    """Kernel mont_redc: Montgomery reduction of a 64-bit value, asserting its bounds (t below
+    2^32 P and below 2^64 - 2^32 P, so t + u*P fits 64 bits and the shifted value is below 2P)."""
+    assert 0 <= t < (1 << 32) * P[I] and t < (1 << 64) - (1 << 32) * P[I]
+    u = ((t & M32) * (P[I] - 2)) & M32
+    s = t + u * P[I]
+    assert s < (1 << 64) and s % R32 == 0
+    r = s >> 32
+    assert r < 2 * P[I]
+    r = r if r < P[I] else r - P[I]
+    assert r == t * pow(R32, -1, P[I]) % P[I]
+    return r
+
+
 def k_add_mod(I, a, b):
     """Kernel add_mod in 32-bit words: for P < 2^31 the sum fits and an unsigned min of s and
     s - P selects; for P2 the carry out of 32 bits and the borrow of s - P decide."""
@@ -788,9 +808,15 @@ def kernel_crt(r0, r1, r2, n):
     inv02 = to_mont(2, pow(P[0], -1, P[2]))
     inv12 = to_mont(2, pow(P[1], -1, P[2]))
     assert r0 < 2 * P[0] and r1 < P[1] and r2 < P[2]
+    # crt_consts: the scale folded into the first Garner products, their factors negated
+    a1, a2 = k_mont(1, ks[1], inv01), k_mont(2, ks[2], inv02)
+    m01, m02 = P[1] - inv01, P[2] - inv02
     v0 = k_mont(0, r0, ks[0])
-    v1 = k_mont(1, k_sub_mod(1, k_mont(1, r1, ks[1]), v0), inv01)
-    t = k_mont(2, k_sub_mod(2, k_mont(2, r2, ks[2]), v0), inv02)
+    # v1 = (c1 - v0) inv01 and t = (c2 - v0) inv02 (c_i = r_i n^-1) as one reduction each
+    v1 = k_redc(1, r1 * a1 + v0 * m01)
+    t = k_redc(2, r2 * a2 + v0 * m02)
+    assert v1 == k_mont(1, k_sub_mod(1, k_mont(1, r1, ks[1]), v0), inv01)
+    assert t == k_mont(2, k_sub_mod(2, k_mont(2, r2, ks[2]), v0), inv02)
     assert v1 < P[1] < P[2]  # already a residue mod P2
     v2 = k_mont(2, k_sub_mod(2, t, v1), inv12)
     x = v0 + P[0] * (v1 + P[1] * v2)
@@ -804,6 +830,9 @@ def test_crt_recovers_every_coefficient():
     them."""
     rng = random.Random(7)
     assert P[0] * P[1] * P[2] > (1 << 26) * ((1 << 32) - 1) ** 2
+    for I in (1, 2):  # mont_redc's bound at the largest two-product sums of the CRT
+        t = (P[I] - 1) ** 2 + (P[0] - 1) * (P[I] - 1)
+        assert t < (1 << 32) * P[I] and t < (1 << 64) - (1 << 32) * P[I]
     xs = [0, 1, P[0], P[0] * P[1], P[2] * P[1] - 1, (1 << 26) * ((1 << 32) - 1) ** 2]
     xs += [rng.randrange(1 << 90) for _ in range(3000)] + [rng.randrange(1 << 64) for _ in range(500)]
     for n in (2, 1 << 11, 1 << 24):
