@@ -96,12 +96,11 @@ def c1_cpu() -> dict:
     return cpu_time.c1(0)
 
 
-def batch_leg(hip, engine, L, pairs_count: int, steps: int) -> dict:
-    """The batched product-tree workload: `pairs_count` independent products whose operand sizes
-    are log-uniform in [2^12, 2^24] bits (SplitMix64 from the fixed seed), uniform random limbs
-    from a seeded generator with the exact top bit set, resident in HBM.  Timed per plan mode;
-    every mode's products must equal the plain mode's limb for limb, and a sample is checked
-    against residues on the host."""
+def batch_inputs(hip, L, pairs_count: int):
+    """Operands of the batched workload in one device pool: log-uniform bit lengths in
+    [2^12, 2^24] (SplitMix64 from the fixed seed), uniform random limbs from a seeded generator
+    with the exact top bit set; pair i's operands at offa[i], offb[i] (its product's slot in an
+    output pool of the same layout starts at offa[i])."""
     bits = L.batch_bits(pairs_count)
     la = np.array([-(-ba // 28) for ba, _ in bits], np.int64)
     lb = np.array([-(-bb // 28) for _, bb in bits], np.int64)
@@ -115,6 +114,66 @@ def batch_leg(hip, engine, L, pairs_count: int, steps: int) -> dict:
             r = nb - 28 * (n - 1)
             host[off + n - 1] = (host[off + n - 1] & ((1 << r) - 1)) | (1 << (r - 1))
     pool = hip.Buffer.from_array(host)
+    return bits, la, lb, offa, offb, host, pool, tot_in
+
+
+def batch_only(hip, engine, L, pairs_count: int, mode: str, steps: int) -> int:
+    """The batched workload in one plan mode only (one warm-up batch, then `steps` timed
+    batches): the command that kernel traces of the batch are taken with."""
+    from bigint_amd import planner
+    engine.init(0)
+    if os.path.exists(planner.DEFAULT_PATH):
+        planner.load()
+    m = {"plain": engine.PLAN_FIXED, "table": engine.PLAN_TABLE, "learned": engine.PLAN_LEARNED}[mode]
+    bits, la, lb, offa, offb, host, pool, tot_in = batch_inputs(hip, L, pairs_count)
+    out = hip.Buffer(4 * tot_in)
+    stream = hip.Stream()
+    pairs = [(pool.ptr + 4 * int(offa[i]), int(la[i]), pool.ptr + 4 * int(offb[i]), int(lb[i]),
+              out.ptr + 4 * int(offa[i]), int(la[i] + lb[i])) for i in range(pairs_count)]
+    engine.mul_batch_device(pairs, m, stream.handle)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        engine.mul_batch_device(pairs, m, stream.handle)
+    stream.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    print(json.dumps({"batch_mode": mode, "pairs": pairs_count, "ms": round(dt * 1e3, 3),
+                      "Gbit_s": round(sum(a + b for a, b in bits) / dt / 1e9, 2)}), flush=True)
+    return 0
+
+
+def valu_rates() -> int:
+    """Issue rate of v_add_u32, v_mul_lo_u32, v_mul_hi_u32 and v_mad_u64_u32 (libbigint_rates.so:
+    eight independent chains per thread, eight waves per SIMD), in wave-instructions per CU per
+    clock: the peak the kernels' VALU issue fractions are stated against."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(ROOT, "bigint_amd", "libbigint_rates.so"))
+    cus, khz = ctypes.c_int(), ctypes.c_int()
+    if lib.bi_rates_device(ctypes.byref(cus), ctypes.byref(khz)):
+        raise RuntimeError("bi_rates_device failed")
+    blocks, iters = 16 * cus.value, 20000
+    forms = [("v_add_u32", 8), ("v_mul_lo_u32", 8), ("v_mul_hi_u32 + v_xor_b32", 16),
+             ("v_mad_u64_u32 + v_xor_b32", 16)]
+    rows = []
+    for op, (name, per_iter) in enumerate(forms):
+        ms = ctypes.c_float()
+        rc = lib.bi_rates_run(op, blocks, iters, ctypes.byref(ms))
+        if rc:
+            raise RuntimeError(f"bi_rates_run({op}) failed: {rc}")
+        per_ns = blocks * 4 * iters * per_iter / (ms.value * 1e-3) / cus.value / 1e9
+        rows.append({"form": name, "ms": round(ms.value, 3), "wave_instr_per_CU_per_ns": round(per_ns, 3)})
+    print(json.dumps({"valu_rates": rows, "CUs": cus.value, "peak_clock_MHz": khz.value / 1e3,
+                      "blocks": blocks, "threads_per_block": 256, "iters": iters}), flush=True)
+    return 0
+
+
+def batch_leg(hip, engine, L, pairs_count: int, steps: int) -> dict:
+    """The batched product-tree workload: `pairs_count` independent products whose operand sizes
+    are log-uniform in [2^12, 2^24] bits (SplitMix64 from the fixed seed), uniform random limbs
+    from a seeded generator with the exact top bit set, resident in HBM.  Timed per plan mode;
+    every mode's products must equal the plain mode's limb for limb, and a sample is checked
+    against residues on the host."""
+    bits, la, lb, offa, offb, host, pool, tot_in = batch_inputs(hip, L, pairs_count)
     outs = {m: hip.Buffer(4 * tot_in) for m in (0, 1, 2)}
     stream = hip.Stream()
     res = {}
@@ -320,6 +379,10 @@ def main() -> int:
     ap.add_argument("--cpu-sweep", action="store_true",
                     help="run only the CPU restatement: 2^28 x 2^28 at 1, 8 and all physical cores, "
                          "and a 1/16 sample of the batched workload, sequential and across threads")
+    ap.add_argument("--batch-only", choices=("plain", "table", "learned"), default=None,
+                    help="run only the batched workload in one plan mode, --steps batches")
+    ap.add_argument("--valu-rates", action="store_true",
+                    help="run only the issue-rate microbenchmark of the integer VALU forms")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -334,6 +397,10 @@ def main() -> int:
 
     if args.copy_calibration:
         return copy_calibration(hip)
+    if args.valu_rates:
+        return valu_rates()
+    if args.batch_only:
+        return batch_only(hip, engine, L, args.batch_pairs, args.batch_only, args.steps)
     if args.sweep:
         return size_sweep(hip, engine, L)
 

@@ -22,6 +22,7 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++20", "-fPIC",
 SOURCES = ["ntt_kernels.hip", "add_kernels.hip", "planner_kernels.hip", "engine.cpp"]
 HOST_SOURCES = ["host/mul.cpp", "host/mul_backend_hip.cpp", "host/bigint.cpp", "host/device_int.cpp"]
 HOST_LIB = os.path.join(HERE, "libbigint.so")
+RATES_LIB = os.path.join(HERE, "libbigint_rates.so")
 CXX = os.environ.get("BIGINT_CXX", "/opt/rocm/llvm/bin/clang++")
 HOST_FLAGS = ["-O2", "-std=gnu++20", "-fPIC", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"),
               "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__"]
@@ -50,6 +51,10 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # issue-rate microbenchmark of the integer VALU forms (measurement support, bench.py --valu-rates)
+    tmp = RATES_LIB + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + ["-shared", os.path.join(CSRC, "valu_rates.hip"), "-o", tmp])
+    os.replace(tmp, RATES_LIB)
     # host C++ library (BigInt/BigFloat API) on top of the C ABI
     hobjs = []
     for src in HOST_SOURCES:
