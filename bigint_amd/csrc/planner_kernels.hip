@@ -38,42 +38,39 @@ __device__ __forceinline__ MlpView view(const float* p, int K) {
     return v;
 }
 
-// One thread per sample: full forward pass; optionally keeps the activations.
-__global__ void k_mlp_forward(const float* __restrict__ x, int B, const float* __restrict__ p,
-                              int K, float* __restrict__ y, float* __restrict__ h1o,
-                              float* __restrict__ h2o) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+// One block of kHid threads per sample: thread j computes hidden unit j of both layers and, for
+// j < K, output j, each as a sequential fmaf chain over its inputs in index order; the layer
+// activations are shared through LDS.  Optionally keeps the activations.
+__global__ __launch_bounds__(kHid) void k_mlp_forward(const float* __restrict__ x, int B,
+                                                      const float* __restrict__ p, int K,
+                                                      float* __restrict__ y, float* __restrict__ h1o,
+                                                      float* __restrict__ h2o) {
+    __shared__ float sh1[kHid], sh2[kHid];
+    const int s = blockIdx.x;
+    const int j = threadIdx.x;
     if (s >= B) return;
     const MlpView v = view(p, K);
-    float in[kIn];
+    float a = v.b1[j];
 #pragma unroll
-    for (int i = 0; i < kIn; ++i) in[i] = x[s * kIn + i];
-    float h1[kHid];
-#pragma unroll
-    for (int j = 0; j < kHid; ++j) {
-        float a = v.b1[j];
-#pragma unroll
-        for (int i = 0; i < kIn; ++i) a = fmaf(in[i], v.W1[i * kHid + j], a);
-        h1[j] = a > 0.f ? a : 0.f;
-    }
-    float h2[kHid];
-    for (int j = 0; j < kHid; ++j) {
-        float a = v.b2[j];
+    for (int i = 0; i < kIn; ++i) a = fmaf(x[s * kIn + i], v.W1[i * kHid + j], a);
+    const float h1 = a > 0.f ? a : 0.f;
+    sh1[j] = h1;
+    __syncthreads();
+    float c = v.b2[j];
 #pragma unroll 16
-        for (int i = 0; i < kHid; ++i) a = fmaf(h1[i], v.W2[i * kHid + j], a);
-        h2[j] = a > 0.f ? a : 0.f;
-    }
-    for (int k = 0; k < K; ++k) {
-        float a = v.b3[k];
+    for (int i = 0; i < kHid; ++i) c = fmaf(sh1[i], v.W2[i * kHid + j], c);
+    const float h2 = c > 0.f ? c : 0.f;
+    sh2[j] = h2;
+    __syncthreads();
+    if (j < K) {
+        float o = v.b3[j];
 #pragma unroll 16
-        for (int i = 0; i < kHid; ++i) a = fmaf(h2[i], v.W3[i * K + k], a);
-        y[s * K + k] = a;
+        for (int i = 0; i < kHid; ++i) o = fmaf(sh2[i], v.W3[i * K + j], o);
+        y[s * K + j] = o;
     }
     if (h1o) {
-        for (int j = 0; j < kHid; ++j) {
-            h1o[(size_t)s * kHid + j] = h1[j];
-            h2o[(size_t)s * kHid + j] = h2[j];
-        }
+        h1o[(size_t)s * kHid + j] = h1;
+        h2o[(size_t)s * kHid + j] = h2;
     }
 }
 
@@ -236,7 +233,8 @@ hipError_t mlp_forward(const float* x, int B, const float* p, int K, float* y, f
                        float* h2, hipStream_t st) {
     if (B <= 0) return hipSuccess;
     (void)hipGetLastError();  // report this launch's error, not an earlier ignored one
-    hipLaunchKernelGGL(k_mlp_forward, dim3((B + 127) / 128), dim3(128), 0, st, x, B, p, K, y, h1, h2);
+    static_assert(kMaxPlans <= kHid, "one output per hidden-unit thread");
+    hipLaunchKernelGGL(k_mlp_forward, dim3(B), dim3(kHid), 0, st, x, B, p, K, y, h1, h2);
     return hipGetLastError();
 }
 
