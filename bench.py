@@ -320,8 +320,60 @@ def size_sweep(hip, engine, L) -> int:
         print(json.dumps(row), file=sys.stderr, flush=True)
         for buf in (da, db, dout):
             buf.free()
-    print(json.dumps({"size_sweep": rows}), flush=True)
+    copies = host_copy_breakdown(hip, engine, L)
+    print(json.dumps({"size_sweep": rows, "host_copies_2e28": copies}), flush=True)
     return 0 if all(r["exact"] for r in rows) else 1
+
+
+def host_copy_breakdown(hip, engine, L) -> dict:
+    """Where the end-to-end time of the 2^28-bit host-pointer product goes: both operands up
+    (bi_hip_copy_to_device, one call), the product down (bi_hip_copy_to_host) into memory already
+    faulted in and into fresh pages, and bi_hip_mul with a faulted-in output; medians of 5."""
+    import ctypes
+    Lb = engine.lib()
+    a = L.random_operand(1 << 28, L.SEED)
+    b = L.random_operand(1 << 28, L.SEED + 1)
+    n = a.size + b.size
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    dbuf = hip.Buffer(4 * n)
+    out = np.ones(n, np.uint32)
+
+    def med(f, reps=5):
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            f()
+            t.append((time.perf_counter() - t0) * 1e3)
+        return round(float(np.median(t)), 3)
+
+    def check(rc):
+        if rc:
+            raise engine.EngineError(rc, "host copy breakdown")
+
+    srcs = (ctypes.c_void_p * 2)(a.ctypes.data, b.ctypes.data)
+    sz = (ctypes.c_size_t * 2)(a.size * 4, b.size * 4)
+    up = med(lambda: check(Lb.bi_hip_copy_to_device(dbuf.ptr, srcs, sz, 2)))
+    dst = (ctypes.c_void_p * 1)(out.ctypes.data)
+    dsz = (ctypes.c_size_t * 1)(n * 4)
+    down = med(lambda: check(Lb.bi_hip_copy_to_host(dst, dsz, 1, dbuf.ptr)))
+
+    def fresh_down():
+        f = np.zeros(n, np.uint32)
+        d = (ctypes.c_void_p * 1)(f.ctypes.data)
+        t0 = time.perf_counter()
+        check(Lb.bi_hip_copy_to_host(d, dsz, 1, dbuf.ptr))
+        return (time.perf_counter() - t0) * 1e3
+    down_fresh = round(float(np.median([fresh_down() for _ in range(5)])), 3)
+    k = ctypes.c_size_t()
+    for _ in range(2):
+        check(Lb.bi_hip_mul(a.ctypes.data_as(u32p), a.size, b.ctypes.data_as(u32p), b.size, 0,
+                            out.ctypes.data_as(u32p), out.size, ctypes.byref(k)))
+    mul_ms = med(lambda: check(Lb.bi_hip_mul(a.ctypes.data_as(u32p), a.size, b.ctypes.data_as(u32p), b.size,
+                                             0, out.ctypes.data_as(u32p), out.size, ctypes.byref(k))))
+    dbuf.free()
+    return {"bytes_each_way": 4 * n, "to_device_ms": up, "to_host_ms": down,
+            "to_host_fresh_pages_ms": down_fresh, "bi_hip_mul_faulted_in_output_ms": mul_ms,
+            "to_device_GB_s": round(4 * n / up / 1e6, 1), "to_host_GB_s": round(4 * n / down / 1e6, 1)}
 
 
 def cpu_sweep() -> int:
