@@ -38,6 +38,12 @@ namespace {
 constexpr uint32_t kPh[3] = {kP[0], kP[1], kP[2]};  // the device primes and generators
 constexpr uint32_t kGh[3] = {kG[0], kG[1], kG[2]};
 constexpr int kTileH = 4096;
+
+// Output tiles of the Garner + carry kernel for a product of `limbs` limbs (one carry-scan status
+// word each).
+constexpr uint32_t carry_tiles(size_t limbs) {
+    return (uint32_t)((limbs + kCarryTileLimbs - 1) / kCarryTileLimbs);
+}
 constexpr uint32_t kMaxLog = 26;
 
 // Device allocations made while serving calls (workspaces, staging, descriptors, caller buffers).
@@ -645,7 +651,7 @@ public:
     // Device words one unchunked product needs: the residues of both operands (one for a square)
     // and the carry-scan status words.
     static size_t single_words(uint32_t lg, uint32_t la, uint32_t lb, bool square) {
-        return ((size_t)3 << lg) * (square ? 1 : 2) + ((size_t)la + lb + 4095u) / 4096u + 64;
+        return ((size_t)3 << lg) * (square ? 1 : 2) + carry_tiles((size_t)la + lb) + 64;
     }
 
     // Queue one unchunked product on plan pl with its residues and status words in w (at least
@@ -654,7 +660,7 @@ public:
                       bool square, uint32_t* d_out, uint32_t* w, hipStream_t st) {
         const uint32_t n = 1u << pl->log_n;
         const uint32_t out_len = la + lb;
-        const uint32_t ntiles = (out_len + 4095u) / 4096u;
+        const uint32_t ntiles = carry_tiles(out_len);
         uint32_t* res_a = w;
         uint32_t* res_b = square ? nullptr : w + (size_t)3 * n;
         uint32_t* status = w + (size_t)3 * n * (square ? 1 : 2);
@@ -666,7 +672,7 @@ public:
         gj.out_len = out_len;
         for (int I = 0; I < 3; ++I) gj.kscale[I] = pl->kscale[I];
         gj.status = status;
-        gj.counter = status + ntiles;
+        gj.fix = status + ntiles;
         if ((uint64_t)words_of(la) + words_of(lb) - 1 > n) {
             gj.wrap_a = d_a;
             gj.wrap_b = square ? d_a : d_b;
@@ -763,7 +769,7 @@ public:
     static constexpr uint32_t kSplitLog = 25;
     static constexpr uint32_t kSplitChunk = 1u << kSplitLog;
     // The product's limb count must stay below 2^32 - 2^13, so tile counts such as
-    // (la + lb + 4095) / 4096 and word offsets stay exact in the kernels' 32-bit arithmetic.
+    // carry_tiles(la + lb) and word offsets stay exact in the kernels' 32-bit arithmetic.
     static constexpr uint64_t kMaxProductLimbs = 0xFFFFFFFFull - 8192u;
     static bool product_supported(size_t la, size_t lb) {
         if (la == 0 || lb == 0 || la > kMaxProductLimbs || lb > kMaxProductLimbs) return false;
@@ -1061,7 +1067,7 @@ public:
                 const size_t nj = kind == kBForward ? 1 : it.nchunks;
                 njobs += nj;
                 if (kind == kBForward) continue;
-                words += 3 * n * (kind == kPlain ? 2 : nj) + (it.la + it.lb + 4095u) / 4096u + 1;
+                words += 3 * n * (kind == kPlain ? 2 : nj) + carry_tiles((size_t)it.la + it.lb) + 1;
             }
             grp_words = std::max(grp_words, words);
             desc_need += njobs * sizeof(PassJob) + idx.size() * sizeof(GarnerJob) + 512;
@@ -1152,13 +1158,13 @@ public:
                     }
                 }
                 const uint32_t out_len = it.la + it.lb;
-                const uint32_t nt = (out_len + 4095u) / 4096u;
+                const uint32_t nt = carry_tiles(out_len);
                 g.out = p.out;
                 g.n = (uint32_t)n;
                 g.out_len = out_len;
                 for (int I = 0; I < 3; ++I) g.kscale[I] = pl->kscale[I];
                 g.status = st_cur;
-                g.counter = st_cur + nt;
+                g.fix = st_cur + nt;
                 st_cur += nt + 1;
                 gj[j] = g;
                 L.max_out = std::max(L.max_out, out_len);
@@ -1268,7 +1274,7 @@ public:
         Plan* pl = plan(lg, 12);
         if (!pl) return BI_HIP_ENOMEM;
         const uint32_t out_len = la + lb;
-        const uint32_t nt = (out_len + 4095u) / 4096u;
+        const uint32_t nt = carry_tiles(out_len);
         const size_t nspec = ca + (sq ? 0 : cb);
         uint32_t* w = workspace((nspec + nd) * 3 * n + nt + 64);
         if (!w) return BI_HIP_ENOMEM;
@@ -1315,7 +1321,7 @@ public:
         g.out_len = out_len;
         for (int I = 0; I < 3; ++I) g.kscale[I] = pl->kscale[I];
         g.status = status;
-        g.counter = status + nt;
+        g.fix = status + nt;
         g.nchunks = nd;
         g.chunk_len = L;
         g.log_chunk = kSplitLog;

@@ -653,6 +653,7 @@ __device__ __forceinline__ uint32_t map_compose(uint32_t second, uint32_t first)
     return r;
 }
 constexpr uint32_t kMapIdentity = 0b11100100u;  // c -> c
+constexpr uint32_t kFixBit = 1u << 8;           // status word: tile written with carry-in 0
 
 struct Pieces {
     uint32_t w0, w1, w2;  // x = w0 + w1 2^32 + w2 2^64
@@ -745,16 +746,6 @@ __device__ __forceinline__ Pieces coeff_pieces(uint32_t r0, uint32_t r1, uint32_
     return Pieces{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)(x >> 64)};
 }
 
-// Status words carry their whole payload (flag << 8 | map or carry), so they are published and
-// polled with relaxed agent-scope atomics: no acquire/release fence, hence no L2 write-back or
-// invalidation per poll (the L2 is private to each XCD).
-__device__ __forceinline__ uint32_t status_load(const uint32_t* p) {
-    return __hip_atomic_load(gmem(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void status_store(uint32_t* p, uint32_t v) {
-    __hip_atomic_store(gmem(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-constexpr uint32_t kAgg = 1u, kInc = 2u;
 
 // CHUNKED: the jobs are chunked products (nchunks > 1); otherwise every job is unchunked.  Two
 // instantiations keep the chunk loop's register demand out of the common kernel.
@@ -765,10 +756,8 @@ constexpr uint32_t kAgg = 1u, kInc = 2u;
 // through LDS); thread 0 takes them from coefficients base - 1 and base - 2, which threads 0..2
 // reconstruct together with base - 3 (for the tile's local carry rule).
 template <bool CHUNKED>
-__global__ __launch_bounds__(kGThreads) void k_garner_carry(GarnerArgs a) {
-    const GarnerJob jb = a.jobs ? load_desc(a.jobs + blockIdx.y) : a.job;
+__device__ __forceinline__ void garner_tile(const GarnerArgs& a, const GarnerJob& jb, uint32_t tile) {
     constexpr uint32_t kWaves = kGThreads / 64;
-    __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave[kWaves];     // slow path: each wave's inclusive carry map
     __shared__ uint32_t s_wc[kWaves];       // lane 63 of each wave: carry-out of its run for carry-in 0
     __shared__ uint32_t s_wconst[kWaves];   // each wave: all its runs pass on a fixed carry
@@ -776,11 +765,6 @@ __global__ __launch_bounds__(kGThreads) void k_garner_carry(GarnerArgs a) {
     __shared__ uint32_t s_edge[kWaves][3];  // lane 63 of each wave: w1 and w2 of its last word, w2 of the one before
     __shared__ uint32_t s_halo[3][3];       // pieces of words base - 3, base - 2, base - 1
     const uint32_t tid = threadIdx.x;
-    if (tid == 0) s_tile = __hip_atomic_fetch_add(gmem(jb.counter), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint32_t ntiles = (jb.out_len + kCarryTile - 1u) / kCarryTile;
-    if (tile >= ntiles) return;
     const uint32_t base = tile * kWordTile;  // first word (coefficient) of the tile
     const uint32_t n = jb.n;
     // An interior tile reads only aligned position pairs inside the plane (its window
@@ -1018,51 +1002,15 @@ __global__ __launch_bounds__(kGThreads) void k_garner_carry(GarnerArgs a) {
         for (uint32_t w = 0; w < kWaves; ++w) agg = map_compose(s_wave[w], agg);
     }
 
-    // 4. carry into the tile: resolved locally, or by a decoupled look-back of the first wave
-    //    (64 predecessor tiles read at once).  Every tile publishes its status either way.
-    if (wave == 0) {
-        uint32_t carry_in = 0;
-        const uint32_t loc = __shfl(local, 0, 64);
-        if (tile == 0) {
-            if (lane == 0) status_store(&jb.status[0], (kInc << 8) | map_apply(agg, 0u));
-        } else if (loc != 0xFFu) {
-            carry_in = loc;
-            if (lane == 0) status_store(&jb.status[tile], (kInc << 8) | map_apply(agg, loc));
-        } else {
-            // A tile whose carry-out does not depend on its carry-in (every tile but those with a
-            // run of all-ones words at the start) publishes that carry as inclusive at once, so
-            // successors end their look-back here without waiting for this one's.
-            const bool fixed = agg == (agg & 3u) * 0x55u;
-            if (lane == 0)
-                status_store(&jb.status[tile], fixed ? ((kInc << 8) | (agg & 3u)) : ((kAgg << 8) | agg));
-            uint32_t acc = kMapIdentity;  // composition of the nearer tiles already visited
-            int32_t top = (int32_t)tile - 1;
-            while (true) {
-                const int32_t j = top - (int32_t)lane;
-                uint32_t v = kInc << 8;  // before the first tile: inclusive carry 0
-                if (j >= 0) {
-                    do {
-                        v = status_load(&jb.status[j]);
-                    } while ((v >> 8) == 0u);
-                }
-                const uint64_t incs = __ballot((v >> 8) == kInc);
-                const uint32_t first = incs ? (uint32_t)__builtin_ctzll(incs) : 64u;
-                uint32_t mm = kMapIdentity;  // A_top o A_top-1 o ... over lanes [0, first)
-                for (uint32_t l = 0; l < 64; ++l) {
-                    const uint32_t x = __shfl(v, (int)l, 64);
-                    if (l < first) mm = map_compose(mm, x & 0xFFu);
-                }
-                if (first < 64) {
-                    const uint32_t cv = __shfl(v, (int)first, 64) & 0xFFu;
-                    carry_in = map_apply(acc, map_apply(mm, cv));
-                    break;
-                }
-                acc = map_compose(acc, mm);
-                top -= 64;
-            }
-            if (lane == 0 && !fixed) status_store(&jb.status[tile], (kInc << 8) | map_apply(agg, carry_in));
-        }
-        if (lane == 0) s_carry_in = carry_in;
+    // 4. The carry into the tile: 0 for tile 0, else resolved locally (above).  A tile that cannot
+    //    resolve it is written as if it were 0 and flagged; k_carry_fixup then scans the tiles'
+    //    carry maps and adds the true carry-in to such tiles.  No block waits for another, so the
+    //    tiles need no ordering and no counter.  Every tile publishes its carry map.
+    if (tid == 0) {
+        const bool unresolved = tile != 0 && local == 0xFFu;
+        gmem(jb.status)[tile] = (unresolved ? kFixBit : 0u) | agg;
+        if (unresolved) gmem(jb.fix)[0] = 1u;
+        s_carry_in = tile == 0 || unresolved ? 0u : local;
     }
     // the carry-out of run t - 1 (fast path), read before the barrier so every lane takes part
     uint32_t prev = __shfl_up(co0, 1, 64);
@@ -1114,6 +1062,96 @@ __global__ __launch_bounds__(kGThreads) void k_garner_carry(GarnerArgs a) {
             const uint32_t idx = m2 * kGThreads + tid;
             if (idx < lim) gmem(out)[idx] = s_res[idx];
         }
+    }
+}
+
+// One tile per block (grid.x = tiles, grid.y = job).
+template <bool CHUNKED>
+__global__ __launch_bounds__(kGThreads) void k_garner_carry(GarnerArgs a) {
+    const GarnerJob jb = a.jobs ? load_desc(a.jobs + blockIdx.y) : a.job;
+    const uint32_t ntiles = (jb.out_len + kCarryTile - 1u) / kCarryTile;
+    if (blockIdx.x < ntiles) garner_tile<CHUNKED>(a, jb, blockIdx.x);
+}
+
+// Carry fix-up of one job (grid.y = job, one block of kFixThreads threads): nothing to do unless a
+// tile of the job was written with an assumed carry-in of 0.  Then the block scans the tiles'
+// carry maps in chunks of kFixThreads tiles (carry_in(0) = 0, carry_in(t + 1) = map_t(carry_in(t)))
+// and adds each flagged tile's true carry-in c at its first limb: the tile's limbs hold its value
+// with carry-in 0, and its value with carry-in c is that plus c modulo the tile (what passes out
+// of the tile is the next tile's carry-in, which the maps account for).  The addition changes
+// limb 0 and, when it overflows, clears the run of all-ones limbs after it and increments the
+// first limb that is not all ones.
+constexpr uint32_t kFixThreads = 1024;
+
+__global__ __launch_bounds__(kFixThreads) void k_carry_fixup(GarnerArgs a) {
+    const GarnerJob jb = a.jobs ? load_desc(a.jobs + blockIdx.y) : a.job;
+    if (gmem(jb.fix)[0] == 0u) return;
+    constexpr uint32_t kWaves = kFixThreads / 64;
+    __shared__ uint32_t s_wmap[kWaves];
+    __shared__ uint32_t s_carry;   // carry into the current chunk's first tile
+    __shared__ uint32_t s_first;   // the ripple's first limb that is not all ones
+    __shared__ uint32_t s_nfix;
+    __shared__ uint32_t s_fix[kFixThreads][2];  // tiles of the chunk to fix: tile, carry-in
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t ntiles = (jb.out_len + kCarryTile - 1u) / kCarryTile;
+    if (tid == 0) s_carry = 0u;
+    for (uint32_t t0 = 0; t0 < ntiles; t0 += kFixThreads) {
+        const uint32_t t = t0 + tid;
+        const uint32_t st = t < ntiles ? gmem(jb.status)[t] : kMapIdentity;
+        const uint32_t m = st & 0xFFu;
+        // exclusive scan of the maps across the block: inclusive within the wave, then waves
+        uint32_t inc = m;
+#pragma unroll
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(inc, d, 64);
+            if (lane >= d) inc = map_compose(inc, o);
+        }
+        if (lane == 63u) s_wmap[wave] = inc;
+        if (tid == 0) s_nfix = 0u;
+        __syncthreads();
+        uint32_t pre = kMapIdentity;
+        for (uint32_t w = 0; w < wave; ++w) pre = map_compose(s_wmap[w], pre);
+        uint32_t excl = __shfl_up(inc, 1, 64);
+        if (lane == 0) excl = kMapIdentity;
+        excl = map_compose(excl, pre);
+        const uint32_t cin = map_apply(excl, s_carry);  // carry into tile t
+        if (t < ntiles && (st & kFixBit) && cin != 0u) {
+            const uint32_t k = atomicAdd(&s_nfix, 1u);
+            s_fix[k][0] = t;
+            s_fix[k][1] = cin;
+        }
+        uint32_t chunk = kMapIdentity;
+        for (uint32_t w = 0; w < kWaves; ++w) chunk = map_compose(s_wmap[w], chunk);
+        __syncthreads();  // s_carry, s_wmap and the fix list are complete
+        const uint32_t carry_next = map_apply(chunk, s_carry);
+        const uint32_t nfix = s_nfix;
+        for (uint32_t f = 0; f < nfix; ++f) {
+            const uint32_t tile = s_fix[f][0], c = s_fix[f][1];
+            uint32_t* out = jb.out + (size_t)tile * kCarryTile;
+            const uint32_t lim = min(kCarryTile, jb.out_len - tile * kCarryTile);
+            if (tid == 0) s_first = lim;
+            __syncthreads();
+            const uint32_t v0 = gmem(out)[0] + c;
+            if (v0 > kDigitMask) {
+                // ripple: find the first limb after limb 0 that is not all ones
+                for (uint32_t i0 = 1; i0 < lim; i0 += kFixThreads) {
+                    const uint32_t i = i0 + tid;
+                    if (i < lim && gmem(out)[i] != kDigitMask) atomicMin(&s_first, i);
+                    __syncthreads();
+                    const uint32_t found = s_first;
+                    __syncthreads();  // every thread has read s_first before the next round lowers it
+                    if (found < lim) break;
+                }
+                const uint32_t first = s_first;
+                for (uint32_t i = 1 + tid; i < first; i += kFixThreads) gmem(out)[i] = 0u;
+                if (tid == 0 && first < lim) gmem(out)[first] += 1u;
+            }
+            __syncthreads();  // every thread has read limb 0 (and s_first) before they change
+            if (tid == 0) gmem(out)[0] = v0 & kDigitMask;
+            __syncthreads();
+        }
+        if (tid == 0) s_carry = carry_next;
+        __syncthreads();
     }
 }
 
@@ -1234,6 +1272,7 @@ hipError_t launch_diag_accumulate(const uint32_t* spec_a, const uint32_t* spec_b
 hipError_t launch_garner_carry(const GarnerArgs& a, uint32_t max_out_len, uint32_t njobs,
                                bool chunked, hipStream_t st) {
     const uint32_t ntiles = (max_out_len + kCarryTile - 1u) / kCarryTile;
+
     (void)hipGetLastError();  // report this launch's error, not an earlier ignored one
     for (uint32_t j0 = 0; j0 < njobs; j0 += kMaxGridY) {
         GarnerArgs b = a;
@@ -1241,6 +1280,7 @@ hipError_t launch_garner_carry(const GarnerArgs& a, uint32_t max_out_len, uint32
         dim3 grid(ntiles, std::min(kMaxGridY, njobs - j0), 1);
         hipLaunchKernelGGL(chunked ? k_garner_carry<true> : k_garner_carry<false>, grid, dim3(kGThreads),
                            0, st, b);
+        hipLaunchKernelGGL(k_carry_fixup, dim3(1, grid.y, 1), dim3(kFixThreads), 0, st, b);
     }
     return hipGetLastError();
 }

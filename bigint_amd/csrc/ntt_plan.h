@@ -45,7 +45,7 @@ struct PassArgs {
     const uint32_t* beta;
     uint32_t beta_stride;      // blocks per prime
     // The first kernel of a launch sequence zeroes these words (the carry-scan status words and
-    // tile counters of the sequence's Garner launch) before anything else; null elsewhere.
+    // fix-up flags of the sequence's Garner launch) before anything else; null elsewhere.
     uint32_t* clear;
     uint32_t clear_words;
 };
@@ -64,8 +64,9 @@ struct GarnerJob {
     uint32_t* out;       // out_len limbs
     uint32_t n, out_len;
     uint32_t kscale[3];  // n^-1 * R^2 mod P_i, so mont_mul(y, kscale) = y * n^-1 * R
-    uint32_t* status;    // look-back status words, one per output tile, zeroed before launch
-    uint32_t* counter;   // dynamic tile counter, zeroed before launch
+    uint32_t* status;    // one word per output tile: its carry map, and kFixBit when the tile
+                         // was written with an assumed carry-in of 0
+    uint32_t* fix;       // set to 1 by any such tile (zeroed before launch): the fix-up runs
     // Chunked product: the long operand was cut into nchunks pieces of chunk_len words, each
     // multiplied with a transform of length n >= chunk_len + (words of B) - 1; chunk c's
     // residues start at y + c * chunk_stride and its product lands at word offset c * chunk_len.

@@ -315,8 +315,8 @@ def test_wrapped_top_coefficient(wa, wb):
 def test_carry_runs_across_tiles():
     """a * (2^32 + 1) is a + (a << 32): with a's words alternating so that a_j + a_(j-1) =
     2^32 - 1, one carry from word 2 runs through three whole carry tiles (3584 words each).
-    Those tiles' carry maps depend on their carry-in, so the tile look-back has to compose
-    them back to the tile where the carry was born."""
+    Those tiles cannot resolve their carry-in locally: they are written with carry-in 0 and the
+    carry fix-up composes the tile maps back to the tile where the carry was born."""
     W = 3 * 3584 + 100
     words = [(1 << 32) - 1, 5] + [((1 << 32) - 6) if j % 2 == 0 else 5 for j in range(2, W)]
     a = sum(w << (32 * j) for j, w in enumerate(words))
@@ -326,6 +326,20 @@ def test_carry_runs_across_tiles():
     # and the square of such a run, for carries born inside the product
     p2 = engine.mul(L.int_to_limbs(a), None)
     assert L.limbs_to_int(p2) == a * a
+
+
+def test_carry_runs_across_a_thousand_tiles():
+    """The same construction over 1,100 carry tiles (3.94 M words, 126 Mbit): every tile after
+    the first is written with an assumed carry-in and fixed, so the fix-up scans its tile maps in
+    two chunks of 1024 and ripples a carry through every tile."""
+    W = 1100 * 3584 + 100
+    words = np.full(W, 5, dtype=np.uint64)
+    words[2::2] = (1 << 32) - 6
+    words[0] = (1 << 32) - 1
+    a = int.from_bytes(words.astype("<u4").tobytes(), "little")
+    b = (1 << 32) + 1
+    p = engine.mul(L.int_to_limbs(a), L.int_to_limbs(b))
+    assert L.limbs_to_int(p) == a * b
 
 
 def check_residues3(a, b, p):

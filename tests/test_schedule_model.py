@@ -238,12 +238,20 @@ def map_compose(second, first):
     return sum(map_apply(second, map_apply(first, c)) << (2 * c) for c in range(4))
 
 
+FIXUPS = []  # (tile, carry-in) of every tile the modelled fix-up changed
+
+
 def garner_carry(coeff, out_len):
-    """Kernel k_garner_carry's tile/thread arithmetic on exact coefficients coeff(j) (zero
-    outside the product): 32-bit pieces, word sums with the two-word halo, per-thread carry maps
-    over 14 words, composition across threads and tiles, then 14 words -> 16 limbs."""
+    """Kernels k_garner_carry and k_carry_fixup on exact coefficients coeff(j) (zero outside the
+    product): 32-bit pieces, word sums with the two-word halo, per-thread carry maps over 7
+    words, composition across threads, 7 words -> 8 limbs.  A tile's carry-in is 0 for tile 0
+    and the local rule's value where it applies; otherwise the tile is written with carry-in 0
+    and flagged, and the fix-up scans the tile maps and ripples the true carry-in into it.  The
+    result is checked against the same arithmetic with every tile's true carry-in."""
     ntiles = (out_len + KLT - 1) // KLT
-    out = [None] * out_len
+    out = [None] * out_len       # every tile with its true carry-in
+    got = [None] * out_len       # the kernels' way
+    tile_maps, flagged = [], []
     carry_in = 0
     for tile in range(ntiles):
         base = tile * KWT
@@ -269,10 +277,16 @@ def garner_carry(coeff, out_len):
             own[0] += piece(nb, 1) + piece(nb - 1, 2)
             own[1] += piece(nb, 2)
             assert own == sums[GW * t:GW * t + GW], (tile, t)
+        written = 0  # the carry-in the Garner kernel writes the tile with
+        flag = False
         if tile:  # kernel's local resolution of the carry into the tile, checked against the scan
             sl = piece(base - 1, 0) + piece(base - 2, 1) + piece(base - 3, 2)
             if sl & 0xFFFFFFFF <= 0xFFFFFFFD:
                 assert sl >> 32 == carry_in
+                written = sl >> 32
+            else:
+                flag = True
+        flagged.append(flag)
         maps = []
         for t in range(GT):
             c = [0, 1, 2, 3]
@@ -285,30 +299,49 @@ def garner_carry(coeff, out_len):
         # and 2 agree), so run t receives run t - 1's carry-out for carry-in 0
         co0 = [map_apply(m, 0) for m in maps]
         fixed_runs = all(map_apply(m, 0) == map_apply(m, 2) for m in maps)
-        excl = 0b11100100
-        for t in range(GT):
-            carry = map_apply(excl, carry_in)
-            assert carry <= 2
-            if fixed_runs:
-                assert carry == (carry_in if t == 0 else co0[t - 1])
-            words = []
-            for q in range(GW):
-                v = sums[GW * t + q] + carry
-                words.append(v & 0xFFFFFFFF)
-                carry = v >> 32
-            for q in range(GL):
-                bit = 28 * q
-                wi, bo = bit >> 5, bit & 31
-                v = words[wi] >> bo
-                if bo > 4:
-                    v |= words[wi + 1] << (32 - bo)
-                i = tile * KLT + GL * t + q
-                if i < out_len:
-                    out[i] = v & MASK
-            excl = map_compose(maps[t], excl)
+        for cin, dst in ((carry_in, out), (written, got)):
+            excl = 0b11100100
+            for t in range(GT):
+                carry = map_apply(excl, cin)
+                assert carry <= 2
+                if fixed_runs:
+                    assert carry == (cin if t == 0 else co0[t - 1])
+                words = []
+                for q in range(GW):
+                    v = sums[GW * t + q] + carry
+                    words.append(v & 0xFFFFFFFF)
+                    carry = v >> 32
+                for q in range(GL):
+                    bit = 28 * q
+                    wi, bo = bit >> 5, bit & 31
+                    v = words[wi] >> bo
+                    if bo > 4:
+                        v |= words[wi + 1] << (32 - bo)
+                    i = tile * KLT + GL * t + q
+                    if i < out_len:
+                        dst[i] = v & MASK
+                excl = map_compose(maps[t], excl)
+        tile_maps.append(excl)
         carry_in = map_apply(excl, carry_in)
     assert carry_in == 0
-    return out
+    # k_carry_fixup: carry_in(0) = 0, carry_in(t + 1) = map_t(carry_in(t)); a flagged tile with a
+    # nonzero carry-in gets it added at limb 0, rippling through the all-ones limbs of the tile
+    c = 0
+    for tile in range(ntiles):
+        if flagged[tile] and c:
+            FIXUPS.append((tile, c))
+            lo, lim = tile * KLT, min(KLT, out_len - tile * KLT)
+            v0 = got[lo] + c
+            if v0 > MASK:
+                first = next((i for i in range(1, lim) if got[lo + i] != MASK), lim)
+                for i in range(1, first):
+                    got[lo + i] = 0
+                if first < lim:
+                    got[lo + first] += 1
+            got[lo] = v0 & MASK
+        c = map_apply(tile_maps[tile], c)
+    assert got == out
+    return got
 
 
 def coeff_pos(j, n):
@@ -642,8 +675,11 @@ def test_garner_carry_run_across_tiles():
     conv = [words[j] + (words[j - 1] if j else 0) for j in range(W)] + [words[-1]]
     a = sum(w << (32 * j) for j, w in enumerate(words))
     la = (32 * W + 27) // 28
+    FIXUPS.clear()
     out = garner_carry(lambda j: conv[j] if 0 <= j < len(conv) else 0, la + 2)
     assert sum(x << (28 * i) for i, x in enumerate(out)) == a * ((1 << 32) + 1)
+    # tiles 1-3 cannot resolve their carry-in locally; the fix-up adds it, rippling through them
+    assert [t for t, _ in FIXUPS] == [1, 2, 3]
 
 
 M32 = (1 << 32) - 1
