@@ -100,7 +100,8 @@ def batch_inputs(hip, L, pairs_count: int):
     """Operands of the batched workload in one device pool: log-uniform bit lengths in
     [2^12, 2^24] (SplitMix64 from the fixed seed), uniform random limbs from a seeded generator
     with the exact top bit set; pair i's operands at offa[i], offb[i] (its product's slot in an
-    output pool of the same layout starts at offa[i])."""
+    output pool of the same layout starts at offa[i]).  Also returns the generator, for samples
+    drawn after the operands."""
     bits = L.batch_bits(pairs_count)
     la = np.array([-(-ba // 28) for ba, _ in bits], np.int64)
     lb = np.array([-(-bb // 28) for _, bb in bits], np.int64)
@@ -114,7 +115,7 @@ def batch_inputs(hip, L, pairs_count: int):
             r = nb - 28 * (n - 1)
             host[off + n - 1] = (host[off + n - 1] & ((1 << r) - 1)) | (1 << (r - 1))
     pool = hip.Buffer.from_array(host)
-    return bits, la, lb, offa, offb, host, pool, tot_in
+    return bits, la, lb, offa, offb, host, pool, tot_in, rng
 
 
 def batch_only(hip, engine, L, pairs_count: int, mode: str, steps: int) -> int:
@@ -125,7 +126,7 @@ def batch_only(hip, engine, L, pairs_count: int, mode: str, steps: int) -> int:
     if os.path.exists(planner.DEFAULT_PATH):
         planner.load()
     m = {"plain": engine.PLAN_FIXED, "table": engine.PLAN_TABLE, "learned": engine.PLAN_LEARNED}[mode]
-    bits, la, lb, offa, offb, host, pool, tot_in = batch_inputs(hip, L, pairs_count)
+    bits, la, lb, offa, offb, host, pool, tot_in, _ = batch_inputs(hip, L, pairs_count)
     out = hip.Buffer(4 * tot_in)
     stream = hip.Stream()
     pairs = [(pool.ptr + 4 * int(offa[i]), int(la[i]), pool.ptr + 4 * int(offb[i]), int(lb[i]),
@@ -173,7 +174,7 @@ def batch_leg(hip, engine, L, pairs_count: int, steps: int) -> dict:
     from a seeded generator with the exact top bit set, resident in HBM.  Timed per plan mode;
     every mode's products must equal the plain mode's limb for limb, and a sample is checked
     against residues on the host."""
-    bits, la, lb, offa, offb, host, pool, tot_in = batch_inputs(hip, L, pairs_count)
+    bits, la, lb, offa, offb, host, pool, tot_in, rng = batch_inputs(hip, L, pairs_count)
     outs = {m: hip.Buffer(4 * tot_in) for m in (0, 1, 2)}
     stream = hip.Stream()
     res = {}
