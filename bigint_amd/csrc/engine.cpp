@@ -866,6 +866,22 @@ public:
         return best;
     }
 
+    // The plan id whose plan the heuristic picks (0 when none of the ids matches it).
+    static int heuristic_id(uint32_t la, uint32_t lb, bool sq) {
+        const PairPlan h = heuristic_plan(la, lb, sq);
+        for (int id = 0; id < kNumPlanIds; ++id) {
+            PairPlan q{};
+            if (plan_for_id(la, lb, sq, id, q) && q.lg == h.lg && q.chunk_len == h.chunk_len && q.mid_lr == h.mid_lr)
+                return id;
+        }
+        return 0;
+    }
+
+    // Learned mode keeps the fallback's plan (table, or heuristic) unless the model predicts a
+    // gain over it of more than this many log2 units (about 5.7 %), about half the model's RMS
+    // training error: where the model is not clearly better, the measured table stands.
+    static constexpr float kModelMargin = 0.08f;
+
     // installed planner state
     std::vector<int8_t> table;  // [19][19][13] plan ids by rounded (log2 bits a, log2 bits b, log2 batch)
     int mlp_k = 0;
@@ -906,11 +922,18 @@ public:
         const int B = (int)shapes.size(), K = mlp_k;
         ids.assign(B, 0);
         if (B == 0 || K == 0 || !d_mlp) return BI_HIP_EINVAL;
-        const size_t words = (size_t)B * 3 + 2 * (size_t)B * K + ((size_t)B * K + 3) / 4 + 4 + (size_t)B;
+        const size_t words = (size_t)B * 3 + 2 * (size_t)B * K + ((size_t)B * K + 3) / 4 + 4 + 2 * (size_t)B;
         float* w = scratch(words + 64);
         if (!w) return BI_HIP_ENOMEM;
         std::vector<float> x((size_t)B * 3), off((size_t)B * K, 0.f);
+        std::vector<int32_t> prior((size_t)B);
         for (int i = 0; i < B; ++i) {
+            // the fallback's plan: the table's, or the heuristic's where the table defers to it
+            int t = table_plan(shapes[i][0], shapes[i][1], shapes[i][2]);
+            PairPlan tmp;
+            if (t < 0 || t >= K || !plan_for_id(shapes[i][0], shapes[i][1], sq[i], t, tmp))
+                t = heuristic_id(shapes[i][0], shapes[i][1], sq[i]);
+            prior[i] = t < K ? t : -1;
             const double f[3] = {std::log2(28.0 * shapes[i][0]), std::log2(28.0 * shapes[i][1]),
                                  std::log2((double)std::max(1u, shapes[i][2]))};
             for (int j = 0; j < 3; ++j) x[(size_t)i * 3 + j] = (float)((f[j] - mlp_norm[j]) / mlp_norm[3 + j]);
@@ -925,12 +948,15 @@ public:
         float* doff = dy + (size_t)B * K;
         uint8_t* dallow = reinterpret_cast<uint8_t*>(doff + (size_t)B * K);
         int32_t* dchoice = reinterpret_cast<int32_t*>(doff + (size_t)B * K + ((size_t)B * K + 3) / 4 + 1);
+        int32_t* dprior = dchoice + B;
         if (hipMemcpyAsync(dx, x.data(), x.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
             hipMemcpyAsync(doff, off.data(), off.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
-            hipMemcpyAsync(dallow, allowed.data(), allowed.size(), hipMemcpyHostToDevice, st) != hipSuccess)
+            hipMemcpyAsync(dallow, allowed.data(), allowed.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(dprior, prior.data(), prior.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
             return BI_HIP_EDEVICE;
         if (mlp_forward(dx, B, d_mlp, K, dy, nullptr, nullptr, st) != hipSuccess) return BI_HIP_EDEVICE;
-        if (mlp_choose(dy, doff, dallow, B, K, dchoice, st) != hipSuccess) return BI_HIP_EDEVICE;
+        if (mlp_choose(dy, doff, dallow, dprior, kModelMargin, B, K, dchoice, st) != hipSuccess)
+            return BI_HIP_EDEVICE;
         if (hipMemcpyAsync(ids.data(), dchoice, (size_t)B * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
             return BI_HIP_EDEVICE;
         return hipStreamSynchronize(st) == hipSuccess ? BI_HIP_OK : BI_HIP_EDEVICE;
@@ -1635,16 +1661,7 @@ int bi_hip_choose_plans(const uint32_t* shapes, const int* is_square, size_t cou
         bi::Engine::PairPlan p{};
         if (out[i] < 0 || !bi::Engine::plan_for_id(sh[i][0], sh[i][1], sqv[i], out[i], p)) {
             // report which id the heuristic corresponds to
-            const bi::Engine::PairPlan h = bi::Engine::heuristic_plan(sh[i][0], sh[i][1], sqv[i]);
-            out[i] = 0;
-            for (int id = 0; id < bi::Engine::kNumPlanIds; ++id) {
-                bi::Engine::PairPlan q{};
-                if (bi::Engine::plan_for_id(sh[i][0], sh[i][1], sqv[i], id, q) && q.lg == h.lg &&
-                    q.chunk_len == h.chunk_len && q.mid_lr == h.mid_lr) {
-                    out[i] = id;
-                    break;
-                }
-            }
+            out[i] = bi::Engine::heuristic_id(sh[i][0], sh[i][1], sqv[i]);
         }
         ids[i] = out[i];
     }
@@ -1695,8 +1712,8 @@ int bi_hip_load_planner(const char* path) {
     }
     std::fclose(f);
     if (!ok) return BI_HIP_EINVAL;
-    for (int8_t t : table)
-        if (t < 0 || t >= K) return BI_HIP_EINVAL;
+    for (int8_t t : table)  // plan ids, or -1: the built-in heuristic
+        if (t < -1 || t >= K) return BI_HIP_EINVAL;
     return bi_hip_set_planner(table.data(), params.data(), K, norm.data());
 }
 

@@ -209,8 +209,8 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 
 // Plan choice: argmin of the predicted cost over the plans allowed for each sample.
 __global__ void k_argmin_plan(const float* __restrict__ y, const float* __restrict__ offset,
-                              const uint8_t* __restrict__ allowed, int B, int K,
-                              int32_t* __restrict__ choice) {
+                              const uint8_t* __restrict__ allowed, const int32_t* __restrict__ prior,
+                              float margin, int B, int K, int32_t* __restrict__ choice) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= B) return;
     int best = -1;
@@ -222,6 +222,12 @@ __global__ void k_argmin_plan(const float* __restrict__ y, const float* __restri
             best = k;
             bv = c;
         }
+    }
+    // the prior plan stands unless the model predicts a gain over it larger than the margin
+    const int pr = prior ? prior[s] : -1;
+    if (best >= 0 && pr >= 0 && pr < K && allowed[s * K + pr]) {
+        const float cp = y[s * K + pr] + (offset ? offset[s * K + pr] : 0.f);
+        if (cp - bv < margin) best = pr;
     }
     choice[s] = best < 0 ? 0 : best;
 }
@@ -265,12 +271,12 @@ hipError_t mlp_adam(float* p, const float* g, float* m, float* v, int NP, float 
     return hipGetLastError();
 }
 
-hipError_t mlp_choose(const float* y, const float* offset, const uint8_t* allowed, int B, int K,
-                      int32_t* choice, hipStream_t st) {
+hipError_t mlp_choose(const float* y, const float* offset, const uint8_t* allowed, const int32_t* prior,
+                      float margin, int B, int K, int32_t* choice, hipStream_t st) {
     if (B <= 0) return hipSuccess;
     (void)hipGetLastError();  // report this launch's error, not an earlier ignored one
-    hipLaunchKernelGGL(k_argmin_plan, dim3((B + 255) / 256), dim3(256), 0, st, y, offset, allowed, B, K,
-                       choice);
+    hipLaunchKernelGGL(k_argmin_plan, dim3((B + 255) / 256), dim3(256), 0, st, y, offset, allowed, prior,
+                       margin, B, K, choice);
     return hipGetLastError();
 }
 

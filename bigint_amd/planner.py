@@ -127,34 +127,37 @@ def group_best(recs):
     return d
 
 
-def build_table(recs) -> np.ndarray:
+def build_table(recs, heuristic: dict | None = None) -> np.ndarray:
     """Bucket by rounded log2 features; each cell takes the plan with the lowest mean log-time
-    ratio to the per-shape best among shapes in the cell; empty cells copy the nearest filled
-    cell."""
+    ratio to the per-shape best among shapes in the cell (plans not allowed for a shape count as
+    2^10 times slower).  With `heuristic` ({(la, lb, k): the plan id the engine's built-in cost
+    heuristic picks}), the heuristic itself is a candidate too, stored as -1: the engine answers
+    -1 with the heuristic's exact-size choice, so a cell whose shapes straddle a power-of-two
+    boundary keeps it.  Empty cells copy the nearest filled cell."""
     shapes = group_best(recs)
     score = {}
-    for (la, lb, k), plans in shapes.items():
+    for key, plans in shapes.items():
+        la, lb, k = key
         best = min(plans.values())
         f = features(28 * la, 28 * lb, k)
         cell = (min(18, max(0, round(f[0]) - 10)), min(18, max(0, round(f[1]) - 10)),
                 min(12, max(0, round(f[2]))))
-        for p, us in plans.items():
-            s = score.setdefault(cell, {})
-            s.setdefault(p, []).append(math.log2(us / best))
-        # plans not allowed for a shape in the cell are penalised
+        s = score.setdefault(cell, {})
         for p in range(K):
-            if p not in plans:
-                score[cell].setdefault(p, []).append(10.0)
-    table = np.full(TABLE_SHAPE, -1, dtype=np.int8)
+            s.setdefault(p, []).append(math.log2(plans[p] / best) if p in plans else 10.0)
+        h = heuristic.get(key) if heuristic is not None else None
+        s.setdefault(-1, []).append(math.log2(plans[h] / best) if h in plans else 10.0)
+    table = np.zeros(TABLE_SHAPE, dtype=np.int8)
+    filled = np.zeros(TABLE_SHAPE, dtype=bool)
     for cell, s in score.items():
-        table[cell] = min(s, key=lambda p: np.mean(s[p]))
-    filled = np.argwhere(table >= 0)
-    if filled.size:
-        for idx in np.argwhere(table < 0):
-            d = np.abs(filled - idx).sum(axis=1)
-            table[tuple(idx)] = table[tuple(filled[int(np.argmin(d))])]
-    else:
-        table[:] = 0
+        cand = [p for p in s if p >= 0 or heuristic is not None]
+        table[cell] = min(cand, key=lambda p: (np.mean(s[p]), p))
+        filled[cell] = True
+    idx_filled = np.argwhere(filled)
+    if idx_filled.size:
+        for idx in np.argwhere(~filled):
+            d = np.abs(idx_filled - idx).sum(axis=1)
+            table[tuple(idx)] = table[tuple(idx_filled[int(np.argmin(d))])]
     return table
 
 
@@ -196,9 +199,10 @@ def init_params(seed: int = 0) -> np.ndarray:
 
 
 def train_gpu(X: np.ndarray, T: np.ndarray, M: np.ndarray, epochs: int = 3000, lr: float = 3e-3,
-              seed: int = 0, log=None):
+              seed: int = 0, log=None, lr_end: float | None = None):
     """Full-batch Adam on the GPU through bi_hip_mlp_grad / bi_hip_mlp_adam.  Inputs are
-    normalised; returns (params, norm6, loss history)."""
+    normalised; the learning rate falls from lr to lr_end on a cosine (constant when lr_end is
+    None).  Returns (params, norm6, loss history)."""
     from . import hip
     mean = X.mean(axis=0)
     std = X.std(axis=0) + 1e-6
@@ -220,7 +224,8 @@ def train_gpu(X: np.ndarray, T: np.ndarray, M: np.ndarray, epochs: int = 3000, l
         rc = L.bi_hip_mlp_grad(x.ptr, t.ptr, mk.ptr, Xn.shape[0], p.ptr, K, g.ptr, st.handle)
         if rc:
             raise engine.EngineError(rc, "bi_hip_mlp_grad")
-        rc = L.bi_hip_mlp_adam(p.ptr, g.ptr, m.ptr, v.ptr, NP, lr, 0.9, 0.999, 1e-8, ep, st.handle)
+        lr_ep = lr if lr_end is None else lr_end + 0.5 * (lr - lr_end) * (1 + math.cos(math.pi * (ep - 1) / epochs))
+        rc = L.bi_hip_mlp_adam(p.ptr, g.ptr, m.ptr, v.ptr, NP, lr_ep, 0.9, 0.999, 1e-8, ep, st.handle)
         if rc:
             raise engine.EngineError(rc, "bi_hip_mlp_adam")
         if ep % 100 == 0 or ep == 1:

@@ -6,8 +6,10 @@
 1. Draws training shapes (operand sizes log-uniform over [2^10, 2^28] bits, batch sizes
    log-uniform in [1, 64]) and times every allowed plan of each (bigint_amd.planner.Timer:
    planned device batches, HIP events on a dedicated stream).
-2. Builds the autotuned table from the training timings.
-3. Trains the MLP on the GPU (bi_hip_mlp_grad / bi_hip_mlp_adam).
+2. Builds the autotuned table from the training timings (per cell the best fixed plan, or the
+   built-in heuristic where it does better).
+3. Trains the MLP on the GPU (bi_hip_mlp_grad / bi_hip_mlp_adam; full-batch Adam, 40,000 epochs
+   with a cosine learning-rate decay by default).
 4. Times every allowed plan on held-out shapes and reports, for the plain plan (0), the table and
    the MLP: mean regret (t_chosen / t_best - 1), geometric-mean speedup over plain, and the
    total-time ratio plain / chosen.  Exactness is independent of the plan (tests/test_planner.py).
@@ -35,17 +37,41 @@ def log(msg: str) -> None:
     print(f"[tune {time.strftime('%H:%M:%S')}] {msg}", flush=True)
 
 
+def mlp_alone(keys, params, norm):
+    """The model's own argmin over the allowed plans (the learned mode without its table prior):
+    forward pass on the GPU (bi_hip_mlp_forward), plus the log2 transform-count cost per plan."""
+    from bigint_amd import hip
+    X = np.array([P.features(28 * la, 28 * lb, k) for la, lb, k in keys], np.float32)
+    Xn = ((X - norm[:3]) / norm[3:]).astype(np.float32)
+    dx, dp = hip.Buffer.from_array(Xn), hip.Buffer.from_array(np.asarray(params, np.float32))
+    dy = hip.Buffer(4 * len(keys) * P.K)
+    st = hip.Stream()
+    rc = engine.lib().bi_hip_mlp_forward(dx.ptr, len(keys), dp.ptr, P.K, dy.ptr, st.handle)
+    if rc:
+        raise engine.EngineError(rc, "bi_hip_mlp_forward")
+    st.synchronize()
+    Y = dy.download(np.float32, len(keys) * P.K).reshape(len(keys), P.K)
+    out = []
+    for i, (la, lb, k) in enumerate(keys):
+        cand = [(Y[i][p] + np.log2(P.plan_cost(la, lb, p)), p) for p in P.allowed(la, lb)]
+        out.append(min(cand)[1])
+    return out
+
+
 def evaluate(recs_h, table, params, norm):
-    """Regret of plain / table / MLP choices on held-out records."""
+    """Regret of plain / heuristic / table / learned choices on held-out records; `learned` is the
+    engine's learned mode (the model with the table's plan as its prior), `mlp_alone` the model's
+    own argmin."""
     shapes = P.group_best(recs_h)
     keys = list(shapes)
     P.install(table, params, norm)
     ch_table = P.choose([(la, lb, k) for la, lb, k in keys], engine.PLAN_TABLE)
     ch_mlp = P.choose([(la, lb, k) for la, lb, k in keys], engine.PLAN_LEARNED)
     ch_heur = P.choose([(la, lb, k) for la, lb, k in keys], engine.PLAN_FIXED)  # built-in heuristic
+    ch_alone = mlp_alone(keys, params, norm)
     out = {}
     for name, ch in [("plain", [0] * len(keys)), ("heuristic", ch_heur), ("table", ch_table),
-                     ("mlp", ch_mlp)]:
+                     ("mlp_alone", ch_alone), ("mlp", ch_mlp)]:
         reg, sp, tot_c, tot_p, hit = [], [], 0.0, 0.0, 0
         for key, c in zip(keys, ch):
             plans = shapes[key]
@@ -70,7 +96,10 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=7000, help="training shapes")
     ap.add_argument("--heldout", type=int, default=3500, help="held-out shapes")
-    ap.add_argument("--epochs", type=int, default=4000)
+    ap.add_argument("--epochs", type=int, default=40000)
+    ap.add_argument("--lr", type=float, default=3e-3, help="Adam learning rate (start)")
+    ap.add_argument("--lr-end", type=float, default=3e-5,
+                    help="final learning rate of the cosine decay (negative: constant rate)")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--out", default=P.DEFAULT_PATH)
     ap.add_argument("--report", default=None)
@@ -104,10 +133,13 @@ def main() -> int:
         train_shapes = shapes(args.train, args.seed)
         recs = P.collect(train_shapes, timer, log)
     log(f"training timings: {len(recs)} runs over {len(train_shapes)} shapes in {time.time() - t0:.0f}s")
-    table = P.build_table(recs)
+    keys = sorted(P.group_best(recs))
+    heur = dict(zip(keys, P.choose(keys, engine.PLAN_FIXED)))  # the built-in heuristic's choices
+    table = P.build_table(recs, heur)
     X, T, M = P.dataset(recs)
     t1 = time.time()
-    params, norm, hist = P.train_gpu(X, T, M, epochs=args.epochs, seed=args.seed, log=log)
+    params, norm, hist = P.train_gpu(X, T, M, epochs=args.epochs, lr=args.lr, seed=args.seed, log=log,
+                                     lr_end=args.lr_end if args.lr_end >= 0 else None)
     log(f"MLP trained on the GPU in {time.time() - t1:.1f}s, final loss {hist[-1][1]:.5f}")
 
     t2 = time.time()
@@ -120,7 +152,7 @@ def main() -> int:
     ev = evaluate(recs_h, table, params, norm)
     log(json.dumps(ev))
     meta = {"train_runs": len(recs), "train_shapes": len(train_shapes), "heldout": ev,
-            "loss_history": hist, "epochs": args.epochs}
+            "loss_history": hist, "epochs": args.epochs, "lr": args.lr, "lr_end": args.lr_end}
     P.save(args.out, table, params, norm, meta)
     if args.report:
         with open(args.report, "w") as f:

@@ -238,8 +238,11 @@ int bi_hip_mul_batch_device_planned(const bi_hip_dpair* pairs, size_t count,
 int bi_hip_choose_plans(const uint32_t* shapes, const int* is_square, size_t count, int mode,
                         int32_t* ids);
 /* Install the autotuned table (int8 [19][19][13] plan ids indexed by round(log2 bits of a) - 10,
- * round(log2 bits of b) - 10, round(log2 batch); NULL keeps the current one) and/or the MLP
- * (fp32 parameters for K outputs, input normalisation {mean[3], std[3]}; NULL keeps it). */
+ * round(log2 bits of b) - 10, round(log2 batch), -1 where the cell defers to the built-in cost
+ * heuristic; NULL keeps the current one) and/or the MLP (fp32 parameters for K outputs, input
+ * normalisation {mean[3], std[3]}; NULL keeps it).  In BI_HIP_PLAN_LEARNED mode the model's
+ * choice replaces the table's (or the heuristic's) plan only where it predicts a gain of more
+ * than 0.08 in log2 time. */
 int bi_hip_set_planner(const int8_t* table, const float* mlp_params, int K, const float* norm6);
 /* Install table and MLP from the binary file bigint_amd/planner.py writes
  * (b"BIPL", int32 K, int8 table[19*19*13], int32 count, float32 params[count], float32 norm[6]);

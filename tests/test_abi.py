@@ -44,3 +44,30 @@ def test_host_only_entry_points():
     assert engine.transform_log2(38347923, 38347924) == -1
     # the 2^28 x 2^28 workload: 9,586,981 limbs = 8,388,609 words each -> n = 2^24
     assert engine.transform_log2(9586981, 9586981) == 24
+
+
+def test_planner_file_validation(tmp_path):
+    """bi_hip_load_planner accepts table cells of -1 (the cell defers to the built-in heuristic)
+    and rejects ids outside [-1, K).  Validation precedes device set-up, so without a GPU a valid
+    file gets past it (returning the device error) while an invalid one is BI_HIP_EINVAL."""
+    if not os.path.exists(engine.LIB_PATH):
+        pytest.skip("libbigint_hip.so not built")
+    import numpy as np
+    from bigint_amd import planner as P
+    L = engine.lib()
+    L.bi_hip_load_planner.argtypes = [__import__("ctypes").c_char_p]
+    params = np.zeros(L.bi_hip_mlp_param_count(P.K), np.float32)
+    norm = np.array([0, 0, 0, 1, 1, 1], np.float32)
+    table = np.zeros(P.TABLE_SHAPE, np.int8)
+    table[3, 4, 5] = -1
+    good = tmp_path / "good.bin"
+    P.write_binary(str(good), table, params, norm)
+    rc_good = L.bi_hip_load_planner(str(good).encode())
+    table[3, 4, 5] = -2
+    bad = tmp_path / "bad.bin"
+    P.write_binary(str(bad), table, params, norm)
+    assert L.bi_hip_load_planner(str(bad).encode()) == engine.EINVAL
+    table[3, 4, 5] = P.K
+    P.write_binary(str(bad), table, params, norm)
+    assert L.bi_hip_load_planner(str(bad).encode()) == engine.EINVAL
+    assert rc_good != engine.EINVAL

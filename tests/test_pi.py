@@ -13,9 +13,12 @@ PI_GPU = os.path.join(ROOT, "bigint_amd", "_build", "pi")
 PI_CPU = os.path.join(ROOT, "oracle", "pi_cpu")
 
 
-def run(binary, digits, path, mode=1):
-    out = subprocess.run([binary, str(digits), str(mode), path], capture_output=True, text=True,
-                         timeout=600, check=True)
+PLANNER = os.path.join(ROOT, "bigint_amd", "plans", "planner.bin")
+
+
+def run(binary, digits, path, mode=1, planner=None):
+    args = [binary, str(digits), str(mode), path] + ([planner] if planner else [])
+    out = subprocess.run(args, capture_output=True, text=True, timeout=600, check=True)
     with open(path) as f:
         return f.read().strip(), out.stdout
 
@@ -42,9 +45,12 @@ def test_pi_cpu_small(tmp_path):
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", [0, 1, 2])
 def test_pi_gpu_matches_cpu_run(tmp_path, mode):
+    """Plain, table and learned plans (the committed planner file, which must load)."""
     digits = 200000
-    gpu, log = run(PI_GPU, digits, str(tmp_path / "pi_gpu.txt"), mode)
+    gpu, log = run(PI_GPU, digits, str(tmp_path / "pi_gpu.txt"), mode, PLANNER if mode else None)
     assert '"gpu_products": 0' not in log
+    if mode:
+        assert '"planner_loaded": true' in log
     cpu, _ = run(PI_CPU, digits, str(tmp_path / "pi_cpu.txt"))
     assert gpu == cpu
     check_digits(gpu[:20002], 20000)

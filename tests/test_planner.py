@@ -37,6 +37,24 @@ def test_reference_gradient_matches_finite_differences():
         assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
 
 
+def test_table_keeps_the_heuristic_where_it_is_better():
+    """A table cell holding shapes on both sides of a power-of-two boundary: plan 0 is best for
+    one, plan 3 for the other, plan 1 is the best single plan for both.  Without the heuristic
+    the cell takes plan 1; offered the heuristic's per-shape choices (each shape's best), the cell
+    stores -1, which the engine answers with the heuristic's exact-size choice."""
+    from bigint_amd import planner as P
+    recs = []
+    for la, times in [(1000, {0: 10.0, 1: 12.0, 3: 20.0}), (1100, {0: 30.0, 1: 12.0, 3: 10.0})]:
+        recs += [{"la": la, "lb": 1000, "k": 1, "plan": p, "us": us} for p, us in times.items()]
+    cell = (round(np.log2(28 * 1000)) - 10, round(np.log2(28 * 1000)) - 10, 0)
+    assert P.build_table(recs)[cell] == 1
+    t = P.build_table(recs, {(1000, 1000, 1): 0, (1100, 1000, 1): 3})
+    assert t[cell] == -1
+    assert (t == -1).all()  # the one filled cell fills every other
+    # a heuristic that picks the wrong plan everywhere loses to plan 1
+    assert P.build_table(recs, {(1000, 1000, 1): 3, (1100, 1000, 1): 0})[cell] == 1
+
+
 @pytest.mark.gpu
 def test_gpu_forward_and_gradient_match_reference():
     from bigint_amd import engine, hip
