@@ -368,6 +368,9 @@ public:
     bool ready = false;
     int device = -1;
     hipStream_t stream = nullptr;
+    // Plan inference (the MLP kernels) runs on a stream of its own: the caller waits for the
+    // choices, and on the caller's stream that wait would also cover every product queued there.
+    hipStream_t plan_stream = nullptr;
     uint32_t* wtab = nullptr;  // [3][4096]
     GarnerConsts gk{};
     std::map<uint32_t, std::unique_ptr<Plan>> plans;
@@ -386,7 +389,8 @@ public:
         if (hipSetDevice(dev) != hipSuccess) return BI_HIP_EDEVICE;
         device = dev;
         g_fail_alloc = std::getenv("BI_HIP_TEST_FAIL_ALLOC") != nullptr;
-        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
+        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&plan_stream, hipStreamNonBlocking) != hipSuccess)
             return BI_HIP_EDEVICE;
         // root table W[m + i] = w_{2m}^{bitrev_{log m}(i)}, Montgomery form, one per prime; the
         // inverse stages use it too (their output order is mirrored, which Garner undoes)
@@ -881,6 +885,8 @@ public:
     // gain over it of more than this many log2 units (about 5.7 %), about half the model's RMS
     // training error: where the model is not clearly better, the measured table stands.
     static constexpr float kModelMargin = 0.08f;
+    // Largest batch size of the model's training shapes (bigint_amd/planner.py sample_shapes).
+    static constexpr uint32_t kModelMaxBatch = 1024;
 
     // installed planner state
     std::vector<int8_t> table;  // [19][19][13] plan ids by rounded (log2 bits a, log2 bits b, log2 batch)
@@ -917,9 +923,9 @@ public:
 
     // Learned choice for a list of shapes: MLP forward + masked argmin on the GPU.
     int mlp_choose_ids(const std::vector<std::array<uint32_t, 3>>& shapes, const std::vector<bool>& sq,
-                       const std::vector<uint8_t>& allowed, std::vector<int32_t>& ids,
-                       hipStream_t st) {
+                       const std::vector<uint8_t>& allowed, std::vector<int32_t>& ids) {
         const int B = (int)shapes.size(), K = mlp_k;
+        const hipStream_t st = plan_stream;
         ids.assign(B, 0);
         if (B == 0 || K == 0 || !d_mlp) return BI_HIP_EINVAL;
         const size_t words = (size_t)B * 3 + 2 * (size_t)B * K + ((size_t)B * K + 3) / 4 + 4 + 2 * (size_t)B;
@@ -934,8 +940,10 @@ public:
             if (t < 0 || t >= K || !plan_for_id(shapes[i][0], shapes[i][1], sq[i], t, tmp))
                 t = heuristic_id(shapes[i][0], shapes[i][1], sq[i]);
             prior[i] = t < K ? t : -1;
+            // batch sizes beyond the training range (per-product times are flat there: the GPU
+            // is full) are presented as the range's largest
             const double f[3] = {std::log2(28.0 * shapes[i][0]), std::log2(28.0 * shapes[i][1]),
-                                 std::log2((double)std::max(1u, shapes[i][2]))};
+                                 std::log2((double)std::min(kModelMaxBatch, std::max(1u, shapes[i][2])))};
             for (int j = 0; j < 3; ++j) x[(size_t)i * 3 + j] = (float)((f[j] - mlp_norm[j]) / mlp_norm[3 + j]);
             // the model predicts log2(time / transform-count cost) per plan
             for (int k = 0; k < K; ++k) {
@@ -982,13 +990,48 @@ public:
         const size_t count = shapes.size();
         ids.assign(count, -1);
         if (mode == BI_HIP_PLAN_LEARNED && mlp_k > 0) {
-            std::vector<uint8_t> allowed(count * (size_t)mlp_k, 0);
-            for (size_t i = 0; i < count; ++i) {
+            // The choice depends on the shape only, and a product-tree level repeats shapes by the
+            // thousand: the model runs once per distinct (la, lb, batch, square).
+            std::vector<uint32_t> slot(count);
+            std::vector<std::array<uint32_t, 3>> ushapes;
+            std::vector<bool> usq;
+            {
+                size_t cap = 16;
+                while (cap < 2 * count) cap <<= 1;
+                std::vector<uint32_t> table_idx(cap, ~0u);  // open addressing over the distinct shapes
+                for (size_t i = 0; i < count; ++i) {
+                    const uint64_t h = (((uint64_t)shapes[i][0] * 0x9E3779B97F4A7C15ull) ^
+                                        ((uint64_t)shapes[i][1] * 0xC2B2AE3D27D4EB4Full) ^
+                                        ((uint64_t)shapes[i][2] << 1) ^ (uint64_t)sqv[i]) * 0x165667B19E3779F9ull;
+                    size_t j = (size_t)(h >> 32) & (cap - 1);
+                    while (true) {
+                        const uint32_t u = table_idx[j];
+                        if (u == ~0u) {
+                            table_idx[j] = (uint32_t)ushapes.size();
+                            slot[i] = (uint32_t)ushapes.size();
+                            ushapes.push_back(shapes[i]);
+                            usq.push_back(sqv[i]);
+                            break;
+                        }
+                        if (ushapes[u] == shapes[i] && usq[u] == sqv[i]) {
+                            slot[i] = u;
+                            break;
+                        }
+                        j = (j + 1) & (cap - 1);
+                    }
+                }
+            }
+            const size_t nu = ushapes.size();
+            std::vector<uint8_t> allowed(nu * (size_t)mlp_k, 0);
+            for (size_t i = 0; i < nu; ++i) {
                 PairPlan tmp;
                 for (int k = 0; k < mlp_k && k < kNumPlanIds; ++k)
-                    allowed[i * mlp_k + k] = plan_for_id(shapes[i][0], shapes[i][1], sqv[i], k, tmp) ? 1 : 0;
+                    allowed[i * mlp_k + k] = plan_for_id(ushapes[i][0], ushapes[i][1], usq[i], k, tmp) ? 1 : 0;
             }
-            return mlp_choose_ids(shapes, sqv, allowed, ids, st);
+            std::vector<int32_t> uids;
+            if (int rc = mlp_choose_ids(ushapes, usq, allowed, uids)) return rc;
+            for (size_t i = 0; i < count; ++i) ids[i] = uids[slot[i]];
+            return BI_HIP_OK;
         }
         if (mode == BI_HIP_PLAN_TABLE || mode == BI_HIP_PLAN_LEARNED)
             for (size_t i = 0; i < count; ++i) ids[i] = table_plan(shapes[i][0], shapes[i][1], shapes[i][2]);
