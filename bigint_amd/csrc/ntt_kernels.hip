@@ -15,9 +15,10 @@
 //     pointwise product and the first inverse pass;
 //   * twiddles for a tile are generated into LDS from a 4096-entry root table and one per-block
 //     root (tw[k] = W[k] * beta^tr), so no n-sized twiddle table is read;
-//   * Garner CRT is fused with carry propagation, which runs as a decoupled look-back scan over
-//     carry maps {0..3} -> {0..3} (ntt_multithread.cpp:384-442 does the same CRT and a serial
-//     three-way overlapped carry).
+//   * Garner CRT is fused with carry propagation: a scan of carry maps {0..3} -> {0..3} inside
+//     each tile, the carry between tiles settled from the previous tile's last word, and a fix-up
+//     pass that scans the tile maps for the rare tile that cannot settle it
+//     (ntt_multithread.cpp:384-442 does the same CRT and a serial three-way overlapped carry).
 //
 // Twiddle convention: W[m + i] = w_{2m}^{bitrev_{log2 m}(i)} (Montgomery form), used by the
 // butterfly of block i at the stage with m blocks, in the forward and in the inverse stages (the
@@ -633,7 +634,7 @@ __global__ __launch_bounds__(1 << (LR - 4)) void k_mid_pass(PassArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Garner CRT + carry propagation (decoupled look-back).
+// Garner CRT + carry propagation (tile-local carry resolution and a fix-up pass).
 //
 // Coefficient j of the convolution of the operands' 32-bit words is recovered from the three
 // inverse-transform outputs y_p (which carry a factor n * R^-1) as x_j < 2^90 (at most 2^26 terms
@@ -936,7 +937,7 @@ __device__ __forceinline__ void garner_tile(const GarnerArgs& a, const GarnerJob
     // The carry into the tile, resolved locally when possible: the previous tile's last word has
     // the sum s = w0(base-1) + w1(base-2) + w2(base-3) and a carry-in of at most 2, so its
     // carry-out is s >> 32 whatever that carry-in, unless s mod 2^32 > 2^32 - 3.  Then the tile
-    // needs nothing from its predecessors (0xFF: not resolved, use the look-back).
+    // needs nothing from its predecessors (0xFF: not resolved, left to k_carry_fixup).
     uint32_t local = 0xFFu;
     if (tid == 0 && tile != 0) {
         const uint64_t sl = (uint64_t)s_halo[2][0] + s_halo[1][1] + s_halo[0][2];

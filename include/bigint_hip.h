@@ -193,7 +193,7 @@ enum {
     BI_HIP_K_OUTER_FWD = 1, /* strided forward passes after the first */
     BI_HIP_K_MID = 2,       /* contiguous block pass: forward + pointwise + inverse */
     BI_HIP_K_OUTER_INV = 3, /* strided inverse passes */
-    BI_HIP_K_GARNER = 4,    /* Garner CRT + carry look-back scan */
+    BI_HIP_K_GARNER = 4,    /* Garner CRT + carry scan (and the carry fix-up check) */
     BI_HIP_K_OUTER_FIRST = 5, /* first strided forward pass (reads limbs, all three primes) */
     BI_HIP_K_COUNT = 6
 };
