@@ -84,25 +84,54 @@ struct Layout {
 
 // Twiddle table of one tile: tw[k], k in [1, 2^LR), for the stage with half-span
 // tr = 2^LR / (2 * 2^floor(log2 k)) rows:  tw[k] = W[k] * beta^tr.  bpow holds the block's
-// beta^(2^e), e < LR (precomputed per block, kBetaPowStride words); null encodes beta = 1.
-template <int I, int LR>
-__device__ __forceinline__ void build_twiddles(uint32_t* __restrict__ tw,
-                                               const uint32_t* __restrict__ wtab,
-                                               const uint32_t* __restrict__ bpow,
-                                               uint32_t nthreads) {
+// beta^(2^e), e < LR (precomputed per block, kBetaPowStride words); null encodes beta = 1, for
+// which the powers are the Montgomery form of 1 (a product with it returns the root unchanged, so
+// every block takes the same branch-free build).
+//
+// Built in two steps by NT threads, entry k = tid + NT*j for j < 2^LR / NT: load() issues every
+// load of the build at once, so a caller can put them ahead of its tile's data loads, and store()
+// forms the products and writes the table, waiting on memory once (a loop that loads and
+// multiplies entry by entry waits for each entry's loads, and for every load in flight before
+// them).  For j >= 1, floor(log2 k) = log2 NT + floor(log2 j) is the same for the whole block, so
+// those beta powers come from scalar loads; only j = 0 picks its power per lane.
+template <int I, int LR, int NT>
+struct TileTwiddles {
     static_assert(LR <= (int)kBetaPowStride, "beta power table too short");
-    constexpr uint32_t R = 1u << LR;
-    const uint32_t tid = threadIdx.x;
-    if (!bpow) {
-        for (uint32_t k = tid; k < R; k += nthreads) tw[k] = gmem(wtab)[k];
-        return;
+    static_assert((NT & (NT - 1)) == 0, "thread count must be a power of two");
+    static constexpr int R = 1 << LR;
+    static constexpr int J = R >= NT ? R / NT : 1;  // entries per thread
+    static constexpr int LNT = __builtin_ctz(NT);
+    static constexpr int LJ = __builtin_ctz(J);     // distinct uniform powers (j >= 1)
+    // 2^32 mod P: the Montgomery form of 1, the beta power of a block without a root
+    static constexpr uint32_t kOne = (uint32_t)((1ull << 32) % kP[I]);
+    uint32_t w[J];
+    uint32_t b0;
+    uint32_t bj[LJ > 0 ? LJ : 1];
+
+    __device__ __forceinline__ void load(const uint32_t* __restrict__ wtab,
+                                         const uint32_t* __restrict__ bpow) {
+        const uint32_t tid = threadIdx.x;
+#pragma unroll
+        for (int j = 0; j < J; ++j) w[j] = (R >= NT || tid < (uint32_t)R) ? gmem(wtab)[tid + NT * j] : 0u;
+        b0 = kOne;
+#pragma unroll
+        for (int e = 0; e < LJ; ++e) bj[e] = kOne;
+        if (bpow) {
+            const uint32_t log_m = 31u - __builtin_clz(tid | 1u);  // k = 0 is never read
+            if (R >= NT || tid < (uint32_t)R) b0 = gmem(bpow)[LR - 1 - (int)log_m];
+#pragma unroll
+            for (int e = 0; e < LJ; ++e) bj[e] = gmem(bpow)[LR - 1 - LNT - e];
+        }
     }
-    for (uint32_t k = tid; k < R; k += nthreads) {
-        if (k == 0) continue;
-        const uint32_t log_m = 31u - __builtin_clz(k);
-        tw[k] = mont_mul<I>(gmem(wtab)[k], gmem(bpow)[LR - 1 - log_m]);
+
+    __device__ __forceinline__ void store(uint32_t* __restrict__ tw) const {
+        const uint32_t tid = threadIdx.x;
+        if (R < NT && tid >= (uint32_t)R) return;
+        tw[tid] = mont_mul<I>(w[0], b0);
+#pragma unroll
+        for (int j = 1; j < J; ++j) tw[tid + NT * j] = mont_mul<I>(w[j], bj[31 - __builtin_clz(j)]);
     }
-}
+};
 
 // The block's beta powers in a pass's table ([prime][block][kBetaPowStride]), or null for
 // block 0 (beta = 1) and for passes without per-block roots.
@@ -379,9 +408,10 @@ __device__ __forceinline__ void outer_body(const PassArgs& a, const PassJob& job
     const uint32_t cg = t & ((1u << log_ncg) - 1u);
     const uint32_t tile_base = (blk << (LR + a.log_s)) + (cg << LC);
 
-    // both directions use the forward roots (see coeff_pos)
-    const uint32_t* wt = a.wtab + I * kTile;
-    build_twiddles<I, LR>(tw, wt, block_pows<I>(a.beta, a.beta_stride, blk), kThreads);
+    // both directions use the forward roots (see coeff_pos); the table's loads go out ahead of
+    // the tile's
+    TileTwiddles<I, LR, kThreads> tt;
+    tt.load(a.wtab + I * kTile, block_pows<I>(a.beta, a.beta_stride, blk));
 
     // residue planes in and out: a first pass, which reads limbs, runs as k_outer_first
     uint32_t* plane = (op == 0 ? job.res_a : job.res_b) + (size_t)I * n;
@@ -390,11 +420,13 @@ __device__ __forceinline__ void outer_body(const PassArgs& a, const PassJob& job
     uint32_t x[kE];
     if (!INV) {
         global_io<LR, LC, S::stages(G0), S::trlo(G0), false>(plane, kPlane, x, tile_base, a.log_s);
+        tt.store(tw);
         __syncthreads();
         tile_transform<I, false, LR, LC>(x, lds, tw);
         global_io<LR, LC, S::stages(GL), S::trlo(GL), true>(plane, kPlane, x, tile_base, a.log_s);
     } else {
         global_io<LR, LC, S::stages(GL), S::trlo(GL), false>(plane, kPlane, x, tile_base, a.log_s);
+        tt.store(tw);
         __syncthreads();
         tile_transform<I, true, LR, LC>(x, lds, tw);
         global_io<LR, LC, S::stages(G0), S::trlo(G0), true>(plane, kPlane, x, tile_base, a.log_s);
@@ -424,16 +456,20 @@ __device__ __forceinline__ void outer_first_body(const PassArgs& a, const PassJo
     // row of the tile lies past the operand (an operand filling at most half the transform, the
     // usual case), those rows are not read and the first stage is a copy.
     const bool half = tile_base + (1u << (LR - 1 + a.log_s)) >= source_words(src);
+    // The three primes' twiddle tables (3 x 2^LR words) are built at once, their loads issued
+    // ahead of the limb loads, so no prime waits for its table behind a barrier.
+    TileTwiddles<0, LR, kThreads> tt0;
+    TileTwiddles<1, LR, kThreads> tt1;
+    TileTwiddles<2, LR, kThreads> tt2;
+    tt0.load(a.wtab + 0 * kTile, block_pows<0>(a.beta, a.beta_stride, blk));
+    tt1.load(a.wtab + 1 * kTile, block_pows<1>(a.beta, a.beta_stride, blk));
+    tt2.load(a.wtab + 2 * kTile, block_pows<2>(a.beta, a.beta_stride, blk));
     uint32_t words[kE];
     if (half) load_words<LR, LC, S::stages(G0), S::trlo(G0), true>(src, words, tile_base, a.log_s);
     else load_words<LR, LC, S::stages(G0), S::trlo(G0), false>(src, words, tile_base, a.log_s);
-    // The three primes' twiddle tables (3 x 2^LR words) are built at once, overlapping the limb
-    // loads, so no prime waits for its table behind a barrier.
-    static_for<kNumPrimes>([&](auto pi) {
-        constexpr int I = (int)pi;
-        build_twiddles<I, LR>(tw + (I << LR), a.wtab + I * kTile,
-                              block_pows<I>(a.beta, a.beta_stride, blk), kThreads);
-    });
+    tt0.store(tw + (0 << LR));
+    tt1.store(tw + (1 << LR));
+    tt2.store(tw + (2 << LR));
     __syncthreads();
     static_for<kNumPrimes>([&](auto pi) {
         constexpr int I = (int)pi;
@@ -506,21 +542,20 @@ __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, 
     const uint32_t blk = blockIdx.x;
     const uint32_t tile_base = blk << LR;
     // one twiddle table per tile serves the forward and the inverse stages (see coeff_pos)
-    const uint32_t* wf = a.wtab + I * kTile;
-    const uint32_t* bf = block_pows<I>(a.beta, a.beta_stride, blk);
+    TileTwiddles<I, LR, T> tt;
+    tt.load(a.wtab + I * kTile, block_pows<I>(a.beta, a.beta_stride, blk));
     if (a.flags & kInvOnly) {  // a product spectrum from the diagonal sums: inverse stages only
-        build_twiddles<I, LR>(tw, wf, bf, T);
         using Sx = Sched<LR>;
         constexpr LimbSrc kPl{nullptr, 0u, 0u, 0u};
         uint32_t* p = job.res_a + (size_t)I * n;
         uint32_t x[kE];
         global_io<LR, 0, Sx::stages(Sx::NG - 1), Sx::trlo(Sx::NG - 1), false>(p, kPl, x, tile_base, 0);
+        tt.store(tw);
         __syncthreads();
         tile_transform<I, true, LR, 0>(x, lds, tw);
         global_io<LR, 0, Sx::stages(0), Sx::trlo(0), true>(p, kPl, x, tile_base, 0);
         return;
     }
-    build_twiddles<I, LR>(tw, wf, bf, T);
 
     constexpr bool from_limbs = LIMBS;
     const bool square = (a.flags & kSquare) != 0;
@@ -531,6 +566,19 @@ __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, 
     constexpr LimbSrc kPlane{nullptr, 0u, 0u, 0u};
     const LimbSrc src_a = from_limbs ? LimbSrc{job.limbs_a, job.la, job.a_off, job.a_cnt} : kPlane;
     global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pa, src_a, xa, tile_base, 0);
+    // B's residues are loaded before A is transformed where that costs no occupancy (2^11 and
+    // 2^12-point tiles hold 16 more registers within the same waves per SIMD), after it otherwise
+    constexpr bool kEarlyB = LR >= 11;
+    const bool need_b = !square && !(a.flags & kFwdOnly);
+    const bool bpre = (a.flags & kBPre) != 0;
+    uint32_t* pb = job.res_b + (size_t)I * n;
+    const LimbSrc src_b = from_limbs ? LimbSrc{job.limbs_b, job.lb, 0u, ~0u} : kPlane;
+    uint32_t xb[kE];
+    if (kEarlyB && need_b) {
+        if (bpre) global_io<LR, 0, S::stages(GL), S::trlo(GL), false>(pb, kPlane, xb, tile_base, 0);
+        else global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pb, src_b, xb, tile_base, 0);
+    }
+    tt.store(tw);
     if (from_limbs) words_to_residues<I>(xa);
     __syncthreads();  // forward twiddles ready
     tile_transform<I, false, LR, 0>(xa, lds, tw);
@@ -539,13 +587,10 @@ __device__ __forceinline__ void mid_body(const PassArgs& a, const PassJob& job, 
         return;
     }
     if (!square) {
-        uint32_t* pb = job.res_b + (size_t)I * n;
-        uint32_t xb[kE];
-        if (a.flags & kBPre) {
-            global_io<LR, 0, S::stages(GL), S::trlo(GL), false>(pb, kPlane, xb, tile_base, 0);
+        if (bpre) {
+            if (!kEarlyB) global_io<LR, 0, S::stages(GL), S::trlo(GL), false>(pb, kPlane, xb, tile_base, 0);
         } else {
-            const LimbSrc src_b = from_limbs ? LimbSrc{job.limbs_b, job.lb, 0u, ~0u} : kPlane;
-            global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pb, src_b, xb, tile_base, 0);
+            if (!kEarlyB) global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pb, src_b, xb, tile_base, 0);
             if (from_limbs) words_to_residues<I>(xb);
             tile_transform<I, false, LR, 0>(xb, lds, tw);
         }
@@ -576,6 +621,12 @@ __device__ __forceinline__ void mid_pair_body(const PassArgs& a, const PassJob& 
     uint32_t* pb = job.res_b + (size_t)I * n;
     const uint32_t* wf = a.wtab + I * kTile;  // forward roots serve both directions
     const uint32_t blk0 = 2u * blockIdx.x;
+    // Each tile's twiddle loads are issued ahead of the loads and stores in front of its table
+    // build, so the build waits only for them.  The loop runs as a loop (its body is large), so the
+    // reload for the second tile is unconditional: a conditional one would keep the first tile's
+    // values live through the whole body.  On the second pass it reloads the second tile's.
+    TileTwiddles<I, LR, T> tt;
+    tt.load(wf, block_pows<I>(a.beta, a.beta_stride, blk0));
     uint32_t xa[kE], xb[kE], na[kE];
     global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pa, kPlane, xa, blk0 << LR, 0);
     if (!square) global_io<LR, 0, S::stages(G0), S::trlo(G0), false>(pb, kPlane, xb, blk0 << LR, 0);
@@ -588,7 +639,7 @@ __device__ __forceinline__ void mid_pair_body(const PassArgs& a, const PassJob& 
             for (int k = 0; k < kE; ++k) xa[k] = na[k];
             __syncthreads();  // the first tile is done with tw and lds
         }
-        build_twiddles<I, LR>(tw, wf, block_pows<I>(a.beta, a.beta_stride, blk), T);
+        tt.store(tw);
         __syncthreads();
         tile_transform<I, false, LR, 0>(xa, lds, tw);
         if (!square) {
@@ -602,6 +653,7 @@ __device__ __forceinline__ void mid_pair_body(const PassArgs& a, const PassJob& 
             for (int k = 0; k < kE; ++k) xa[k] = pointwise<I>(xa[k], xa[k]);
         }
         tile_transform<I, true, LR, 0>(xa, lds, tw);
+        tt.load(wf, block_pows<I>(a.beta, a.beta_stride, blk0 + 1u));
         global_io<LR, 0, S::stages(G0), S::trlo(G0), true>(pa, kPlane, xa, blk << LR, 0);
     }
 }
