@@ -882,9 +882,13 @@ public:
     }
 
     // Learned mode keeps the fallback's plan (table, or heuristic) unless the model predicts a
-    // gain over it of more than this many log2 units (about 5.7 %), about half the model's RMS
-    // training error: where the model is not clearly better, the measured table stands.
-    static constexpr float kModelMargin = 0.08f;
+    // gain over it of more than kModelMargin log2 units (about 2.8 %), and keeps it for launch-
+    // bound work, where the fallback's plan is predicted to take under kModelMinBatchUs for the
+    // whole batch: there launch costs make plan timings irregular and the measured table is the
+    // better judge.  Both were chosen on the training timings (the lowest mean regret over a grid
+    // of margins 0.04-0.16 and floors 0-2000 us) and checked on the held-out ones.
+    static constexpr float kModelMargin = 0.04f;
+    static constexpr double kModelMinBatchUs = 200.0;
     // Largest batch size of the model's training shapes (bigint_amd/planner.py sample_shapes).
     static constexpr uint32_t kModelMaxBatch = 1024;
 
@@ -928,10 +932,10 @@ public:
         const hipStream_t st = plan_stream;
         ids.assign(B, 0);
         if (B == 0 || K == 0 || !d_mlp) return BI_HIP_EINVAL;
-        const size_t words = (size_t)B * 3 + 2 * (size_t)B * K + ((size_t)B * K + 3) / 4 + 4 + 2 * (size_t)B;
+        const size_t words = (size_t)B * 3 + 2 * (size_t)B * K + ((size_t)B * K + 3) / 4 + 4 + 3 * (size_t)B;
         float* w = scratch(words + 64);
         if (!w) return BI_HIP_ENOMEM;
-        std::vector<float> x((size_t)B * 3), off((size_t)B * K, 0.f);
+        std::vector<float> x((size_t)B * 3), off((size_t)B * K, 0.f), lbatch((size_t)B);
         std::vector<int32_t> prior((size_t)B);
         for (int i = 0; i < B; ++i) {
             // the fallback's plan: the table's, or the heuristic's where the table defers to it
@@ -945,6 +949,7 @@ public:
             const double f[3] = {std::log2(28.0 * shapes[i][0]), std::log2(28.0 * shapes[i][1]),
                                  std::log2((double)std::min(kModelMaxBatch, std::max(1u, shapes[i][2])))};
             for (int j = 0; j < 3; ++j) x[(size_t)i * 3 + j] = (float)((f[j] - mlp_norm[j]) / mlp_norm[3 + j]);
+            lbatch[i] = (float)std::log2((double)std::max(1u, shapes[i][2]));
             // the model predicts log2(time / transform-count cost) per plan
             for (int k = 0; k < K; ++k) {
                 const double c = plan_cost(shapes[i][0], shapes[i][1], sq[i], k);
@@ -957,13 +962,16 @@ public:
         uint8_t* dallow = reinterpret_cast<uint8_t*>(doff + (size_t)B * K);
         int32_t* dchoice = reinterpret_cast<int32_t*>(doff + (size_t)B * K + ((size_t)B * K + 3) / 4 + 1);
         int32_t* dprior = dchoice + B;
+        float* dlbatch = reinterpret_cast<float*>(dprior + B);
         if (hipMemcpyAsync(dx, x.data(), x.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
             hipMemcpyAsync(doff, off.data(), off.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
             hipMemcpyAsync(dallow, allowed.data(), allowed.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
-            hipMemcpyAsync(dprior, prior.data(), prior.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+            hipMemcpyAsync(dprior, prior.data(), prior.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(dlbatch, lbatch.data(), lbatch.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
             return BI_HIP_EDEVICE;
         if (mlp_forward(dx, B, d_mlp, K, dy, nullptr, nullptr, st) != hipSuccess) return BI_HIP_EDEVICE;
-        if (mlp_choose(dy, doff, dallow, dprior, kModelMargin, B, K, dchoice, st) != hipSuccess)
+        if (mlp_choose(dy, doff, dallow, dprior, kModelMargin, dlbatch, (float)std::log2(kModelMinBatchUs), B, K,
+                       dchoice, st) != hipSuccess)
             return BI_HIP_EDEVICE;
         if (hipMemcpyAsync(ids.data(), dchoice, (size_t)B * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
             return BI_HIP_EDEVICE;

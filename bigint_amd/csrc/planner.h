@@ -18,9 +18,12 @@ hipError_t mlp_grad(const float* x, const float* t, const float* mask, int B, co
                     float* part_loss, float* grad, hipStream_t st);
 hipError_t mlp_adam(float* p, const float* g, float* m, float* v, int NP, float lr, float b1,
                     float b2, float eps, int step, hipStream_t st);
-// Per sample the allowed plan minimising y + offset, unless prior[s] (the table's or the
-// heuristic's plan, -1 for none) is allowed and scores within `margin` of it: then prior[s].
+// Per sample the allowed plan minimising y + offset (log2 of the predicted time per product),
+// unless prior[s] (the table's or the heuristic's plan, -1 for none) is allowed and either scores
+// within `margin` of it or, with log2_batch (log2 of each sample's batch size), is predicted to
+// take under 2^min_log2_us for the whole batch: then prior[s].
 hipError_t mlp_choose(const float* y, const float* offset, const uint8_t* allowed, const int32_t* prior,
-                      float margin, int B, int K, int32_t* choice, hipStream_t st);
+                      float margin, const float* log2_batch, float min_log2_us, int B, int K,
+                      int32_t* choice, hipStream_t st);
 
 }  // namespace bi

@@ -210,7 +210,8 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 // Plan choice: argmin of the predicted cost over the plans allowed for each sample.
 __global__ void k_argmin_plan(const float* __restrict__ y, const float* __restrict__ offset,
                               const uint8_t* __restrict__ allowed, const int32_t* __restrict__ prior,
-                              float margin, int B, int K, int32_t* __restrict__ choice) {
+                              float margin, const float* __restrict__ log2_batch, float min_log2_us,
+                              int B, int K, int32_t* __restrict__ choice) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= B) return;
     int best = -1;
@@ -223,11 +224,12 @@ __global__ void k_argmin_plan(const float* __restrict__ y, const float* __restri
             bv = c;
         }
     }
-    // the prior plan stands unless the model predicts a gain over it larger than the margin
+    // the prior plan stands unless the model predicts a gain over it larger than the margin, and
+    // (with log2_batch) unless its predicted time for the whole batch is below 2^min_log2_us
     const int pr = prior ? prior[s] : -1;
     if (best >= 0 && pr >= 0 && pr < K && allowed[s * K + pr]) {
         const float cp = y[s * K + pr] + (offset ? offset[s * K + pr] : 0.f);
-        if (cp - bv < margin) best = pr;
+        if (cp - bv < margin || (log2_batch && cp + log2_batch[s] < min_log2_us)) best = pr;
     }
     choice[s] = best < 0 ? 0 : best;
 }
@@ -272,11 +274,12 @@ hipError_t mlp_adam(float* p, const float* g, float* m, float* v, int NP, float 
 }
 
 hipError_t mlp_choose(const float* y, const float* offset, const uint8_t* allowed, const int32_t* prior,
-                      float margin, int B, int K, int32_t* choice, hipStream_t st) {
+                      float margin, const float* log2_batch, float min_log2_us, int B, int K,
+                      int32_t* choice, hipStream_t st) {
     if (B <= 0) return hipSuccess;
     (void)hipGetLastError();  // report this launch's error, not an earlier ignored one
     hipLaunchKernelGGL(k_argmin_plan, dim3((B + 255) / 256), dim3(256), 0, st, y, offset, allowed, prior,
-                       margin, B, K, choice);
+                       margin, log2_batch, min_log2_us, B, K, choice);
     return hipGetLastError();
 }
 

@@ -107,6 +107,9 @@ def main() -> int:
     ap.add_argument("--from-records", default=None,
                     help="train and evaluate on saved records (JSON lines with a 'split' field) "
                          "instead of timing new shapes")
+    ap.add_argument("--evaluate", default=None,
+                    help="with --from-records: evaluate this planner file (table and model as "
+                         "committed) on the held-out records, without training")
     args = ap.parse_args()
 
     engine.init(0)
@@ -121,6 +124,19 @@ def main() -> int:
             if k * (la + lb) <= max_out and la <= timer.max_limbs and lb <= timer.max_limbs:
                 res.append((la, lb, k))
         return res
+
+    if args.evaluate:
+        if not args.from_records:
+            ap.error("--evaluate needs --from-records")
+        with open(args.from_records) as f:
+            recs_h = [r for r in (json.loads(line) for line in f if line.strip()) if r.get("split") == "heldout"]
+        table, params, norm, _ = P.load(args.evaluate, do_install=False)
+        ev = evaluate(recs_h, table, params, norm)
+        log(json.dumps(ev))
+        if args.report:
+            with open(args.report, "w") as f:
+                json.dump({"evaluation": ev, "planner": os.path.basename(args.evaluate)}, f, indent=1)
+        return 0
 
     t0 = time.time()
     if args.from_records:

@@ -242,7 +242,8 @@ int bi_hip_choose_plans(const uint32_t* shapes, const int* is_square, size_t cou
  * heuristic; NULL keeps the current one) and/or the MLP (fp32 parameters for K outputs, input
  * normalisation {mean[3], std[3]}; NULL keeps it).  In BI_HIP_PLAN_LEARNED mode the model's
  * choice replaces the table's (or the heuristic's) plan only where it predicts a gain of more
- * than 0.08 in log2 time. */
+ * than 0.04 in log2 time and the table's plan is predicted to take at least 200 us for the batch
+ * (below that, launch costs dominate and the measured table decides). */
 int bi_hip_set_planner(const int8_t* table, const float* mlp_params, int K, const float* norm6);
 /* Install table and MLP from the binary file bigint_amd/planner.py writes
  * (b"BIPL", int32 K, int8 table[19*19*13], int32 count, float32 params[count], float32 norm[6]);
