@@ -12,6 +12,8 @@
 // Q(1, r).
 //
 // usage: pi <decimal digits> [plan mode 0|1|2] [output file] [planner file]
+#include <malloc.h>
+
 #include <bit>
 #include <chrono>
 #include <cmath>
@@ -362,6 +364,14 @@ int main(int argc, char** argv) {
     auto [P, Q] = split(dec_digits / 14 + 5, plan_mode, levels_ms);
     const auto t_split = clk::now();
     const double eng_split = mul::engine_time_ms();
+
+    // The Newton iterations, the final products and the decimal output allocate and free vectors
+    // of 50-270 MB many times.  glibc serves blocks above its mmap threshold with fresh mappings,
+    // so every such vector paid for its pages again (about 0.6 s of page faults at 10^8 digits);
+    // from here on they stay in the heap and freed blocks are reused.  (Set after the splitting:
+    // its eight threads allocating many small vectors run faster with the default policy.)
+    mallopt(M_MMAP_THRESHOLD, 1 << 30);
+    mallopt(M_TRIM_THRESHOLD, 1 << 30);
 
     const bool verbose = std::getenv("PI_LEVELS") != nullptr;
     auto phase = [&, t = clk::now()](const char* name) mutable {

@@ -36,6 +36,102 @@ auto thread_pool_size() -> uint32_t {
     return g_pool_threads.load();
 }
 
+// ---- limb passes over large operands on worker threads ------------------------------------------
+// Additions, subtractions, small-factor products and shifts of operands of millions of limbs
+// (the Newton iterations of a 10^8-digit pi) are memory passes; they run in contiguous chunks on
+// up to the pool's thread count.  The count is read without fixing the pool: as in the
+// reference, only products fix it.
+namespace {
+constexpr size_t kPassMinLimbs = size_t{1} << 20;  // below this a pass stays on the calling thread
+constexpr size_t kPassChunkLimbs = size_t{1} << 18;
+
+size_t pass_threads(size_t n) {
+    if (n < kPassMinLimbs) return 1;
+    uint32_t p = g_pool_threads.load();
+    if (p == 0) p = std::max(1u, std::thread::hardware_concurrency());
+    return std::max<size_t>(1, std::min<size_t>(p, n / kPassChunkLimbs));
+}
+
+// f(t, lo, hi) for nt contiguous chunks of [0, n); chunk 0 on the calling thread.
+template <class F>
+void run_chunks(size_t n, size_t nt, F&& f) {
+    const size_t per = (n + nt - 1) / nt;
+    std::vector<std::thread> th;
+    th.reserve(nt - 1);
+    for (size_t t = 1; t < nt; ++t)
+        th.emplace_back([&f, t, per, n] { f(t, std::min(n, t * per), std::min(n, (t + 1) * per)); });
+    f(0, 0, std::min(n, per));
+    for (auto& x : th) x.join();
+}
+
+// A pass whose limbs pass a carry (or borrow) upwards: body(lo, hi) computes limbs [lo, hi) with
+// carry-in 0 and returns the carry-out; fix(lo, hi, c) adds carry c into limbs [lo, hi) (a
+// ripple from lo) and returns what passes hi.  Chunks run in parallel, then the carries are
+// settled chunk by chunk from the bottom.  Returns the carry out of limb n - 1.  The carry into a
+// chunk is the sum of its predecessor's own carry-out and what rippled through it; for additions
+// and subtractions that sum is at most 1 (a chunk that a carry ripples through entirely had no
+// carry-out of its own).
+template <class Body, class Fix>
+uint128_t carry_pass(size_t n, Body&& body, Fix&& fix) {
+    const size_t nt = pass_threads(n);
+    if (nt <= 1) return body(size_t{0}, n);
+    std::vector<uint128_t> cout(nt);
+    run_chunks(n, nt, [&](size_t t, size_t lo, size_t hi) { cout[t] = lo < hi ? body(lo, hi) : 0; });
+    const size_t per = (n + nt - 1) / nt;
+    uint128_t c = cout[0];
+    for (size_t t = 1; t < nt; ++t) {
+        const size_t lo = std::min(n, t * per), hi = std::min(n, (t + 1) * per);
+        c = (c && lo < hi ? fix(lo, hi, c) : c) + cout[t];
+    }
+    return c;
+}
+
+// d[0, n) *= b in 28-bit limbs; returns the carry out of the top limb.
+uint128_t mul_small_pass(uint32_t* d, size_t n, uint64_t b) {
+    return carry_pass(
+        n,
+        [d, b](size_t lo, size_t hi) {
+            uint128_t carry = 0;
+            for (size_t i = lo; i < hi; ++i) {
+                const uint128_t v = (uint128_t)b * d[i] + carry;
+                d[i] = (uint32_t)v & DIGIT_MASK;
+                carry = v >> DIGIT_BITS;
+            }
+            return carry;
+        },
+        [d](size_t lo, size_t hi, uint128_t c) {
+            for (size_t i = lo; c && i < hi; ++i) {
+                c += d[i];
+                d[i] = (uint32_t)c & DIGIT_MASK;
+                c >>= DIGIT_BITS;
+            }
+            return c;
+        });
+}
+
+// Ripple a carry of 1 (add) or a borrow of 1 (subtract) into r[lo, hi); returns 1 if it passes hi.
+uint128_t ripple_add1(uint32_t* r, size_t lo, size_t hi) {
+    for (size_t i = lo; i < hi; ++i) {
+        if (r[i] != DIGIT_MASK) {
+            ++r[i];
+            return 0;
+        }
+        r[i] = 0;
+    }
+    return 1;
+}
+uint128_t ripple_sub1(uint32_t* r, size_t lo, size_t hi) {
+    for (size_t i = lo; i < hi; ++i) {
+        if (r[i] != 0) {
+            --r[i];
+            return 0;
+        }
+        r[i] = DIGIT_MASK;
+    }
+    return 1;
+}
+}  // namespace
+
 std::array<size_type, 2> BigInt::DEC_STRING_BRUTE_THRESHOLDS = DEC_STRING_BRUTE_THRESHOLDS_DEFAULT;
 
 // ---- parsing (src/bigint.cpp:25-124) ------------------------------------------------------------
@@ -417,24 +513,44 @@ namespace {
 // |x| + |y| into r (limbs), canonical
 void mag_add(Digits& r, const Digits& x, const Digits& y) {
     const size_type n = std::max(x.size(), y.size());
-    r.resize(n);
-    uint32_t carry = 0;
-    for (size_type i = 0; i < n; ++i) {
-        const uint32_t v = (i < x.size() ? x[i] : 0u) + (i < y.size() ? y[i] : 0u) + carry;
-        r[i] = v & DIGIT_MASK;
-        carry = v >> DIGIT_BITS;
-    }
-    if (carry) r.push_back(carry);
+    const size_type nx = x.size(), ny = y.size();
+    r.resize(n);  // r may be x or y: only limbs past the old end are new
+    const uint32_t* xp = x.data();
+    const uint32_t* yp = y.data();
+    uint32_t* rp = r.data();
+    const uint128_t carry = carry_pass(
+        n,
+        [=](size_t lo, size_t hi) {
+            uint32_t c = 0;
+            for (size_t i = lo; i < hi; ++i) {
+                const uint32_t v = (i < nx ? xp[i] : 0u) + (i < ny ? yp[i] : 0u) + c;
+                rp[i] = v & DIGIT_MASK;
+                c = v >> DIGIT_BITS;
+            }
+            return (uint128_t)c;
+        },
+        [=](size_t lo, size_t hi, uint128_t) { return ripple_add1(rp, lo, hi); });
+    if (carry) r.push_back((uint32_t)carry);
 }
 // |x| - |y| into r with |x| >= |y|
 void mag_sub(Digits& r, const Digits& x, const Digits& y) {
-    r.resize(x.size());
-    uint32_t borrow = 0;
-    for (size_type i = 0; i < x.size(); ++i) {
-        const uint32_t v = x[i] - (i < y.size() ? y[i] : 0u) - borrow;
-        r[i] = v & DIGIT_MASK;
-        borrow = v >> 31;  // wrapped
-    }
+    const size_type nx = x.size(), ny = y.size();
+    r.resize(nx);
+    const uint32_t* xp = x.data();
+    const uint32_t* yp = y.data();
+    uint32_t* rp = r.data();
+    (void)carry_pass(
+        nx,
+        [=](size_t lo, size_t hi) {
+            uint32_t borrow = 0;
+            for (size_t i = lo; i < hi; ++i) {
+                const uint32_t v = xp[i] - (i < ny ? yp[i] : 0u) - borrow;
+                rp[i] = v & DIGIT_MASK;
+                borrow = v >> 31;  // wrapped
+            }
+            return (uint128_t)borrow;
+        },
+        [=](size_t lo, size_t hi, uint128_t) { return ripple_sub1(rp, lo, hi); });
 }
 }  // namespace
 
@@ -475,12 +591,7 @@ void BigInt::unsigned_inplace_mul(uint64_t b) {
         *this <<= std::countr_zero(b);
         return;
     }
-    uint128_t carry = 0;
-    for (auto& d : data_) {
-        const uint128_t v = (uint128_t)b * d + carry;
-        d = (uint32_t)v & DIGIT_MASK;
-        carry = v >> DIGIT_BITS;
-    }
+    uint128_t carry = mul_small_pass(data_.data(), data_.size(), b);
     while (carry) {
         data_.push_back((uint32_t)carry & DIGIT_MASK);
         carry >>= DIGIT_BITS;
@@ -735,17 +846,45 @@ void BigFloat::shift(int64_t offset) {
     remove_leading_zero();
 }
 
+// In place, in chunks: each chunk reads the neighbour limb across its edge before any chunk
+// writes (edge[]), so the chunks are independent.
 void BigFloat::shift_right(Digits& v, uint32_t offset) {
-    const size_type n = v.size();
-    for (size_type i = 0; i + 1 < n; ++i) v[i] = ((v[i] >> offset) | (v[i + 1] << (DIGIT_BITS - offset))) & DIGIT_MASK;
-    v[n - 1] >>= offset;
+    const size_t n = v.size();
+    uint32_t* d = v.data();
+    const size_t nt = pass_threads(n), per = (n + nt - 1) / nt;
+    if (nt == 1) {
+        for (size_t i = 0; i + 1 < n; ++i) d[i] = ((d[i] >> offset) | (d[i + 1] << (DIGIT_BITS - offset))) & DIGIT_MASK;
+        d[n - 1] >>= offset;
+        return;
+    }
+    std::vector<uint32_t> edge(nt);  // limb above each chunk (0 above the top)
+    for (size_t t = 0; t < nt; ++t) edge[t] = (t + 1) * per < n ? d[(t + 1) * per] : 0u;
+    run_chunks(n, nt, [&](size_t t, size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; ++i) {
+            const uint32_t above = i + 1 < hi ? d[i + 1] : edge[t];
+            d[i] = ((d[i] >> offset) | (above << (DIGIT_BITS - offset))) & DIGIT_MASK;
+        }
+    });
 }
 
 void BigFloat::shift_left(Digits& v, uint32_t offset) {
     v.push_back(0);
-    for (size_type i = v.size() - 1; i > 0; --i)
-        v[i] = ((v[i] << offset) | (v[i - 1] >> (DIGIT_BITS - offset))) & DIGIT_MASK;
-    v[0] = (v[0] << offset) & DIGIT_MASK;
+    const size_t n = v.size();
+    uint32_t* d = v.data();
+    const size_t nt = pass_threads(n), per = (n + nt - 1) / nt;
+    if (nt == 1) {
+        for (size_t i = n - 1; i > 0; --i) d[i] = ((d[i] << offset) | (d[i - 1] >> (DIGIT_BITS - offset))) & DIGIT_MASK;
+        d[0] = (d[0] << offset) & DIGIT_MASK;
+        return;
+    }
+    std::vector<uint32_t> edge(nt);  // limb below each chunk (0 below the bottom)
+    for (size_t t = 0; t < nt; ++t) edge[t] = t > 0 && t * per <= n ? d[t * per - 1] : 0u;
+    run_chunks(n, nt, [&](size_t t, size_t lo, size_t hi) {
+        for (size_t i = hi; i-- > lo;) {
+            const uint32_t below = i > lo ? d[i - 1] : edge[t];
+            d[i] = ((d[i] << offset) | (below >> (DIGIT_BITS - offset))) & DIGIT_MASK;
+        }
+    });
 }
 
 auto BigFloat::remove_leading_zero() -> size_type {
@@ -794,13 +933,19 @@ auto BigFloat::add_or_sub(const BigFloat& a, const BigFloat& b, bool is_sub) -> 
     uint32_t* r = res.data_.data();
     if (!subtract) {  // |a| + |b| with the sign of a
         res.is_neg_ = a.is_neg_;
-        uint32_t c = 0;
-        for (int64_t i = 0; i < lr; ++i) {
-            const uint32_t t = A(i) + B(i) + c;
-            r[i] = t & DIGIT_MASK;
-            c = t >> DIGIT_BITS;
-        }
-        r[lr] = c;
+        const uint128_t c = carry_pass(
+            (size_t)lr,
+            [&](size_t lo, size_t hi) {
+                uint32_t cc = 0;
+                for (int64_t i = (int64_t)lo; i < (int64_t)hi; ++i) {
+                    const uint32_t t = A(i) + B(i) + cc;
+                    r[i] = t & DIGIT_MASK;
+                    cc = t >> DIGIT_BITS;
+                }
+                return (uint128_t)cc;
+            },
+            [r](size_t lo, size_t hi, uint128_t) { return ripple_add1(r, lo, hi); });
+        r[lr] = (uint32_t)c;
     } else {  // the larger magnitude minus the smaller, with the larger one's sign
         int64_t top = lr - 1;
         while (top >= 0 && A(top) == B(top)) --top;
@@ -812,13 +957,19 @@ auto BigFloat::add_or_sub(const BigFloat& a, const BigFloat& b, bool is_sub) -> 
         }
         const bool a_big = A(top) > B(top);
         res.is_neg_ = a_big ? a.is_neg_ : !a.is_neg_;
-        uint32_t borrow = 0;
-        for (int64_t i = 0; i < lr; ++i) {
-            const uint32_t x = a_big ? A(i) : B(i);
-            const uint32_t y = (a_big ? B(i) : A(i)) + borrow;
-            borrow = x < y;
-            r[i] = (x + (borrow << DIGIT_BITS) - y) & DIGIT_MASK;
-        }
+        (void)carry_pass(
+            (size_t)lr,
+            [&](size_t lo, size_t hi) {
+                uint32_t borrow = 0;
+                for (int64_t i = (int64_t)lo; i < (int64_t)hi; ++i) {
+                    const uint32_t x = a_big ? A(i) : B(i);
+                    const uint32_t y = (a_big ? B(i) : A(i)) + borrow;
+                    borrow = x < y;
+                    r[i] = (x + (borrow << DIGIT_BITS) - y) & DIGIT_MASK;
+                }
+                return (uint128_t)borrow;
+            },
+            [r](size_t lo, size_t hi, uint128_t) { return ripple_sub1(r, lo, hi); });
         r[lr] = 0;
     }
     res.remove_leading_zero();
@@ -836,12 +987,7 @@ void BigFloat::unsigned_inplace_mul(uint64_t b) {
         *this <<= std::countr_zero(b);
         return;
     }
-    uint128_t carry = 0;
-    for (auto& d : data_) {
-        const uint128_t v = (uint128_t)b * d + carry;
-        d = (uint32_t)v & DIGIT_MASK;
-        carry = v >> DIGIT_BITS;
-    }
+    uint128_t carry = mul_small_pass(data_.data(), data_.size(), b);
     while (carry) {
         data_.push_back((uint32_t)carry & DIGIT_MASK);
         carry >>= DIGIT_BITS;

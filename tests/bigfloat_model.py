@@ -26,9 +26,7 @@ def canon_tail(f):
     neg, D, pp = f
     if D == 0:
         return (False, 0, 0)
-    z = 0
-    while limb(D, z) == 0:
-        z += 1
+    z = ((D & -D).bit_length() - 1) // DB  # whole zero limbs at the bottom
     return (neg, D >> (DB * z), pp - z)
 
 

@@ -127,6 +127,13 @@ int main() {
                 if (op == "fmul") out << XF(BigFloat::mul(fa, fb, prec));
                 else if (op == "fadd") out << XF(fa + fb);
                 else out << XF(fa - fb);
+            } else if (op == "fmuli" || op == "fshift") {  // BigFloat *= int64, BigFloat <<= bits
+                long long pa, k;
+                in >> a >> pa >> k;
+                BigFloat fa = F(a, pa);
+                if (op == "fmuli") fa *= (int64_t)k;
+                else fa <<= (int64_t)k;
+                out << XF(fa);
             } else if (op == "fsqr") {  // one object as both operands: the squaring route
                 long long pa;
                 size_t prec;

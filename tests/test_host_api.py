@@ -517,3 +517,33 @@ def test_batch_of_host_products_on_pool_threads(calc):
     assert int(calc("batch", 20000, 77, 8000)) == _batch_checksum(20000, 77, 8000)
     g1, h1 = map(int, calc("counts").split())
     assert g1 == g0 and h1 - h0 == 20000
+
+
+def test_limb_passes_on_worker_threads(calc):
+    """Additions, subtractions, small-factor products and shifts of operands above a million
+    limbs run in chunks on worker threads, carries settled between chunks afterwards: compare them
+    with CPython ints and the BigFloat model, including carries and borrows that ripple through
+    every chunk (all-ones and power-of-two operands)."""
+    rng = random.Random(11)
+    n = 2_600_000  # limbs
+    big = rng.getrandbits(28 * n) | (1 << (28 * n - 1))
+    big2 = rng.getrandbits(28 * n - 5) | 1
+    ones = (1 << (28 * n)) - 1
+    power = 1 << (28 * n)
+    # BigInt magnitude passes
+    for a, b in ((big, big2), (ones, 1), (power, -1), (big, -big2), (-big2, big)):
+        assert unhx(calc("add", hx(a), hx(b))) == a + b
+        assert unhx(calc("sub", hx(a), hx(b))) == a - b
+    for a, k in ((big, 10005), (ones, 3), (big2, (1 << 62) + 12345), (-ones, -(1 << 40) - 7)):
+        assert unhx(calc("muli", hx(a), k)) == a * k
+    # BigFloat passes (frames offset by a few limbs, so the operands overlap partially)
+    for a, pa, b, pb in ((big, 3, big2, 7), (ones, 0, 1, 0), (power, 2, 1, 2), (-big, 5, big2, 1)):
+        fa, fb = BF.from_int(a, -28 * pa), BF.from_int(b, -28 * pb)
+        assert calc("fadd", hx(a), pa, hx(b), pb) == xf(BF.add_or_sub(fa, fb, False))
+        assert calc("fsub", hx(a), pa, hx(b), pb) == xf(BF.add_or_sub(fa, fb, True))
+    for a, pa, k in ((big, 4, 10005), (ones, 1, 7), (-big2, 2, (1 << 50) + 3)):
+        neg, D, pp = BF.from_int(a, -28 * pa)
+        assert calc("fmuli", hx(a), pa, k) == xf((neg != (k < 0), D * abs(k), pp))
+    # (shift_right runs where the low limb has enough trailing zero bits: big << 20 by 13)
+    for a, pa, k in ((big, 4, -13), (big << 20, 4, -13), (big, 4, 13), (ones, 0, -1), (big2 | (1 << 200), 3, 27)):
+        assert calc("fshift", hx(a), pa, k) == xf(BF.shift(BF.from_int(a, -28 * pa), k))
