@@ -1051,3 +1051,48 @@ def test_garner_staging_chunked_reversed(n, L, nch):
                     v += planes[cc][coeff_pos(wl, n)]
             exp.append(v)
         assert got == exp, base
+
+
+def tile_twiddles(I, W, bpow, log_r, nt):
+    """TileTwiddles<I, LR, NT> (ntt_kernels.hip): thread tid builds entries k = tid + NT*j for
+    j < J = max(1, 2^LR / NT); the beta power of j >= 1 is bpow[LR - 1 - log2 NT - floor(log2 j)],
+    one scalar per j for the whole block; j = 0 takes bpow[LR - 1 - floor(log2 (tid | 1))]; with
+    no per-block root every power is the Montgomery form of 1."""
+    R = 1 << log_r
+    J = R // nt if R >= nt else 1
+    lnt, lj = nt.bit_length() - 1, J.bit_length() - 1
+    one = R32 % P[I]
+    tw = [None] * R
+    for tid in range(nt):
+        if R < nt and tid >= R:
+            continue
+        w = [W[tid + nt * j] for j in range(J)]
+        if bpow is None:
+            b0, bj = one, [one] * lj
+        else:
+            e0 = log_r - 1 - ((tid | 1).bit_length() - 1)
+            assert 0 <= e0 < log_r
+            b0 = bpow[e0]
+            assert all(0 <= log_r - 1 - lnt - e < log_r for e in range(lj))
+            bj = [bpow[log_r - 1 - lnt - e] for e in range(lj)]
+        for j in range(J):
+            assert tw[tid + nt * j] is None
+            tw[tid + nt * j] = mont(I, w[j], b0 if j == 0 else bj[j.bit_length() - 1])
+    return tw
+
+
+@pytest.mark.parametrize("I", [0, 1, 2])
+def test_tile_twiddles_match_the_entrywise_build(I):
+    """The two-step per-thread table build equals the entry-by-entry definition
+    tw[k] = W[k] * beta^(2^(LR - 1 - floor(log2 k))) for every entry the butterflies read (k >= 1),
+    for the strided passes (LR 1..8, 256 threads) and the block passes (LR 4..12, 2^(LR-4) threads),
+    with and without a per-block root."""
+    rng = random.Random(40 + I)
+    W = wtab(I, False)
+    beta = rng.randrange(2, P[I])
+    for log_r, nt in [(lr, 256) for lr in range(1, 9)] + [(lr, 1 << (lr - 4)) for lr in range(4, 13)]:
+        bpow = [to_mont(I, pow(beta, 1 << e, P[I])) for e in range(log_r)]
+        for bp in (bpow, None):
+            got = tile_twiddles(I, W, bp, log_r, nt)
+            want = build_tw(I, W, bp, log_r)
+            assert got[1:] == want[1:], (log_r, nt, bp is None)
