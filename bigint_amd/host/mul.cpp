@@ -5,6 +5,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <thread>
 
@@ -166,9 +167,10 @@ auto mul_digits(const Digits& a, const Digits& b, size_type output_precision,
         const size_type keep = output_precision + 1;
         note_reference_route((words64(sa) + words64(sb)) * 64);
         if ((words64(sa) + words64(sb)) * 64 > host_max_total_bits() && la + lb > keep) {
-            Digits top(keep);
+            // staging for the engine's output: every limb is written, so it is not zero-filled
+            const std::unique_ptr<uint32_t[]> top(new uint32_t[keep]);
             const auto t0 = std::chrono::steady_clock::now();
-            const size_type n = backend::multiply_top(sa.first(la), sb.first(lb), same, keep, top.data());
+            const size_type n = backend::multiply_top(sa.first(la), sb.first(lb), same, keep, top.get());
             g_backend_ns.fetch_add((uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
                                        std::chrono::steady_clock::now() - t0).count(),
                                    std::memory_order_relaxed);
@@ -181,7 +183,7 @@ auto mul_digits(const Digits& a, const Digits& b, size_type output_precision,
                 ++i;
                 ++off_r;
             }
-            Digits res(top.begin() + (std::ptrdiff_t)i, top.begin() + (std::ptrdiff_t)(i + (n - off_r)));
+            Digits res(top.get() + i, top.get() + (i + (n - off_r)));
             if (p_result_point_pos) *p_result_point_pos -= (int64_t)(off_a + off_b + off_r);
             return res;
         }
