@@ -89,8 +89,9 @@ static void small_mul_u64(Small& x, uint64_t v) {
 }
 
 // r = |a| |b| (r distinct from a and b; the caller sets the sign).  Column sums in 64 bits,
-// flushed every 256 products (each below 2^56).
-static void small_mul(Small& r, const Small& a, const Small& b) {
+// flushed every 256 products (each below 2^56); the column loop vectorizes.  Built twice, for
+// AVX2 and generic x86-64, and chosen once at run time.
+__attribute__((always_inline)) inline void small_mul_body(Small& r, const Small& a, const Small& b) {
     r.neg = false;
     if (!a.n || !b.n) {
         r.n = 0;
@@ -115,6 +116,20 @@ static void small_mul(Small& r, const Small& a, const Small& b) {
     r.d[na + nb - 1] = (uint32_t)carry;
     r.n = na + nb;
     while (r.n && r.d[r.n - 1] == 0) --r.n;
+}
+
+__attribute__((target("avx2"))) static void small_mul_avx2(Small& r, const Small& a, const Small& b) {
+    small_mul_body(r, a, b);
+}
+static void small_mul_generic(Small& r, const Small& a, const Small& b) { small_mul_body(r, a, b); }
+
+static void small_mul(Small& r, const Small& a, const Small& b) {
+    static const bool avx2 = [] {
+        __builtin_cpu_init();
+        return __builtin_cpu_supports("avx2") != 0;
+    }();
+    if (avx2) small_mul_avx2(r, a, b);
+    else small_mul_generic(r, a, b);
 }
 
 static int small_cmp_abs(const Small& a, const Small& b) {
@@ -173,18 +188,24 @@ static void leaf_term(uint64_t i, SmallTerm& t) {
     small_set(t.P, rr * p0, i % 2 != 0);
 }
 
+// Terms 1..r merged in blocks of kBlock consecutive terms (the last block may be shorter): the
+// first log2(kBlock) levels of the splitting without a BigInt per leaf or per merge.
+constexpr uint64_t kBlock = 64;
+constexpr int kBlockDepth = std::countr_zero(kBlock);
+
 // (P, Q, R) of terms lo+1 .. hi: the merge (P, Q, R) o (P', Q', R') = (P Q' + R P', Q Q', R R').
-static void block_term(uint64_t lo, uint64_t hi, SmallTerm& out) {
+// scratch holds two terms per level below this one (the recursion is at most kBlockDepth deep),
+// so a block's merges allocate nothing.
+static void block_term(uint64_t lo, uint64_t hi, SmallTerm& out, SmallTerm* scratch) {
     if (hi - lo == 1) {
         leaf_term(hi, out);
         return;
     }
     const uint64_t mid = lo + (hi - lo) / 2;
-    auto parts = std::make_unique<SmallTerm[]>(2);
-    block_term(lo, mid, parts[0]);
-    block_term(mid, hi, parts[1]);
-    const SmallTerm& l = parts[0];
-    const SmallTerm& h = parts[1];
+    SmallTerm& l = scratch[0];
+    SmallTerm& h = scratch[1];
+    block_term(lo, mid, l, scratch + 2);
+    block_term(mid, hi, h, scratch + 2);
     Small t1, t2;
     small_mul(t1, l.P, h.Q);
     t1.neg = l.P.neg && t1.n;
@@ -199,18 +220,15 @@ static BigInt to_bigint(const Small& x) {
     return BigInt::from_digits(Digits(x.d, x.d + x.n), x.neg);
 }
 
-// Terms 1..r merged in blocks of kBlock consecutive terms (the last block may be shorter): the
-// first log2(kBlock) levels of the splitting without a BigInt per leaf or per merge.
-constexpr uint64_t kBlock = 64;
-
 static std::vector<Term> leaves(size_type r) {
     const size_type nblocks = (r + kBlock - 1) / kBlock;
     return parallel_terms(nblocks, [r](size_type lo, size_type hi) {
         std::vector<Term> t;
         t.reserve(hi - lo);
         SmallTerm st;
+        const auto scratch = std::make_unique<SmallTerm[]>(2 * kBlockDepth);
         for (size_type b = lo; b < hi; ++b) {
-            block_term(b * kBlock, std::min<uint64_t>(r, (b + 1) * kBlock), st);
+            block_term(b * kBlock, std::min<uint64_t>(r, (b + 1) * kBlock), st, scratch.get());
             t.push_back(Term{to_bigint(st.P), to_bigint(st.Q), to_bigint(st.R)});
         }
         return t;
@@ -231,6 +249,14 @@ static std::pair<BigInt, BigInt> split(size_type r, int plan_mode, double& gpu_l
     if (verbose)
         std::cerr << "leaves: " << r << " terms in blocks of " << kBlock << ", ms "
                   << std::chrono::duration<double, std::milli>(clk::now() - tl).count() << "\n";
+    // From here on the example allocates and frees vectors of 50-270 MB many times (the
+    // downloads of P and Q, the Newton iterations, the final products, the decimal output).
+    // glibc serves blocks above its mmap threshold with fresh mappings, so every such vector paid
+    // for its pages again (about 0.6 s of page faults at 10^8 digits); keeping them in the heap
+    // lets freed blocks be reused.  (Set after the leaves: their eight threads allocating many
+    // small vectors run faster with the default policy.)
+    mallopt(M_MMAP_THRESHOLD, 1 << 30);
+    mallopt(M_TRIM_THRESHOLD, 1 << 30);
     int level = std::countr_zero(kBlock);  // the block merge did the levels below
     // The tree moves to the backend once every R has PI_DEVICE_MIN_R_LIMBS limbs: on the engine
     // at once (its batches beat host schoolbook on these levels); a backend without batching
@@ -365,13 +391,6 @@ int main(int argc, char** argv) {
     const auto t_split = clk::now();
     const double eng_split = mul::engine_time_ms();
 
-    // The Newton iterations, the final products and the decimal output allocate and free vectors
-    // of 50-270 MB many times.  glibc serves blocks above its mmap threshold with fresh mappings,
-    // so every such vector paid for its pages again (about 0.6 s of page faults at 10^8 digits);
-    // from here on they stay in the heap and freed blocks are reused.  (Set after the splitting:
-    // its eight threads allocating many small vectors run faster with the default policy.)
-    mallopt(M_MMAP_THRESHOLD, 1 << 30);
-    mallopt(M_TRIM_THRESHOLD, 1 << 30);
 
     const bool verbose = std::getenv("PI_LEVELS") != nullptr;
     auto phase = [&, t = clk::now()](const char* name) mutable {
