@@ -191,6 +191,17 @@ def batch_leg(hip, engine, L, pairs_count: int, steps: int) -> dict:
         stream.synchronize()
         dt = (time.perf_counter() - t0) / steps
         res[name] = {"ms": round(dt * 1e3, 3), "Gbit_s": round(sum(a + b for a, b in bits) / dt / 1e9, 2)}
+        if mode != engine.PLAN_FIXED:
+            # latency of one batch on its own (host preparation included, nothing to overlap it
+            # with); the timed loop above queues batches back to back, where a batch's host
+            # preparation overlaps the previous batch's device work
+            lat = []
+            for _ in range(3):
+                t1 = time.perf_counter()
+                engine.mul_batch_device(pairs, mode, stream.handle)
+                stream.synchronize()
+                lat.append(time.perf_counter() - t1)
+            res[name]["single_batch_ms"] = round(min(lat) * 1e3, 3)
     # the learned mode again with stream launches instead of one captured graph per batch
     engine.lib().bi_hip_set_graphs(0)
     try:

@@ -688,23 +688,52 @@ public:
                                ntiles + 1, st);
     }
 
-    // Batches as captured graphs: on by default (bi_hip_set_graphs); the last executable graph is
-    // kept and updated when the next batch has the same shape.
+    // Batches as captured graphs: on by default (bi_hip_set_graphs).  Two executable graphs are
+    // kept; each is updated in place when a later batch has its shape.  A batch call does not wait
+    // for the previous one: it takes the slot launched last if that launch is done (a caller that
+    // synchronises between batches keeps updating one graph), else the other slot, whose launch
+    // it waits for (its `done` event) before the update; batches queued back to back use the two
+    // in turn.
+    struct GraphSlot {
+        hipGraphExec_t exec = nullptr;
+        hipEvent_t done = nullptr;  // recorded after the slot's last launch
+        bool launched = false;
+    };
     bool graphs = true;
-    hipGraphExec_t graph_exec = nullptr;
-    hipStream_t graph_stream = nullptr;  // where graph_exec was last launched
+    GraphSlot gslots[2];
+    int gslot_last = 0;
 
-    // Drop the cached executable graph: called before any buffer it may reference is freed.
+    // Drop the executable graphs: called before any buffer they may reference is freed.
     void graph_reset() {
-        if (!graph_exec) return;
-        if (graph_stream) (void)hipStreamSynchronize(graph_stream);
-        (void)hipGraphExecDestroy(graph_exec);
-        graph_exec = nullptr;
-        graph_stream = nullptr;
+        for (GraphSlot& g : gslots) {
+            if (g.launched) (void)hipEventSynchronize(g.done);
+            g.launched = false;
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
+        }
     }
 
     template <class F>
     int launch_as_graph(F&& launch_all, hipStream_t st) {
+        int si = gslot_last;
+        if (gslots[si].launched) {
+            const hipError_t q = hipEventQuery(gslots[si].done);
+            if (q == hipErrorNotReady) {
+                (void)hipGetLastError();  // a query of a pending event is not an error
+                si ^= 1;
+            } else if (q != hipSuccess) {
+                return BI_HIP_EDEVICE;
+            }
+        }
+        GraphSlot& slot = gslots[si];
+        gslot_last = si;
+        if (!slot.done && hipEventCreateWithFlags(&slot.done, hipEventDisableTiming) != hipSuccess) {
+            slot.done = nullptr;
+            return BI_HIP_EDEVICE;
+        }
+        // the slot's previous launch must be done before its executable graph changes
+        if (slot.launched && hipEventSynchronize(slot.done) != hipSuccess) return BI_HIP_EDEVICE;
+        slot.launched = false;
         if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return BI_HIP_EDEVICE;
         const int rc = launch_all();
         hipGraph_t g = nullptr;
@@ -715,30 +744,70 @@ public:
         }
         if (end != hipSuccess || !g) return BI_HIP_EDEVICE;
         bool ok = false;
-        if (graph_exec) {
-            // the previous launch of this executable graph must be done before it changes
-            if (graph_stream && graph_stream != st && hipStreamSynchronize(graph_stream) != hipSuccess) {
-                (void)hipGraphDestroy(g);
-                return BI_HIP_EDEVICE;
-            }
+        if (slot.exec) {
             hipGraphNode_t err_node = nullptr;
             hipGraphExecUpdateResult res{};
-            ok = hipGraphExecUpdate(graph_exec, g, &err_node, &res) == hipSuccess &&
+            ok = hipGraphExecUpdate(slot.exec, g, &err_node, &res) == hipSuccess &&
                  res == hipGraphExecUpdateSuccess;
             if (!ok) {  // another shape: a new executable graph (and no stale error left behind)
                 (void)hipGetLastError();
-                (void)hipGraphExecDestroy(graph_exec);
-                graph_exec = nullptr;
+                (void)hipGraphExecDestroy(slot.exec);
+                slot.exec = nullptr;
             }
         }
-        if (!ok && hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
-            graph_exec = nullptr;
+        if (!ok && hipGraphInstantiate(&slot.exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            slot.exec = nullptr;
             (void)hipGraphDestroy(g);
             return BI_HIP_EDEVICE;
         }
         (void)hipGraphDestroy(g);
-        graph_stream = st;
-        return hipGraphLaunch(graph_exec, st) == hipSuccess ? BI_HIP_OK : BI_HIP_EDEVICE;
+        if (hipGraphLaunch(slot.exec, st) != hipSuccess || hipEventRecord(slot.done, st) != hipSuccess)
+            return BI_HIP_EDEVICE;
+        slot.launched = true;
+        return BI_HIP_OK;
+    }
+
+    // Pinned host staging of the batch descriptors, two buffers used in turn: the descriptors go
+    // to the device with a copy ordered on the caller's stream after the earlier work that reads
+    // the device buffer, so the host does not wait for that work; a staging buffer is rewritten
+    // only once its copy of two calls earlier is done.
+    struct DescStage {
+        uint8_t* host = nullptr;
+        size_t bytes = 0;
+        hipEvent_t copied = nullptr;
+        bool pending = false;
+    };
+    DescStage dstage[2];
+    int dstage_next = 0;
+
+    // Queue the copy of `bytes` descriptor bytes from src into the device buffer on st (ordered
+    // after every earlier use of the shared buffers).
+    int upload_desc(const uint8_t* src, size_t bytes, hipStream_t st) {
+        DescStage& d = dstage[dstage_next];
+        dstage_next ^= 1;
+        if (!d.copied && hipEventCreateWithFlags(&d.copied, hipEventDisableTiming) != hipSuccess) {
+            d.copied = nullptr;
+            return BI_HIP_EDEVICE;
+        }
+        if (d.pending && hipEventSynchronize(d.copied) != hipSuccess) return BI_HIP_EDEVICE;
+        d.pending = false;
+        if (bytes > d.bytes) {
+            if (d.host) (void)hipHostFree(d.host);
+            d.host = nullptr;
+            d.bytes = 0;
+            if (hipHostMalloc((void**)&d.host, bytes, hipHostMallocDefault) != hipSuccess) {
+                d.host = nullptr;
+                return BI_HIP_ENOMEM;
+            }
+            d.bytes = bytes;
+        }
+        std::memcpy(d.host, src, bytes);
+        if (int rc = order_shared(st)) return rc;
+        if (hipMemcpyAsync(desc, d.host, bytes, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipEventRecord(d.copied, st) != hipSuccess)
+            return BI_HIP_EDEVICE;
+        d.pending = true;
+        return BI_HIP_OK;
     }
 
     // descriptor buffer for batched launches (device), grow-only
@@ -1252,10 +1321,9 @@ public:
             L.status = status;
             launches.push_back(L);
         }
-        // the descriptors of earlier calls may still be read, on this stream or another
-        if (wait_shared() || hipStreamSynchronize(st) != hipSuccess) return BI_HIP_EDEVICE;
-        if (hipMemcpy(desc, hdesc.data(), off, hipMemcpyHostToDevice) != hipSuccess)
-            return BI_HIP_EDEVICE;
+        // the descriptors of earlier calls may still be read, on this stream or another: the new
+        // ones are copied in after that work, in stream order
+        if (int rc = upload_desc(hdesc.data(), off, st)) return rc;
         auto launch_all = [&]() -> int {
             for (auto& L : launches) {
                 const auto* pj = reinterpret_cast<const PassJob*>(desc + L.pj_off);

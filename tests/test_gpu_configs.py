@@ -127,3 +127,51 @@ def test_products_on_two_streams_without_host_sync():
     for t in bufs:
         for x in t:
             x.free()
+
+
+def test_back_to_back_batches_without_host_sync():
+    """Grouped batches of different shapes queued back to back, on one stream and on two streams
+    in turn, with no host synchronisation between the calls: a batch call does not wait for the
+    earlier ones, its descriptors reach the device with a copy ordered after the earlier batches
+    that read the descriptor buffer, and the two executable graphs are used in turn (updated in
+    place or instantiated anew as the shapes change).  Every product must equal the oracle's."""
+    rng = np.random.default_rng(5)
+    batches = []
+    for bi in range(4):
+        pairs = []
+        for k in range(24 + 8 * bi):
+            ba, bb = int(2 ** rng.uniform(12, 19)), int(2 ** rng.uniform(12, 19))
+            pairs.append((L.random_operand(ba, 1000 * bi + 2 * k), L.random_operand(bb, 1000 * bi + 2 * k + 1)))
+        batches.append(pairs)
+    exp = [[oracle.mul_limbs(a, b) for a, b in pairs] for pairs in batches]
+    bufs = [[(hip.Buffer.from_array(a), hip.Buffer.from_array(b), hip.Buffer(4 * (a.size + b.size)))
+             for a, b in pairs] for pairs in batches]
+    sx, sy = hip.Stream(), hip.Stream()
+
+    def queue(i, st, mode):
+        engine.mul_batch_device([(da.ptr, a.size, db.ptr, b.size, do.ptr, a.size + b.size)
+                                 for (a, b), (da, db, do) in zip(batches[i], bufs[i])], mode, st.handle)
+
+    # once each, synchronised: the workspace and descriptor buffers reach their largest size here
+    for i in range(4):
+        queue(i, sx, engine.PLAN_TABLE)
+        sx.synchronize()
+    orders = [[(0, sx), (1, sx), (2, sx), (3, sx), (0, sx), (2, sx)],
+              [(1, sx), (3, sy), (0, sx), (2, sy), (1, sy), (3, sx)]]
+    for rep, order in enumerate(orders):
+        for t in bufs:
+            for _, _, do in t:
+                do.zero()
+        hip.synchronize()
+        for k, (i, st) in enumerate(order):
+            queue(i, st, engine.PLAN_LEARNED if (k + rep) % 2 else engine.PLAN_TABLE)
+        sx.synchronize()
+        sy.synchronize()
+        for i, pairs in enumerate(batches):
+            for j, (a, b) in enumerate(pairs):
+                got = bufs[i][j][2].download(np.uint32, a.size + b.size)
+                assert np.array_equal(L.strip(got), exp[i][j]), (rep, i, j)
+    for t in bufs:
+        for x in t:
+            for y in x:
+                y.free()
