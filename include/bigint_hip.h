@@ -94,9 +94,8 @@ int bi_hip_mul_top(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, i
 
 /* Device-resident variant: a, b, out are device pointers; exactly la + lb limbs are written
  * (the top one may be zero).  Asynchronous on `stream` (a hipStream_t; NULL = the engine's
- * stream), except that a chunked plan waits for earlier work on the stream before it reuses the
- * engine's descriptor buffer.  Both single-product calls use the engine's plan choice (the
- * installed table, else the built-in cost heuristic). */
+ * stream); a chunked plan goes through the batch path (below).  Both single-product calls use the
+ * engine's plan choice (the installed table, else the built-in cost heuristic). */
 int bi_hip_mul_device(const uint32_t* d_a, size_t la, const uint32_t* d_b, size_t lb,
                       int is_square, uint32_t* d_out, size_t out_cap, void* stream);
 
@@ -160,13 +159,16 @@ int bi_hip_copy_to_host(void* const* h_dst, const size_t* bytes, size_t count, c
 
 /* Device-resident batch, asynchronous on `stream` (NULL = the engine's stream).
  * BI_HIP_PLAN_FIXED runs one launch sequence per product; the other modes group products of
- * equal transform length into one launch sequence per group.  The call waits for earlier work
- * on the stream before it reuses its descriptor buffer. */
+ * equal transform length into one launch sequence per group.  The call does not wait for
+ * earlier batches: its descriptors are copied to the device in stream order after the earlier
+ * work that reads the descriptor buffer, so batches queued back to back overlap one's host
+ * preparation with the previous one's device work.  The host may wait for a batch queued two
+ * calls earlier (two staging buffers and two executable graphs are used in turn). */
 int bi_hip_mul_batch_device(const bi_hip_dpair* pairs, size_t count, int plan_mode, void* stream);
 
-/* Grouped batches go to the GPU as one captured hipGraph per call (kernels only), updated in
- * place when the next batch has the same launch shape: enable != 0 (the default) or plain stream
- * launches (0).  No reference counterpart (its product-tree levels are loops of operator*,
+/* Grouped batches go to the GPU as one captured hipGraph per call (kernels only); two
+ * executable graphs are kept and updated in place when a later batch has the same launch shape:
+ * enable != 0 (the default) or plain stream launches (0).  No reference counterpart (its product-tree levels are loops of operator*,
  * examples/pi.cpp:45-58). */
 int bi_hip_set_graphs(int enable);
 
