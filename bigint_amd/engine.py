@@ -188,9 +188,17 @@ def mul_batch(pairs: list[tuple[np.ndarray, np.ndarray | None]], plan_mode: int 
 
 def mul_batch_device(pairs: list[tuple[int, int, int, int, int, int]], plan_mode: int = PLAN_TABLE,
                      stream: int = 0) -> None:
-    """Device batch: each pair is (d_a, la, d_b or 0, lb, d_out, out_cap) with raw device pointers."""
-    arr = (DPair * len(pairs))(*[DPair(a, la, b or None, lb, o, cap) for a, la, b, lb, o, cap in pairs])
-    rc = lib().bi_hip_mul_batch_device(arr, len(pairs), plan_mode, stream or None)
+    """Device batch: each pair is (d_a, la, d_b or 0, lb, d_out, out_cap) with raw device pointers.
+    The pairs go to the engine as one array of six 64-bit words each (bi_hip_dpair's layout: three
+    pointers and three sizes, a null b meaning a square)."""
+    try:
+        arr = np.array(pairs, dtype=np.uint64).reshape(-1, 6)
+    except TypeError:  # a None for d_b
+        arr = np.array([(a, la, b or 0, lb, o, cap) for a, la, b, lb, o, cap in pairs],
+                       dtype=np.uint64).reshape(-1, 6)
+    assert ctypes.sizeof(DPair) == 48
+    rc = lib().bi_hip_mul_batch_device(arr.ctypes.data_as(ctypes.POINTER(DPair)), len(arr), plan_mode,
+                                       stream or None)
     if rc:
         raise EngineError(rc, "bi_hip_mul_batch_device")
 
