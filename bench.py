@@ -450,7 +450,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+              "(a product-tree level would shard by pairs, one huge product by four-step column "
+              "blocks with an all-to-all per transform; neither is built)", file=sys.stderr)
         return 2
 
     if args.cpu_sweep:
