@@ -783,6 +783,7 @@ public:
     // Queue the copy of `bytes` descriptor bytes from src into the device buffer on st (ordered
     // after every earlier use of the shared buffers).
     int upload_desc(const uint8_t* src, size_t bytes, hipStream_t st) {
+        if (bytes == 0) return order_shared(st);  // no grouped launches (products in pieces only)
         DescStage& d = dstage[dstage_next];
         dstage_next ^= 1;
         if (!d.copied && hipEventCreateWithFlags(&d.copied, hipEventDisableTiming) != hipSuccess) {
