@@ -757,13 +757,14 @@ __device__ __forceinline__ uint32_t coeff_pos(uint32_t j, uint32_t n) { return (
 // Words wx and wx - 1 (wx even) of one plane of n inverse-transform outputs, words outside
 // [0, n) reading as zero.  Word w sits at position (n - w) mod n, so for 2 <= wx <= n - 2 the two
 // words are the aligned position pair (n - wx, n - wx + 1); word 0 is at position 0 and word n - 1
-// at position 1.
+// at position 1.  Every wx does one aligned load inside the plane (the pair at position 0 where
+// wx is not in [2, n - 2]) and selects the words it needs, with no branch around the load, so a
+// caller's loads issue back to back and are waited for together; a branch per load makes the
+// compiler wait for each load before it issues the next.
 __device__ __forceinline__ u32x2 load_pair_rev(const uint32_t* __restrict__ yp, uint32_t n, int64_t wx) {
-    if (wx >= 2 && wx <= (int64_t)n - 2) return *gmem(reinterpret_cast<const u32x2*>(yp + (n - (uint32_t)wx)));
-    u32x2 r{0u, 0u};
-    if (wx == 0) r.x = gmem(yp)[0];
-    else if (wx == (int64_t)n) r.y = gmem(yp)[1];
-    return r;
+    const bool mid = wx >= 2 && wx <= (int64_t)n - 2;
+    const u32x2 v = *gmem(reinterpret_cast<const u32x2*>(yp + (mid ? n - (uint32_t)wx : 0u)));
+    return u32x2{mid || wx == 0 ? v.x : 0u, mid || wx == (int64_t)n ? v.y : 0u};
 }
 
 // The coefficient that wrapped from index n onto index 0: product of the operands' top words.
@@ -856,10 +857,13 @@ __device__ __forceinline__ void garner_tile(const GarnerArgs& a, const GarnerJob
                 const uint32_t* yp = jb.y + (size_t)p * n + (n - base);
 #pragma unroll
                 for (uint32_t k = 0; k < kPairLoads; ++k) {
+                    // unconditional loads, all in the plane (lanes past the tile's pairs reload
+                    // pair 0; ie = 1792 of the lanes without a halo pair is the tile's last pair),
+                    // so they issue back to back; only the wanted values are staged
                     const uint32_t i = k * kGThreads + tid;
-                    if (i < kNPairs) v[p][k] = *gmem(reinterpret_cast<const u32x2*>(yp - 2u * i));
+                    v[p][k] = *gmem(reinterpret_cast<const u32x2*>(yp - 2u * (i < kNPairs ? i : 0u)));
                 }
-                if (has_end) h[p] = *gmem(reinterpret_cast<const u32x2*>(yp - 2 * ie));
+                h[p] = *gmem(reinterpret_cast<const u32x2*>(yp - 2 * ie));
             }
         } else if constexpr (!CHUNKED) {
 #pragma unroll
@@ -867,10 +871,11 @@ __device__ __forceinline__ void garner_tile(const GarnerArgs& a, const GarnerJob
                 const uint32_t* yp = jb.y + (size_t)p * n;
 #pragma unroll
                 for (uint32_t k = 0; k < kPairLoads; ++k) {
+                    // lanes past the tile's pairs read zeros (wx = -2), with no branch around the load
                     const uint32_t i = k * kGThreads + tid;
-                    if (i < kNPairs) v[p][k] = load_pair_rev(yp, n, (int64_t)base + 2 * (int64_t)i);
+                    v[p][k] = load_pair_rev(yp, n, i < kNPairs ? (int64_t)base + 2 * (int64_t)i : -2);
                 }
-                if (has_end) h[p] = load_pair_rev(yp, n, (int64_t)base + 2 * ie);
+                h[p] = load_pair_rev(yp, n, has_end ? (int64_t)base + 2 * ie : -2);
             }
         } else {
             // chunk cc covers words [cc L, cc L + n); those reaching [lo, hi] are
@@ -884,21 +889,24 @@ __device__ __forceinline__ void garner_tile(const GarnerArgs& a, const GarnerJob
             for (uint32_t cc = c_lo; cc <= c_hi; ++cc) {
                 const uint32_t* yc = jb.y + (size_t)cc * jb.chunk_stride;
                 const int64_t c0 = (int64_t)cc * L;  // first word of chunk cc
+                // the chunk's loads first, all in flight together, then the sums
+                u32x2 x[3][kPairLoads], xh[3];
 #pragma unroll
                 for (uint32_t p = 0; p < 3u; ++p) {
                     const uint32_t* yp = yc + (size_t)p * n;
 #pragma unroll
                     for (uint32_t k = 0; k < kPairLoads; ++k) {
                         const uint32_t i = k * kGThreads + tid;
-                        if (i < kNPairs) {
-                            const u32x2 x = load_pair_rev(yp, n, (int64_t)base + 2 * (int64_t)i - c0);
-                            v[p][k] = u32x2{add_residue(p, v[p][k].x, x.x), add_residue(p, v[p][k].y, x.y)};
-                        }
+                        x[p][k] = load_pair_rev(yp, n, i < kNPairs ? (int64_t)base + 2 * (int64_t)i - c0 : -2);
                     }
-                    if (has_end) {
-                        const u32x2 x = load_pair_rev(yp, n, (int64_t)base + 2 * ie - c0);
-                        h[p] = u32x2{add_residue(p, h[p].x, x.x), add_residue(p, h[p].y, x.y)};
-                    }
+                    xh[p] = load_pair_rev(yp, n, has_end ? (int64_t)base + 2 * ie - c0 : -2);
+                }
+#pragma unroll
+                for (uint32_t p = 0; p < 3u; ++p) {
+#pragma unroll
+                    for (uint32_t k = 0; k < kPairLoads; ++k)
+                        v[p][k] = u32x2{add_residue(p, v[p][k].x, x[p][k].x), add_residue(p, v[p][k].y, x[p][k].y)};
+                    h[p] = u32x2{add_residue(p, h[p].x, xh[p].x), add_residue(p, h[p].y, xh[p].y)};
                 }
             }
         }
