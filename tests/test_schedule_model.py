@@ -352,18 +352,15 @@ def coeff_pos(j, n):
 
 
 def load_pair_rev(plane, n, wx):
-    """Kernel load_pair_rev: words wx and wx - 1 (wx even) of a plane, zero outside [0, n); every
-    read index is bounds-checked and the pair load must be an aligned position pair."""
+    """Kernel load_pair_rev: words wx and wx - 1 (wx even) of a plane, zero outside [0, n).  One
+    unconditional load of the aligned position pair at n - wx for 2 <= wx <= n - 2, else at 0,
+    then selects; the load index is bounds-checked whatever wx is."""
     assert wx % 2 == 0
-    if 2 <= wx <= n - 2:
-        q = n - wx
-        assert q % 2 == 0 and 0 <= q and q + 1 < n
-        return plane[q], plane[q + 1]
-    if wx == 0:
-        return plane[0], 0
-    if wx == n:
-        return 0, plane[1]
-    return 0, 0
+    mid = 2 <= wx <= n - 2
+    q = n - wx if mid else 0
+    assert q % 2 == 0 and 0 <= q and q + 1 < n
+    x, y = plane[q], plane[q + 1]
+    return (x if mid or wx == 0 else 0), (y if mid or wx == n else 0)
 
 
 def garner_stage(plane, n, base, chunks=None):
@@ -385,6 +382,8 @@ def garner_stage(plane, n, base, chunks=None):
         for k in range(4):
             i = k * GT + tid
             if i >= KWT // 2:
+                for pl, _ in srcs:  # lanes past the tile's pairs load at wx = -2 (zeros)
+                    assert load_pair_rev(pl, n, -2) == (0, 0)
                 continue
             x, y_ = pair(i)
             t = 2 * i + 3
@@ -1025,6 +1024,39 @@ def test_first_pass_half_skip_reads_only_zero_rows(nlimbs, off, cnt, log_n, log_
         for u in [0, 1, P[I] - 1]:
             for w in [to_mont(I, 1), to_mont(I, 12345)]:
                 assert (u + k_mont(I, 0, w)) % P[I] == u and (u - k_mont(I, 0, w)) % P[I] == u
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 64, 4096])
+def test_pair_load_matches_word_definition(n):
+    """The branch-free pair load returns words wx and wx - 1 of the coefficient sequence (word w at
+    position (n - w) mod n, zero outside [0, n)) for every even wx around and inside the plane,
+    with its one load inside the plane for each of them."""
+    plane = [1000 + q for q in range(n)]
+
+    def word(w):
+        return plane[coeff_pos(w, n)] if 0 <= w < n else 0
+    for wx in range(-8, n + 10, 2):
+        assert load_pair_rev(plane, n, wx) == (word(wx), word(wx - 1)), wx
+
+
+@pytest.mark.parametrize("n", [8192, 16384, 1 << 16])
+def test_interior_garner_loads_stay_in_the_plane(n):
+    """Interior Garner tiles load unconditionally: lanes past the tile's 1792 pairs reload pair 0 and
+    lanes without a halo pair load the tile's last pair (ie = 1792).  Every such position, for every
+    lane of every interior tile, is an aligned pair inside the plane."""
+    npairs = KWT // 2
+    for base in range(0, n, KWT):
+        if not (base >= 6 and base + KWT + 2 <= n):
+            continue
+        top = n - base
+        for tid in range(GT):
+            for k in range(4):
+                i = k * GT + tid
+                q = top - 2 * (i if i < npairs else 0)
+                assert q % 2 == 0 and 0 <= q and q + 1 < n
+            ie = -2 if tid == 0 else (-1 if tid == 1 else npairs)
+            q = top - 2 * ie
+            assert q % 2 == 0 and 0 <= q and q + 1 < n
 
 
 @pytest.mark.parametrize("n,L,nch", [(16, 8, 5), (64, 32, 3), (4096, 2048, 4), (8192, 3584 * 2, 3),
